@@ -1,0 +1,1137 @@
+/*
+ * es_oracle.c — CPU ORACLE (test infrastructure only).
+ *
+ * A plain-C restatement of the ElasticScattering reference CPU path, used by
+ * tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py to
+ * check the HIP implementation.  Nothing in the product links or calls it.
+ * Each function cites the reference file:line it follows, operation by
+ * operation (the same evaluation order; build with -ffp-contract=off).
+ *
+ * Two builds:
+ *   liboracle_libm.so  — glibc sin/cos/asin/exp (the reference calls the C
+ *                        runtime's libm; this build is compared bit for bit
+ *                        with the reference headers compiled under oracle/_ref)
+ *   liboracle_esm.so   — the project's FP64 math (es_math.h), which is also
+ *                        what the GPU kernels call; the GPU lifetimes are
+ *                        compared bit for bit with this build.
+ *
+ * Deviations kept explicit (each is a documented decision, not a fix):
+ *   - pow(v, 2) is evaluated as v*v (the correctly rounded square).
+ *   - The impurity generator is es_rng.h (the reference RNG is
+ *     implementation-defined).
+ *   - An optional explicit cells_per_row and an explicit incoherent tau
+ *     extend the reference settings; at their defaults the reference rule
+ *     holds.
+ *   - A trace that has not terminated after 4*cpr^2 + 16 cells is stopped
+ *     and counted in trace_cap_hits (the reference would loop forever; a
+ *     circle enters each square at most four times, so a finite trace never
+ *     reaches the cap).
+ *
+ * Pinning of each function is listed in the test files that call it.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#include "../elasticscattering_amd/csrc/es_rng.h"
+
+#ifdef ORACLE_USE_ESM
+#include "../elasticscattering_amd/csrc/es_math.h"
+#define O_SIN esm_sin
+#define O_COS esm_cos
+#define O_ASIN esm_asin
+#define O_EXP esm_exp
+#else
+#define O_SIN sin
+#define O_COS cos
+#define O_ASIN asin
+#define O_EXP exp
+#endif
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* constants.h:23-38 */
+#define C_M0 9.10938188e-31
+#define C_M (5.0 * C_M0)
+#define C_E 1.60217646e-19
+#define C_HBAR 1.05457148e-34
+#define C_C1 1.15e-9
+#define C_KB 1.380649e-23
+#define C_PI 3.141592653589793238463
+#define C_PI2 6.283185307179586
+#define C_HALF_PI 1.570796326794896619231
+#define C_INF (DBL_MAX)
+
+typedef struct { double x, y; } v2;
+typedef struct { int x, y; } v2i;
+
+/* details.h:31-65 */
+typedef struct {
+    v2 center;
+    double radius, radius_squared;
+    double bound_time, bound_phi;
+    double particle_angle;
+    int clockwise;
+} orc_orbit;
+
+/* details.h:67-73 */
+typedef struct {
+    double phi;
+    v2 position;
+    double angular_speed;
+    orc_orbit orbit;
+} orc_particle;
+
+/* cell_grid.h:31-46 */
+typedef struct {
+    v2 position;
+    double incident_angle;
+    double dphi;
+    v2i entering_cell;
+} orc_intersection;
+
+/* settings.h:36-45 */
+typedef struct {
+    double spawn_region_start;
+    double spawn_region_size;
+    double cell_size;
+    int cells_per_row;
+    double impurity_radius;
+} orc_impurity_settings;
+
+/* settings.h:47-58 */
+typedef struct {
+    double particle_speed;
+    double angular_speed;
+    double alpha;
+    int is_clockwise;
+    int is_coherent;
+    double phi_step_size;
+    double phi_start;
+} orc_particle_settings;
+
+/* particle_metrics.h:22-44, widened to 64 bits. */
+typedef struct {
+    int64_t cells_passed;
+    int64_t impurities_tested;
+    int64_t particles_inside_impurity;
+    int64_t particles_escaped;
+    int64_t particles_at_bound;
+    int64_t max_impurities_tested;
+    int64_t max_cells_passed;
+    int64_t trace_cap_hits;
+} orc_metrics;
+
+static inline v2 mk2(double x, double y) { v2 r; r.x = x; r.y = y; return r; }
+static inline v2i mk2i(int x, int y) { v2i r; r.x = x; r.y = y; return r; }
+static inline double sq(double v) { return v * v; }  /* pow(v, 2) */
+
+/* ---------------------------------------------------------------- details.h */
+
+/* details.h:79-83 */
+EXPORT double orc_smod(double a, double b)
+{
+    return a - b * floor(a / b);
+}
+
+/* details.h:85-90 */
+static double angle_velocity(double pos_angle, int clockwise)
+{
+    double offset = clockwise ? C_HALF_PI : -C_HALF_PI;
+    return orc_smod(pos_angle + offset, C_PI2);
+}
+
+/* details.h:93-107 */
+static double get_angle(v2 pos, const orc_orbit* orbit)
+{
+    const double y_dist = pos.y - orbit->center.y;
+    const int left_side = pos.x < orbit->center.x;
+    double angle = 0;
+    if (fabs(y_dist) > (1e-10 * orbit->radius)) {
+        angle = O_ASIN(y_dist / orbit->radius);
+        angle = left_side ? C_PI - angle : angle;
+    } else {
+        angle = left_side ? C_PI : 0;
+    }
+    return angle_velocity(angle, orbit->clockwise);
+}
+
+/* details.h:111-116 */
+static double get_cross_angle(double p, double q, int clockwise)
+{
+    const double g = clockwise ? (p - q) : (q - p);
+    return orc_smod(g, C_PI2);
+}
+
+/* details.h:119-136 */
+static int angle_in_range(double phi, v2 phi_range, int clockwise)
+{
+    if (fabs(phi_range.x - phi_range.y) < 1e-10) return 1;
+    double low = clockwise ? phi_range.y : phi_range.x;
+    double high = clockwise ? phi_range.x : phi_range.y;
+    high = (high < low) ? (high + C_PI2) : high;
+    return (low < phi && phi < high) || (low < (phi + C_PI2) && (phi + C_PI2) < high);
+}
+
+/* details.h:141-151 (dot(q, q) of v2.h:127 reduces to q.x*q.x + q.y*q.y) */
+static int circles_cross(const orc_orbit* orbit, v2 p2, double r2)
+{
+    const v2 q = mk2(orbit->center.x - p2.x, orbit->center.y - p2.y);
+    const double dist_squared = q.x * q.x + q.y * q.y;
+    const double r_add = orbit->radius + r2;
+    const double r_min = orbit->radius - r2;
+    return (dist_squared < r_add * r_add) && (dist_squared > r_min * r_min);
+}
+
+/* details.h:153-174 */
+static void get_cross_points(const orc_orbit* orbit, v2 p2, double r2, double out[4])
+{
+    const v2 q = mk2(orbit->center.x - p2.x, orbit->center.y - p2.y);
+    const double dist_squared = q.x * q.x + q.y * q.y;
+    const double dist = sqrt(dist_squared);
+    const double xs = (dist_squared + orbit->radius_squared - r2 * r2) / (2.0 * dist);
+    const double ys = sqrt(orbit->radius_squared - xs * xs);
+    const v2 u = mk2((p2.x - orbit->center.x) / dist, (p2.y - orbit->center.y) / dist);
+    out[0] = orbit->center.x + u.x * xs + u.y * ys;
+    out[1] = orbit->center.y + u.y * xs + -u.x * ys;
+    out[2] = orbit->center.x + u.x * xs + -u.y * ys;
+    out[3] = orbit->center.y + u.y * xs + u.x * ys;
+}
+
+/* details.h:182-206; min() is the reference macro ((a) < (b)) ? (a) : (b). */
+static double get_first_cross_time(const orc_orbit* orbit, v2 ip, double ir, double w, v2 valid_range)
+{
+    double cp[4];
+    get_cross_points(orbit, ip, ir, cp);
+    const v2 p1 = mk2(cp[0], cp[1]);
+    const v2 p2 = mk2(cp[2], cp[3]);
+    const double phi1 = get_angle(p1, orbit);
+    const double phi2 = get_angle(p2, orbit);
+    double traversal_time = C_INF;
+    if (angle_in_range(phi1, valid_range, orbit->clockwise)) {
+        const double t = get_cross_angle(orbit->particle_angle, phi1, orbit->clockwise) / w;
+        traversal_time = (traversal_time < t) ? traversal_time : t;
+    }
+    if (angle_in_range(phi2, valid_range, orbit->clockwise)) {
+        const double t = get_cross_angle(orbit->particle_angle, phi2, orbit->clockwise) / w;
+        traversal_time = (traversal_time < t) ? traversal_time : t;
+    }
+    return traversal_time;
+}
+
+/* details.h:208-213 */
+static int inside_impurity(v2 pos, v2 impurity, double impurity_radius)
+{
+    const v2 d = mk2(pos.x - impurity.x, pos.y - impurity.y);
+    return (impurity_radius * impurity_radius) > (d.x * d.x + d.y * d.y);
+}
+
+/* details.h:219-228 */
+EXPORT double orc_bound_time(double phi, double alpha, double w, int coherent, int is_electron, int is_future)
+{
+    if (coherent) return C_INF;
+    const double remaining = orc_smod(phi + alpha, C_HALF_PI);
+    const double dphi = ((is_electron != 0) != (is_future != 0)) ? remaining : (2.0 * alpha - remaining);
+    return dphi / w;
+}
+
+/* details.h:230-241 */
+EXPORT double orc_bound_angle(double phi, double alpha, int clockwise)
+{
+    const double v = floor((phi + alpha) / C_HALF_PI) * C_HALF_PI;
+    const double bound1 = v - alpha;
+    const double bound2 = v + alpha;
+    const double dangle1 = get_cross_angle(phi, bound1, clockwise);
+    const double dangle2 = get_cross_angle(phi, bound2, clockwise);
+    return (dangle1 < dangle2) ? bound1 : bound2;
+}
+
+/* details.h:243-250 */
+static v2 orbit_center(v2 p, v2 velocity, double radius, double vf, int is_electron)
+{
+    v2 shift = mk2(velocity.y, -velocity.x);
+    shift = mk2(shift.x * radius, shift.y * radius);
+    shift = mk2(shift.x / vf, shift.y / vf);
+    return is_electron ? mk2(p.x - shift.x, p.y - shift.y) : mk2(p.x + shift.x, p.y + shift.y);
+}
+
+/* -------------------------------------------------------------- cell_grid.h */
+
+/* cell_grid.h:48-50 */
+static int get_index(v2i p, int cells_per_row) { return p.y * cells_per_row + p.x; }
+
+/* cell_grid.h:52-58 */
+static v2i get_cell(v2 pos, double start, double size, int cells_per_row)
+{
+    return mk2i((int)((pos.x - start) / size * (double)(cells_per_row)),
+                (int)((pos.y - start) / size * (double)(cells_per_row)));
+}
+
+/* cell_grid.h:60-63 */
+static v2 to_world(v2i c, double start, double cell_size)
+{
+    return mk2(start + c.x * cell_size, start + c.y * cell_size);
+}
+
+/* cell_grid.h:65-67 */
+static int within_bounds(v2i p, int cells_per_row)
+{
+    return (p.x >= 0 && p.x < cells_per_row) && (p.y >= 0 && p.y < cells_per_row);
+}
+
+/* cell_grid.h:69-72 */
+static int point_in_segment(double point, double l0, double l1)
+{
+    return (point > l0 && point < l1) || (point < l0 && point > l1);
+}
+
+/* cell_grid.h:74-77 */
+static int different_point(v2 p1, v2 p2, double L)
+{
+    return fabs(p1.x - p2.x) > 1e-6 * L || fabs(p1.y - p2.y) > 1e-6 * L;
+}
+
+/* cell_grid.h:80-89 (the Update_Best_Intersect macro) */
+static void update_best_intersect(const orc_orbit* orbit, v2 pt, v2i next_cell, double L,
+                                  const orc_intersection* last, orc_intersection* closest)
+{
+    const double incident_angle = get_angle(pt, orbit);
+    const double dphi = get_cross_angle(last->incident_angle, incident_angle, orbit->clockwise);
+    if (dphi < closest->dphi && different_point(pt, last->position, L)) {
+        closest->position = pt;
+        closest->entering_cell = next_cell;
+        closest->incident_angle = incident_angle;
+        closest->dphi = dphi;
+    }
+}
+
+/* cell_grid.h:91-129 */
+static int update_first_boundary_intersect(const orc_orbit* orbit, v2i cell_offset, v2 p1, v2 p2, double L,
+                                           const orc_intersection* last, orc_intersection* closest)
+{
+    const v2 u = mk2((p2.x - p1.x) / L, (p2.y - p1.y) / L);
+    const double projection_distance = u.x * (orbit->center.x - p1.x) + u.y * (orbit->center.y - p1.y);
+    const v2 proj = mk2(p1.x + u.x * projection_distance, p1.y + u.y * projection_distance);
+    const double proj_circle_distance_sq = sq(proj.x - orbit->center.x) + sq(proj.y - orbit->center.y);
+    if (proj_circle_distance_sq >= orbit->radius_squared) return 0;
+
+    const double s = sqrt(orbit->radius_squared - proj_circle_distance_sq);
+    const v2 to_edge = mk2(u.x * s, u.y * s);
+    const v2 i1 = mk2(proj.x + to_edge.x, proj.y + to_edge.y);
+    const v2 i2 = mk2(proj.x - to_edge.x, proj.y - to_edge.y);
+
+    const int horizontal_line = fabs(u.x) > fabs(u.y);
+    const int i1_valid = horizontal_line ? point_in_segment(i1.x, p1.x, p2.x) : point_in_segment(i1.y, p1.y, p2.y);
+    const int i2_valid = horizontal_line ? point_in_segment(i2.x, p1.x, p2.x) : point_in_segment(i2.y, p1.y, p2.y);
+
+    const v2i next_cell = mk2i(last->entering_cell.x + cell_offset.x, last->entering_cell.y + cell_offset.y);
+    if (i1_valid) update_best_intersect(orbit, i1, next_cell, L, last, closest);
+    if (i2_valid) update_best_intersect(orbit, i2, next_cell, L, last, closest);
+    return i1_valid || i2_valid;
+}
+
+/* cell_grid.h:132-181 */
+static int get_next_cell(const orc_orbit* orbit, double phi, const orc_impurity_settings* is,
+                         orc_intersection* last, orc_intersection* next)
+{
+    last->dphi = C_PI2;
+    *next = *last;
+
+    const double L = is->cell_size;
+    const v2 low_left = to_world(last->entering_cell, is->spawn_region_start, L);
+    const v2 low_right = mk2(low_left.x + L, low_left.y + 0);
+    const v2 top_right = mk2(low_left.x + L, low_left.y + L);
+    const v2 top_left = mk2(low_left.x + 0, low_left.y + L);
+
+    update_first_boundary_intersect(orbit, mk2i(0, 1), top_left, top_right, L, last, next);
+    update_first_boundary_intersect(orbit, mk2i(0, -1), low_left, low_right, L, last, next);
+    update_first_boundary_intersect(orbit, mk2i(1, 0), low_right, top_right, L, last, next);
+    update_first_boundary_intersect(orbit, mk2i(-1, 0), low_left, top_left, L, last, next);
+
+    {
+        const double delta = 1e-7 * L;
+        const int is_xlow = (next->position.x - low_left.x) < delta;
+        const int is_xtop = (-next->position.x + L + low_left.x) < delta;
+        const int is_ylow = (next->position.y - low_left.y) < delta;
+        const int is_ytop = (-next->position.y + L + low_left.y) < delta;
+
+        v2i offset = mk2i(next->entering_cell.x - last->entering_cell.x, next->entering_cell.y - last->entering_cell.y);
+        if (is_xtop && is_ylow) offset = mk2i(1, -1);
+        if (is_xlow && is_ylow) offset = mk2i(-1, -1);
+        if (is_xlow && is_ytop) offset = mk2i(-1, 1);
+        if (is_xtop && is_ytop) offset = mk2i(1, 1);
+        next->entering_cell = mk2i(last->entering_cell.x + offset.x, last->entering_cell.y + offset.y);
+    }
+
+    const int same_cell = next->entering_cell.x == last->entering_cell.x && next->entering_cell.y == last->entering_cell.y;
+    return !same_cell && within_bounds(next->entering_cell, is->cells_per_row) &&
+           !angle_in_range(phi, mk2(last->incident_angle, next->incident_angle), orbit->clockwise);
+}
+
+/* --------------------------------------------------------------- lifetime.h */
+
+/* lifetime.h:36-58 */
+static orc_particle create_particle(int quadrant, int phi_step_index, v2 position, const orc_particle_settings* ps)
+{
+    orc_particle p;
+    p.position = position;
+    p.phi = ps->phi_start + quadrant * C_HALF_PI + phi_step_index * ps->phi_step_size;
+    p.angular_speed = ps->angular_speed;
+
+    const double c = O_COS(p.phi), s = O_SIN(p.phi);
+    const v2 vel = mk2(c * ps->particle_speed, s * ps->particle_speed);
+    const double bound_angle = orc_bound_angle(p.phi, ps->alpha, ps->is_clockwise);
+
+    orc_orbit orbit;
+    orbit.clockwise = ps->is_clockwise != 0;
+    orbit.radius = ps->particle_speed / ps->angular_speed;
+    orbit.radius_squared = orbit.radius * orbit.radius;
+    orbit.center = orbit_center(p.position, vel, orbit.radius, ps->particle_speed, ps->is_clockwise);
+    orbit.bound_time = orc_bound_time(p.phi, ps->alpha, ps->angular_speed, ps->is_coherent, ps->is_clockwise, 0);
+    orbit.bound_phi = ps->is_coherent ? C_INF : get_cross_angle(p.phi, bound_angle, ps->is_clockwise);
+    orbit.particle_angle = p.phi;
+    p.orbit = orbit;
+    return p;
+}
+
+typedef struct {
+    const double* imp_xy;      /* cell-ordered AoS (x, y) pairs, Grid::ordered_impurities */
+    const int32_t* imp_index;  /* inclusive end offset per cell, Grid::imp_index */
+} orc_buffers;
+
+/* lifetime.h:78-148 */
+static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is, const orc_buffers* b,
+                          orc_metrics* m, int64_t* cells_out, int64_t* tested_out)
+{
+    double lifetime = C_INF;
+    int64_t cells = 0, tested = 0;
+
+    orc_intersection next_intersection;
+    next_intersection.position = p->position;
+    next_intersection.entering_cell = get_cell(p->position, is->spawn_region_start, is->spawn_region_size, is->cells_per_row);
+    next_intersection.dphi = C_PI2;
+    next_intersection.incident_angle = p->phi;
+
+    const int particle_cell_index = get_index(next_intersection.entering_cell, is->cells_per_row);
+    {
+        const int impurity_start = (particle_cell_index == 0) ? 0 : b->imp_index[particle_cell_index - 1];
+        const int impurity_end = b->imp_index[particle_cell_index];
+        for (int i = impurity_start; i < impurity_end; i++) {
+            if (inside_impurity(p->position, mk2(b->imp_xy[2 * i], b->imp_xy[2 * i + 1]), is->impurity_radius)) {
+                m->particles_inside_impurity += 1;
+                *cells_out = 0;
+                *tested_out = 0;
+                return 0;
+            }
+        }
+    }
+
+    const int64_t cap = 4LL * is->cells_per_row * is->cells_per_row + 16;
+    while (1) {
+        orc_intersection entry_point = next_intersection;
+        const int next_cell_available = get_next_cell(&p->orbit, p->phi, is, &entry_point, &next_intersection);
+        const v2 valid_phi_range = mk2(entry_point.incident_angle, next_cell_available ? next_intersection.incident_angle : p->phi);
+
+        const int cell_idx = get_index(entry_point.entering_cell, is->cells_per_row);
+        const int impurity_start = (cell_idx > 0) ? b->imp_index[cell_idx - 1] : 0;
+        const int impurity_end = b->imp_index[cell_idx];
+
+        cells += 1;
+        tested += (impurity_end - impurity_start);
+
+        for (int i = impurity_start; i < impurity_end; i++) {
+            const v2 impurity = mk2(b->imp_xy[2 * i], b->imp_xy[2 * i + 1]);
+            if (circles_cross(&p->orbit, impurity, is->impurity_radius)) {
+                const double t = get_first_cross_time(&p->orbit, impurity, is->impurity_radius, p->angular_speed, valid_phi_range);
+                lifetime = (t < lifetime) ? t : lifetime;
+            }
+        }
+
+        if (lifetime < C_INF) break;
+
+        if (!next_cell_available) {
+            m->particles_escaped += 1;
+            lifetime = 0;
+            break;
+        }
+        if (cells >= cap) {
+            m->trace_cap_hits += 1;
+            lifetime = 0;
+            break;
+        }
+    }
+    m->cells_passed += cells;
+    m->impurities_tested += tested;
+    *cells_out = cells;
+    *tested_out = tested;
+
+    if (lifetime > p->orbit.bound_time) {
+        m->particles_at_bound += 1;
+        return p->orbit.bound_time;
+    }
+    return lifetime;
+}
+
+/* ------------------------------------------------------------------- util.h */
+
+/* util.h:24-28 */
+EXPORT double orc_simpson_weight(int i, int dim)
+{
+    const double main_multiplier = (i % 2 == 0) ? 2.0 : 4.0;
+    const int is_edge = (i == 0) || i == (dim - 1);
+    return is_edge ? 1.0 : main_multiplier;
+}
+
+/* util.h:34-46 (the unused `o` at :43 is dropped) */
+EXPORT double orc_get_sigma(double lt, double phi, double tau, double w)
+{
+    const double z = O_EXP(-lt / tau);
+    double r = O_COS(phi) - O_COS(phi + w * lt) * z;
+    r += w * tau * O_SIN(phi + w * lt) * z;
+    r -= w * tau * O_SIN(phi);
+    return r;
+}
+
+/* --------------------------------------------------------- flat test hooks */
+
+static orc_orbit mk_orbit(double cx, double cy, double R, int cw)
+{
+    orc_orbit o;
+    memset(&o, 0, sizeof(o));
+    o.center = mk2(cx, cy);
+    o.radius = R;
+    o.radius_squared = R * R;
+    o.clockwise = cw != 0;
+    return o;
+}
+
+EXPORT double orc_get_angle(double px, double py, double cx, double cy, double R, int cw)
+{
+    orc_orbit o = mk_orbit(cx, cy, R, cw);
+    return get_angle(mk2(px, py), &o);
+}
+
+EXPORT int orc_angle_in_range(double phi, double lo, double hi, int cw)
+{
+    return angle_in_range(phi, mk2(lo, hi), cw);
+}
+
+EXPORT double orc_cross_angle(double p, double q, int cw) { return get_cross_angle(p, q, cw); }
+
+EXPORT int orc_circles_cross(double cx, double cy, double R, double px, double py, double r)
+{
+    orc_orbit o = mk_orbit(cx, cy, R, 0);
+    return circles_cross(&o, mk2(px, py), r);
+}
+
+EXPORT void orc_cross_points(double cx, double cy, double R, double px, double py, double r, double* out4)
+{
+    orc_orbit o = mk_orbit(cx, cy, R, 0);
+    get_cross_points(&o, mk2(px, py), r, out4);
+}
+
+EXPORT double orc_first_cross_time(double cx, double cy, double R, int cw, double particle_angle,
+                                   double ipx, double ipy, double ir, double w, double lo, double hi)
+{
+    orc_orbit o = mk_orbit(cx, cy, R, cw);
+    o.particle_angle = particle_angle;
+    return get_first_cross_time(&o, mk2(ipx, ipy), ir, w, mk2(lo, hi));
+}
+
+EXPORT int orc_inside_impurity(double px, double py, double ix, double iy, double r)
+{
+    return inside_impurity(mk2(px, py), mk2(ix, iy), r);
+}
+
+EXPORT void orc_orbit_center(double px, double py, double vx, double vy, double R, double vf, int is_electron, double* out2)
+{
+    const v2 c = orbit_center(mk2(px, py), mk2(vx, vy), R, vf, is_electron);
+    out2[0] = c.x;
+    out2[1] = c.y;
+}
+
+EXPORT void orc_get_cell(double px, double py, double start, double size, int cpr, int* out2)
+{
+    const v2i c = get_cell(mk2(px, py), start, size, cpr);
+    out2[0] = c.x;
+    out2[1] = c.y;
+}
+
+EXPORT void orc_to_world(int cx, int cy, double start, double cell_size, double* out2)
+{
+    const v2 w = to_world(mk2i(cx, cy), start, cell_size);
+    out2[0] = w.x;
+    out2[1] = w.y;
+}
+
+EXPORT int orc_different_point(double ax, double ay, double bx, double by, double L)
+{
+    return different_point(mk2(ax, ay), mk2(bx, by), L);
+}
+
+EXPORT int orc_within_bounds(int x, int y, int cpr) { return within_bounds(mk2i(x, y), cpr); }
+EXPORT int orc_get_index(int x, int y, int cpr) { return get_index(mk2i(x, y), cpr); }
+
+/*
+ * GetNextCell on flat state.  last/next = {pos.x, pos.y, incident_angle, dphi},
+ * cells = {x, y}.  `last` is updated like the reference (its dphi is reset).
+ */
+EXPORT int orc_next_cell(double cx, double cy, double R, int cw, double phi,
+                         double spawn_start, double spawn_size, double cell_size, int cpr,
+                         double* last, int* last_cell, double* next, int* next_cell)
+{
+    orc_orbit o = mk_orbit(cx, cy, R, cw);
+    orc_impurity_settings is;
+    is.spawn_region_start = spawn_start;
+    is.spawn_region_size = spawn_size;
+    is.cell_size = cell_size;
+    is.cells_per_row = cpr;
+    is.impurity_radius = 0;
+    orc_intersection a, n;
+    a.position = mk2(last[0], last[1]);
+    a.incident_angle = last[2];
+    a.dphi = last[3];
+    a.entering_cell = mk2i(last_cell[0], last_cell[1]);
+    const int ok = get_next_cell(&o, phi, &is, &a, &n);
+    last[3] = a.dphi;
+    next[0] = n.position.x;
+    next[1] = n.position.y;
+    next[2] = n.incident_angle;
+    next[3] = n.dphi;
+    next_cell[0] = n.entering_cell.x;
+    next_cell[1] = n.entering_cell.y;
+    return ok;
+}
+
+/* ------------------------------------------------------------- Grid.{h,cpp} */
+
+typedef struct {
+    double spawn_lo, spawn_hi;           /* spawn_range (Grid.cpp:33,52) */
+    int cells_per_row;
+    double cell_size;
+    double impurity_radius;
+    int64_t unique_impurity_count;
+    int64_t total_indexed_impurities;
+    double* ordered_xy;                  /* 2 * total_indexed_impurities */
+    int32_t* imp_index;                  /* cells_per_row^2 inclusive ends */
+    uint64_t seed;
+} orc_grid;
+
+/* Grid.cpp:182-188 */
+static v2i grid_get_cell(const orc_grid* g, double x, double y)
+{
+    return mk2i((int)((x - g->spawn_lo) / (g->spawn_hi - g->spawn_lo) * (double)(g->cells_per_row)),
+                (int)((y - g->spawn_lo) / (g->spawn_hi - g->spawn_lo) * (double)(g->cells_per_row)));
+}
+
+/* Grid.h:64-67 */
+static v2 grid_to_world(const orc_grid* g, v2i c)
+{
+    return mk2(g->spawn_lo + c.x * g->cell_size, g->spawn_lo + c.y * g->cell_size);
+}
+
+/*
+ * Grid.cpp:112-180.  Writes the cells (row-major ids) impurity `pos` is added
+ * to, in the order the reference pushes them; returns the count (<= 9).
+ */
+static int grid_targets(const orc_grid* g, v2 pos, double r, int out[9])
+{
+    const int cpr = g->cells_per_row;
+    v2i probe[5];
+    probe[0] = grid_get_cell(g, pos.x, pos.y);
+    probe[1] = grid_get_cell(g, pos.x + r, pos.y);
+    probe[2] = grid_get_cell(g, pos.x - r, pos.y);
+    probe[3] = grid_get_cell(g, pos.x, pos.y + r);
+    probe[4] = grid_get_cell(g, pos.x, pos.y - r);
+
+    int n = 0;
+    v2i added[5];
+    int n_added = 0;
+    for (int i = 0; i < 5; i++) {
+        const v2i c = probe[i];
+        int seen = 0;
+        for (int k = 0; k < n_added; k++)
+            if (added[k].x == c.x && added[k].y == c.y) seen = 1;
+        if (!seen && within_bounds(c, cpr)) {
+            added[n_added++] = c;
+            out[n++] = c.y * cpr + c.x;
+        }
+    }
+
+    const v2 low_left = grid_to_world(g, probe[0]);
+    const v2 low_right = mk2(low_left.x + g->cell_size, low_left.y + 0);
+    const v2 top_left = mk2(low_left.x + 0, low_left.y + g->cell_size);
+    const v2 top_right = mk2(low_left.x + g->cell_size, low_left.y + g->cell_size);
+    const double ir2 = r * r;
+    const v2i cell = grid_get_cell(g, pos.x, pos.y);
+
+    const double d_tl = sq(top_left.x - pos.x) + sq(top_left.y - pos.y);
+    if (d_tl < ir2) {
+        const v2i c = mk2i(cell.x - 1, cell.y + 1);
+        if (within_bounds(c, cpr)) out[n++] = c.y * cpr + c.x;
+    }
+    const double d_tr = sq(top_right.x - pos.x) + sq(top_right.y - pos.y);
+    if (d_tr < ir2) {
+        const v2i c = mk2i(cell.x + 1, cell.y + 1);
+        if (within_bounds(c, cpr)) out[n++] = c.y * cpr + c.x;
+    }
+    const double d_ll = sq(low_left.x - pos.x) + sq(low_left.y - pos.y);
+    if (d_ll < ir2) {
+        const v2i c = mk2i(cell.x - 1, cell.y - 1);
+        if (within_bounds(c, cpr)) out[n++] = c.y * cpr + c.x;
+    }
+    const double d_lr = sq(low_right.x - pos.x) + sq(low_right.y - pos.y);
+    if (d_lr < ir2) {
+        const v2i c = mk2i(cell.x + 1, cell.y - 1);
+        if (within_bounds(c, cpr)) out[n++] = c.y * cpr + c.x;
+    }
+    return n;
+}
+
+/* Grid.cpp:77-110: bin in impurity order, flatten row-major with inclusive ends. */
+static int grid_bin(orc_grid* g, const double* x, const double* y, int64_t n)
+{
+    const int64_t ncell = (int64_t)g->cells_per_row * g->cells_per_row;
+    int32_t* counts = (int32_t*)calloc((size_t)ncell, sizeof(int32_t));
+    int* targets = (int*)malloc((size_t)(n > 0 ? n : 1) * 9 * sizeof(int));
+    unsigned char* ntarg = (unsigned char*)malloc((size_t)(n > 0 ? n : 1));
+    if (!counts || !targets || !ntarg) { free(counts); free(targets); free(ntarg); return -1; }
+
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int k = grid_targets(g, mk2(x[i], y[i]), g->impurity_radius, targets + 9 * i);
+        ntarg[i] = (unsigned char)k;
+        for (int t = 0; t < k; t++) counts[targets[9 * i + t]]++;
+        total += k;
+    }
+    g->total_indexed_impurities = total;
+    g->ordered_xy = (double*)malloc((size_t)(total > 0 ? total : 1) * 2 * sizeof(double));
+    g->imp_index = (int32_t*)malloc((size_t)ncell * sizeof(int32_t));
+    int64_t* fill = (int64_t*)malloc((size_t)ncell * sizeof(int64_t));
+    if (!g->ordered_xy || !g->imp_index || !fill) { free(counts); free(targets); free(ntarg); free(fill); return -1; }
+
+    int64_t acc = 0;
+    for (int64_t c = 0; c < ncell; c++) {
+        fill[c] = acc;
+        acc += counts[c];
+        g->imp_index[c] = (int32_t)acc;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        for (int t = 0; t < ntarg[i]; t++) {
+            const int c = targets[9 * i + t];
+            const int64_t slot = fill[c]++;
+            g->ordered_xy[2 * slot] = x[i];
+            g->ordered_xy[2 * slot + 1] = y[i];
+        }
+    }
+    free(counts);
+    free(targets);
+    free(ntarg);
+    free(fill);
+    return 0;
+}
+
+/* Grid.cpp:40,54 with the optional explicit override. */
+static int cells_per_row_rule(int64_t unique_count, int target, int cpr_override)
+{
+    if (cpr_override > 0) return cpr_override;
+    const int c = (int)round(sqrt(unique_count / (double)target));
+    return c > 1 ? c : 1;
+}
+
+/* Grid.cpp:48-59 (file / caller-provided impurities). */
+EXPORT orc_grid* orc_grid_create(const double* x, const double* y, int64_t n, double region_size,
+                                 double region_extends, double impurity_radius, int target, int cpr_override)
+{
+    orc_grid* g = (orc_grid*)calloc(1, sizeof(orc_grid));
+    if (!g) return NULL;
+    g->impurity_radius = impurity_radius;
+    g->spawn_lo = -region_extends;
+    g->spawn_hi = region_size + region_extends;
+    g->unique_impurity_count = n;
+    g->cells_per_row = cells_per_row_rule(n, target, cpr_override);
+    g->cell_size = (g->spawn_hi - g->spawn_lo) / (double)g->cells_per_row;
+    if (grid_bin(g, x, y, n) != 0) { free(g); return NULL; }
+    return g;
+}
+
+/* Grid.cpp:28-46 with the es_rng.h generator; n <= 0 derives N from density. */
+EXPORT orc_grid* orc_grid_generate(uint64_t seed, double region_size, double region_extends, double density,
+                                   int64_t n, double impurity_radius, int target, int cpr_override)
+{
+    const double lo = -region_extends, hi = region_size + region_extends;
+    if (n <= 0) {
+        const double area_length = hi - lo;
+        const double proposal = area_length * area_length * density;
+        if (proposal > 1e7 || proposal < 1) return NULL;
+        n = (int64_t)ceil(proposal);
+        if (n < 1) n = 1;
+    }
+    double* x = (double*)malloc((size_t)n * sizeof(double));
+    double* y = (double*)malloc((size_t)n * sizeof(double));
+    if (!x || !y) { free(x); free(y); return NULL; }
+    const uint64_t key = esr_key(seed);
+    for (int64_t i = 0; i < n; i++) esr_impurity(key, (uint64_t)i, lo, hi, &x[i], &y[i]);
+    orc_grid* g = orc_grid_create(x, y, n, region_size, region_extends, impurity_radius, target, cpr_override);
+    if (g) g->seed = seed;
+    free(x);
+    free(y);
+    return g;
+}
+
+EXPORT void orc_grid_free(orc_grid* g)
+{
+    if (!g) return;
+    free(g->ordered_xy);
+    free(g->imp_index);
+    free(g);
+}
+
+/* {cells_per_row, unique, total_indexed}, {spawn_lo, spawn_hi, cell_size} */
+EXPORT void orc_grid_info(const orc_grid* g, int64_t* ints3, double* dbl3)
+{
+    ints3[0] = g->cells_per_row;
+    ints3[1] = g->unique_impurity_count;
+    ints3[2] = g->total_indexed_impurities;
+    dbl3[0] = g->spawn_lo;
+    dbl3[1] = g->spawn_hi;
+    dbl3[2] = g->cell_size;
+}
+
+EXPORT void orc_grid_copy(const orc_grid* g, double* ordered_xy, int32_t* imp_index)
+{
+    memcpy(ordered_xy, g->ordered_xy, (size_t)g->total_indexed_impurities * 2 * sizeof(double));
+    memcpy(imp_index, g->imp_index, (size_t)g->cells_per_row * g->cells_per_row * sizeof(int32_t));
+}
+
+/* impurity generator exposure (es_rng.h) */
+EXPORT void orc_generate_impurities(uint64_t seed, int64_t n, double lo, double hi, double* x, double* y)
+{
+    const uint64_t key = esr_key(seed);
+    for (int64_t i = 0; i < n; i++) esr_impurity(key, (uint64_t)i, lo, hi, &x[i], &y[i]);
+}
+
+/* ----------------------------------------------- Simulation.h / SimulationCPU */
+
+typedef struct {
+    /* Simulation ctor (Simulation.h:131-146) */
+    int32_t positions_per_row;
+    int32_t particles_per_quadrant;
+    double region_size;
+    double region_extends;
+    double cell_size;            /* GridInformation.cell_size (sample 0's grid) */
+    /* Settings (ConfigSettings.h:22-34) used by InitSample (Simulation.h:111-126) */
+    double tau;
+    double alpha;
+    int32_t is_clockwise;
+    double particle_speed;
+    /* > 0: tau for the incoherent pass instead of HBAR/(KB*T). */
+    double incoherent_tau;
+    int32_t nthreads;
+} orc_sim_params;
+
+typedef struct {
+    double xx_naive, xy_naive;   /* reference-order serial sum (SimulationCPU.cpp:206-222) */
+    double xx_comp, xy_comp;     /* same terms, compensated (Neumaier) sum */
+} orc_sigma;
+
+typedef struct {
+    orc_metrics pm;
+    int64_t real_particles;
+    double avg_particle_lifetime;
+} orc_run_metrics;
+
+/* Derived simulation state: SimulationSettings / ParticleSettings / ImpuritySettings. */
+typedef struct {
+    int ppr, ppq, ppp, particles_per_row;
+    int64_t total_particles;
+    double distance;
+    v2 small_offset;
+    double integrand_angle_area, phi_integrand_factor, coherent_tau;
+    orc_particle_settings ps;
+    orc_impurity_settings is;
+    double signed_angular_speed;
+} orc_state;
+
+static void init_state(orc_state* st, const orc_sim_params* prm, const orc_grid* g, int coherent, double B)
+{
+    /* Simulation.h:131-146 */
+    st->ppr = prm->positions_per_row;
+    st->ppq = prm->particles_per_quadrant;
+    st->ppp = st->ppq * 4;
+    st->particles_per_row = st->ppr * st->ppp;
+    st->total_particles = (int64_t)st->ppr * st->particles_per_row;
+    st->distance = prm->region_size / (double)(st->ppr - 1);
+    st->small_offset = mk2(prm->cell_size * 0.01, prm->cell_size * 0.005);
+    /* Grid::GetSettings (Grid.h:82-91) */
+    st->is.impurity_radius = g->impurity_radius;
+    st->is.spawn_region_start = g->spawn_lo;
+    st->is.spawn_region_size = g->spawn_hi - g->spawn_lo;
+    st->is.cells_per_row = g->cells_per_row;
+    st->is.cell_size = st->is.spawn_region_size / (double)g->cells_per_row;
+    /* Simulation::InitSample (Simulation.h:111-126) */
+    const double base_area = prm->alpha * 2.0;
+    st->integrand_angle_area = !coherent ? base_area : (C_HALF_PI - base_area);
+    st->phi_integrand_factor = st->integrand_angle_area / ((st->ppq - 1) * 3.0);
+    st->coherent_tau = prm->tau;
+    st->ps.alpha = prm->alpha;
+    st->ps.particle_speed = prm->particle_speed;
+    st->ps.is_clockwise = prm->is_clockwise ? 1 : 0;
+    st->ps.is_coherent = coherent ? 1 : 0;
+    st->ps.phi_start = !coherent ? -prm->alpha : prm->alpha;
+    st->ps.phi_step_size = st->integrand_angle_area / (st->ppq - 1);
+    /* SimulationCPU.cpp:107-108 */
+    st->ps.angular_speed = C_E * B / C_M;
+    st->signed_angular_speed = st->ps.is_clockwise ? -st->ps.angular_speed : st->ps.angular_speed;
+}
+
+/* Simulation.h:67 */
+static int64_t sim_index(const orc_state* st, int i, int j, int q, int p)
+{
+    return (int64_t)j * st->particles_per_row + (int64_t)i * st->ppp + (int64_t)q * st->ppq + p;
+}
+
+typedef struct {
+    const orc_state* st;
+    const orc_grid* g;
+    double* lifetimes;
+    int j0, j1;
+    orc_metrics m;
+} lt_job;
+
+/* SimulationCPU.cpp:112-135 over rows [j0, j1) */
+static void* lifetime_rows(void* arg)
+{
+    lt_job* job = (lt_job*)arg;
+    const orc_state* st = job->st;
+    orc_buffers b;
+    b.imp_xy = job->g->ordered_xy;
+    b.imp_index = job->g->imp_index;
+    memset(&job->m, 0, sizeof(job->m));
+    for (int j = job->j0; j < job->j1; j++) {
+        for (int i = 0; i < st->ppr; i++) {
+            const v2 position = mk2(i * st->distance + st->small_offset.x, j * st->distance + st->small_offset.y);
+            for (int q = 0; q < 4; q++) {
+                for (int p = 0; p < st->ppq; p++) {
+                    const orc_particle particle = create_particle(q, p, position, &st->ps);
+                    int64_t cells = 0, tested = 0;
+                    job->lifetimes[sim_index(st, i, j, q, p)] = trace_orbit(&particle, &st->is, &b, &job->m, &cells, &tested);
+                    if (job->m.max_cells_passed < cells) job->m.max_cells_passed = cells;
+                    if (job->m.max_impurities_tested < tested) job->m.max_impurities_tested = tested;
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+/* SimulationCPU.cpp:103-141; rows are split across threads, results identical. */
+static void compute_lifetimes(const orc_state* st, const orc_grid* g, double* lifetimes, int nthreads, orc_metrics* out)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > st->ppr) nthreads = st->ppr;
+    lt_job* jobs = (lt_job*)calloc((size_t)nthreads, sizeof(lt_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].st = st;
+        jobs[t].g = g;
+        jobs[t].lifetimes = lifetimes;
+        jobs[t].j0 = (int)((int64_t)st->ppr * t / nthreads);
+        jobs[t].j1 = (int)((int64_t)st->ppr * (t + 1) / nthreads);
+    }
+    if (nthreads == 1) {
+        lifetime_rows(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, lifetime_rows, &jobs[t]);
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    memset(out, 0, sizeof(*out));
+    for (int t = 0; t < nthreads; t++) {
+        out->cells_passed += jobs[t].m.cells_passed;
+        out->impurities_tested += jobs[t].m.impurities_tested;
+        out->particles_inside_impurity += jobs[t].m.particles_inside_impurity;
+        out->particles_escaped += jobs[t].m.particles_escaped;
+        out->particles_at_bound += jobs[t].m.particles_at_bound;
+        out->trace_cap_hits += jobs[t].m.trace_cap_hits;
+        if (out->max_cells_passed < jobs[t].m.max_cells_passed) out->max_cells_passed = jobs[t].m.max_cells_passed;
+        if (out->max_impurities_tested < jobs[t].m.max_impurities_tested) out->max_impurities_tested = jobs[t].m.max_impurities_tested;
+    }
+    free(jobs);
+    free(th);
+}
+
+/* Simulation.h:69-76 */
+static double sigma_factor(const orc_state* st, double tau)
+{
+    const double kf = C_M * st->ps.particle_speed / C_HBAR;
+    double outside = (C_E * C_E * kf * kf) / (2.0 * C_PI * C_PI * C_M * C_C1);
+    const double wct = st->ps.angular_speed * tau;
+    outside *= tau / (1.0 + wct * wct);
+    return outside;
+}
+
+/* Simulation.h:88 */
+static double sigma_integrand_factor(const orc_state* st, double tau)
+{
+    return sq(1.0 / (3.0 * (st->ppr - 1))) * st->phi_integrand_factor * sigma_factor(st, tau) * 1e-8;
+}
+
+/* Simulation.h:90 plus the explicit incoherent tau. */
+static double get_tau(const orc_state* st, double temperature, double incoherent_tau)
+{
+    if (st->ps.is_coherent) return st->coherent_tau;
+    if (incoherent_tau > 0) return incoherent_tau;
+    return C_HBAR / (C_KB * temperature);
+}
+
+typedef struct { double s, c; } neum;
+static inline void neum_add(neum* a, double x)
+{
+    const double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+
+/*
+ * SimulationCPU::DeriveTemperature (SimulationCPU.cpp:143-155):
+ * GetTruncatedLifetimes (Simulation.h:78-86), ApplySigma (:178-204),
+ * IntegrateResult (:206-222), each in reference order.
+ */
+static orc_sigma derive_temperature(const orc_state* st, double tau, const double* raw)
+{
+    const double max_lt = 15.0 * tau;  /* Simulation.h:91 */
+    double total_xx = 0, total_xy = 0;
+    neum cxx = {0, 0}, cxy = {0, 0};
+    for (int j = 0; j < st->ppr; j++) {
+        const double wy = orc_simpson_weight(j, st->ppr);
+        for (int i = 0; i < st->ppr; i++) {
+            const double wp = wy * orc_simpson_weight(i, st->ppr);
+            for (int q = 0; q < 4; q++) {
+                for (int p = 0; p < st->ppq; p++) {
+                    const int64_t idx = sim_index(st, i, j, q, p);
+                    const double r = raw[idx];
+                    const double lt = (r < max_lt) ? r : max_lt;
+                    const double phi = st->ps.phi_start + q * C_HALF_PI + p * st->ps.phi_step_size;
+                    const double sigma_base = orc_get_sigma(lt, phi, tau, st->signed_angular_speed);
+                    const double sxx = sigma_base * O_COS(phi);
+                    const double sxy = sigma_base * O_SIN(phi);
+                    const double w = wp * orc_simpson_weight(p, st->ppq);
+                    const double txx = sxx * w, txy = sxy * w;
+                    total_xx += txx;
+                    total_xy += txy;
+                    neum_add(&cxx, txx);
+                    neum_add(&cxy, txy);
+                }
+            }
+        }
+    }
+    const double f = sigma_integrand_factor(st, tau);
+    orc_sigma s;
+    s.xx_naive = total_xx * f;
+    s.xy_naive = total_xy * f;
+    s.xx_comp = (cxx.s + cxx.c) * f;
+    s.xy_comp = (cxy.s + cxy.c) * f;
+    return s;
+}
+
+/*
+ * SimulationCPU::ComputeSigmas (SimulationCPU.cpp:33-73) for one sample/pass:
+ * lifetimes for B, then each temperature.  `lifetimes_out` (P doubles, layout
+ * [j][i][q][p]) may be NULL.  Returns 0 on success.
+ */
+EXPORT int orc_compute_sigmas(const orc_grid* g, const orc_sim_params* prm, int coherent, double B,
+                              const double* temperatures, int nT, orc_sigma* out, orc_run_metrics* metrics,
+                              double* lifetimes_out)
+{
+    orc_state st;
+    init_state(&st, prm, g, coherent, B);
+    double* lts = lifetimes_out;
+    int own = 0;
+    if (!lts) {
+        lts = (double*)malloc((size_t)st.total_particles * sizeof(double));
+        if (!lts) return -1;
+        own = 1;
+    }
+    orc_metrics pm;
+    compute_lifetimes(&st, g, lts, prm->nthreads, &pm);
+    if (metrics) {
+        metrics->pm = pm;
+        metrics->real_particles = st.total_particles - pm.particles_inside_impurity;
+        /* Simulation.h:93-106 */
+        double total = 0;
+        for (int64_t i = 0; i < st.total_particles; i++) total += lts[i];
+        metrics->avg_particle_lifetime = st.total_particles ? total / (double)st.total_particles : 0;
+    }
+    for (int t = 0; t < nT; t++) {
+        const double tau = get_tau(&st, temperatures[t], prm->incoherent_tau);
+        out[t] = derive_temperature(&st, tau, lts);
+    }
+    if (own) free(lts);
+    return 0;
+}
+
+/* Lifetimes only (no temperature stage): for parity and timing. */
+EXPORT int orc_compute_lifetimes(const orc_grid* g, const orc_sim_params* prm, int coherent, double B,
+                                 double* lifetimes_out, orc_metrics* metrics)
+{
+    orc_state st;
+    init_state(&st, prm, g, coherent, B);
+    orc_metrics pm;
+    compute_lifetimes(&st, g, lifetimes_out, prm->nthreads, &pm);
+    if (metrics) *metrics = pm;
+    return 0;
+}
+
+/* Temperature stage only, from given raw lifetimes. */
+EXPORT int orc_derive_temperatures(const orc_grid* g, const orc_sim_params* prm, int coherent, double B,
+                                   const double* raw_lifetimes, const double* temperatures, int nT, orc_sigma* out)
+{
+    orc_state st;
+    init_state(&st, prm, g, coherent, B);
+    for (int t = 0; t < nT; t++) {
+        const double tau = get_tau(&st, temperatures[t], prm->incoherent_tau);
+        out[t] = derive_temperature(&st, tau, raw_lifetimes);
+    }
+    return 0;
+}
+
+/* One particle: CreateParticle + TraceOrbit (lifetime.h:36-148). */
+EXPORT double orc_particle_lifetime(const orc_grid* g, int q, int p, double px, double py,
+                                    double particle_speed, double angular_speed, double alpha,
+                                    int is_clockwise, int is_coherent, double phi_start, double phi_step_size,
+                                    orc_metrics* m)
+{
+    orc_particle_settings ps;
+    ps.particle_speed = particle_speed;
+    ps.angular_speed = angular_speed;
+    ps.alpha = alpha;
+    ps.is_clockwise = is_clockwise;
+    ps.is_coherent = is_coherent;
+    ps.phi_start = phi_start;
+    ps.phi_step_size = phi_step_size;
+    orc_impurity_settings is;
+    is.impurity_radius = g->impurity_radius;
+    is.spawn_region_start = g->spawn_lo;
+    is.spawn_region_size = g->spawn_hi - g->spawn_lo;
+    is.cells_per_row = g->cells_per_row;
+    is.cell_size = is.spawn_region_size / (double)g->cells_per_row;
+    orc_buffers b;
+    b.imp_xy = g->ordered_xy;
+    b.imp_index = g->imp_index;
+    const orc_particle particle = create_particle(q, p, mk2(px, py), &ps);
+    orc_metrics local;
+    memset(&local, 0, sizeof(local));
+    int64_t cells = 0, tested = 0;
+    const double lt = trace_orbit(&particle, &is, &b, m ? m : &local, &cells, &tested);
+    return lt;
+}
+
+/* Math hooks so the tests can measure the math build against glibc. */
+EXPORT double orc_m_sin(double x) { return O_SIN(x); }
+EXPORT double orc_m_cos(double x) { return O_COS(x); }
+EXPORT double orc_m_asin(double x) { return O_ASIN(x); }
+EXPORT double orc_m_exp(double x) { return O_EXP(x); }
