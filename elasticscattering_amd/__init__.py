@@ -1,0 +1,264 @@
+"""MI355X (gfx950) implementation of the ElasticScattering hot path.
+
+Python mirror of the reference's operator API over the C ABI in
+``include/es_abi.h`` (ctypes, no torch types):
+
+* :class:`Simulation` — ``class Simulation`` (src/sim/Simulation.h:56-147):
+  ``init_sample`` / ``compute_sigmas`` like ``InitSample`` / ``ComputeSigmas``;
+  its ``set_impurities`` / ``generate_impurities`` are the two ``Grid``
+  constructors (src/sim/Grid.h:93-94), binned on the GPU and kept there.
+
+The HIP library ``libes_hip.so`` is built in-tree by
+``python -m elasticscattering_amd.build``; importing the API without it raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "EsError", "GridInfo", "GridDesc", "Settings", "Sigma", "Metrics", "SolverDesc",
+    "Simulation", "load_library", "LIB_PATH", "EXPORTED_SYMBOLS",
+]
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libes_hip.so")
+
+ES_OK = 0
+ES_ERR_INVALID_ARG = -1
+ES_ERR_HIP = -2
+ES_ERR_NO_GRID = -3
+ES_ERR_NO_SAMPLE = -4
+ES_ERR_OOM = -5
+ES_ERR_OUT_OF_GRID = -6
+ES_ERR_NO_DEVICE = -7
+ES_PRECISION_FP64 = 0
+
+
+class EsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"es error {code}: {msg}")
+        self.code = code
+
+
+class GridInfo(C.Structure):
+    _fields_ = [
+        ("region_size", C.c_double),
+        ("region_extends", C.c_double),
+        ("cell_size", C.c_double),
+        ("indexed_impurity_count", C.c_int64),
+        ("unique_impurity_count", C.c_int64),
+        ("index_size", C.c_int32),
+        ("cells_per_row", C.c_int32),
+    ]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("region_size", C.c_double),
+        ("region_extends", C.c_double),
+        ("impurity_radius", C.c_double),
+        ("impurity_density", C.c_double),
+        ("target_cell_population", C.c_int32),
+        ("cells_per_row", C.c_int32),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("tau", C.c_double),
+        ("alpha", C.c_double),
+        ("is_clockwise", C.c_int32),
+        ("particle_speed", C.c_double),
+        ("incoherent_tau", C.c_double),
+    ]
+
+
+class Sigma(C.Structure):
+    _fields_ = [("xx", C.c_double), ("xy", C.c_double)]
+
+
+class Metrics(C.Structure):
+    _fields_ = [
+        ("cells_passed", C.c_int64),
+        ("impurities_tested", C.c_int64),
+        ("particles_inside_impurity", C.c_int64),
+        ("particles_escaped", C.c_int64),
+        ("particles_at_bound", C.c_int64),
+        ("max_impurities_tested", C.c_int64),
+        ("max_cells_passed", C.c_int64),
+        ("trace_cap_hits", C.c_int64),
+        ("real_particles", C.c_int64),
+        ("avg_particle_lifetime", C.c_double),
+        ("time_elapsed_lifetimes", C.c_double),
+        ("time_elapsed_temperatures", C.c_double),
+        ("traced_particles", C.c_int64),
+        ("edge_hits", C.c_int64),
+        ("cross_calls", C.c_int64),
+        ("sigma_evals", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [
+        ("positions_per_row", C.c_int32),
+        ("particles_per_quadrant", C.c_int32),
+        ("grid_info", GridInfo),
+        ("precision", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+_PD = C.POINTER(C.c_double)
+_P = C.c_void_p
+
+# name -> (restype, argtypes); every symbol include/es_abi.h declares.
+_SIGNATURES = {
+    "es_solver_create": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(_P)]),
+    "es_solver_destroy": (None, [_P]),
+    "es_last_error": (C.c_char_p, [_P]),
+    "es_grid_set_impurities": (C.c_int, [_P, _PD, _PD, C.c_int64, C.POINTER(GridDesc), C.POINTER(GridInfo)]),
+    "es_grid_generate": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(GridDesc), C.POINTER(GridInfo)]),
+    "es_grid_read": (C.c_int, [_P, _PD, C.POINTER(C.c_int32)]),
+    "es_solver_init_sample": (C.c_int, [_P, C.POINTER(Settings), C.c_int32]),
+    "es_solver_compute_sigmas": (C.c_int, [_P, C.c_double, _PD, C.c_int32, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
+    "es_solver_compute_sigmas_sweep": (C.c_int, [_P, _PD, C.c_int32, _PD, C.c_int32, C.POINTER(Sigma),
+                                                 C.POINTER(Metrics)]),
+    "es_version": (C.c_char_p, []),
+}
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads libes_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is missing: build it with `python -m elasticscattering_amd.build`")
+    lib = C.CDLL(p)
+    for name, (res, args) in _SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_PD) if a is not None else None
+
+
+class Simulation:
+    """``class Simulation`` (Simulation.h:56-147) on one GPU.
+
+    ``Simulation(positions_per_row, particles_per_quadrant, grid_info)`` as in
+    Simulation.h:131; ``grid_info`` is the GridInformation of sample 0's grid
+    (region_size, region_extends, cell_size); when omitted it is taken from the
+    first grid set on the solver, as SimulationRunner.cpp:75-84 does.
+    """
+
+    def __init__(self, positions_per_row, particles_per_quadrant, grid_info=None, device=0):
+        self.lib = load_library()
+        self.ppr = int(positions_per_row)
+        self.ppq = int(particles_per_quadrant)
+        if grid_info is None:
+            gi = GridInfo()  # taken from the first grid set on this solver
+        elif isinstance(grid_info, GridInfo):
+            gi = grid_info
+        else:
+            gi = GridInfo(**grid_info)
+        desc = SolverDesc(self.ppr, self.ppq, gi, ES_PRECISION_FP64, int(device))
+        h = _P()
+        rc = self.lib.es_solver_create(C.byref(desc), C.byref(h))
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_last_error(None).decode())
+        self.h = h
+        self.grid_info = gi
+
+    @property
+    def total_particles(self):
+        return self.ppr * self.ppr * 4 * self.ppq
+
+    def _check(self, rc):
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_last_error(self.h).decode())
+
+    # Grid ------------------------------------------------------------
+    def set_impurities(self, x, y, region_size, region_extends, impurity_radius, target_cell_population=15,
+                       cells_per_row=0):
+        """Grid(std::vector<v2>, ...) (Grid.cpp:48-59), binned on the GPU."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        d = GridDesc(region_size, region_extends, impurity_radius, 0.0, int(target_cell_population),
+                     int(cells_per_row))
+        info = GridInfo()
+        self._check(self.lib.es_grid_set_impurities(self.h, _dptr(x), _dptr(y), len(x), C.byref(d), C.byref(info)))
+        return info
+
+    def generate_impurities(self, seed, region_size, region_extends, impurity_radius, density=0.0, count=0,
+                            target_cell_population=15, cells_per_row=0):
+        """Grid(seed, ...) (Grid.cpp:28-46) with the es_rng.h generator, on the GPU."""
+        d = GridDesc(region_size, region_extends, impurity_radius, density, int(target_cell_population),
+                     int(cells_per_row))
+        info = GridInfo()
+        self._check(self.lib.es_grid_generate(self.h, int(seed), int(count), C.byref(d), C.byref(info)))
+        return info
+
+    def read_grid(self, info):
+        """(ordered impurities [Ni, 2], inclusive end index [cpr^2]) like Grid::GetImpurities/GetIndex."""
+        xy = np.empty(2 * max(info.indexed_impurity_count, 1), dtype=np.float64)
+        idx = np.empty(info.cells_per_row * info.cells_per_row, dtype=np.int32)
+        self._check(self.lib.es_grid_read(self.h, _dptr(xy), idx.ctypes.data_as(C.POINTER(C.c_int32))))
+        return xy[: 2 * info.indexed_impurity_count].reshape(-1, 2), idx
+
+    # Simulation ------------------------------------------------------
+    def init_sample(self, tau, alpha, is_clockwise, particle_speed, coherent, incoherent_tau=0.0):
+        """Simulation::InitSample (Simulation.h:111-126)."""
+        s = Settings(tau, alpha, int(bool(is_clockwise)), particle_speed, incoherent_tau)
+        self._check(self.lib.es_solver_init_sample(self.h, C.byref(s), int(bool(coherent))))
+
+    def compute_sigmas(self, B, temperatures, want_lifetimes=False):
+        """Simulation::ComputeSigmas (Simulation.h:128): ([(xx, xy)] per T, metrics dict, lifetimes|None)."""
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        out = (Sigma * max(len(T), 1))()
+        m = Metrics()
+        lts = np.empty(self.total_particles, dtype=np.float64) if want_lifetimes else None
+        self._check(self.lib.es_solver_compute_sigmas(self.h, float(B), _dptr(T), len(T), out, C.byref(m),
+                                                      _dptr(lts)))
+        return [(out[i].xx, out[i].xy) for i in range(len(T))], m.as_dict(), lts
+
+    def compute_sweep(self, fields, temperatures):
+        """All fields in one launch: (sigma [nB, nT, 2], list of metrics dicts)."""
+        Bs = np.ascontiguousarray(fields, dtype=np.float64)
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        out = (Sigma * max(len(Bs) * len(T), 1))()
+        ms = (Metrics * len(Bs))()
+        self._check(self.lib.es_solver_compute_sigmas_sweep(self.h, _dptr(Bs), len(Bs), _dptr(T), len(T), out, ms))
+        sig = np.array([[out[b * len(T) + t].xx, out[b * len(T) + t].xy] for b in range(len(Bs))
+                        for t in range(len(T))], dtype=np.float64).reshape(len(Bs), len(T), 2)
+        return sig, [ms[b].as_dict() for b in range(len(Bs))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.es_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,650 @@
+// es_kernels.hip — gfx950 kernels of the lifetime / sigma path.
+//
+//   k_generate      impurity centres from es_rng.h (Grid.cpp:61-75 analogue)
+//   k_bin_count     per impurity: target cells (Grid.cpp:112-180), per-cell counts
+//   k_bin_scan      exclusive scan -> cell begin offsets + reference inclusive ends
+//   k_bin_fill      scatter impurity ids into their cells
+//   k_bin_sort      ascending impurity id inside each cell (the reference's order)
+//   k_bin_gather    cell-ordered SoA x[], y[]
+//   k_positions     start cell + "inside an impurity" test per position
+//                   (lifetime.h:84-99), hoisted out of the phi and B loops
+//   k_compact       ordered list of free positions (deterministic)
+//   k_trace<CW,NT>  one particle per work item: CreateParticle + TraceOrbit
+//                   (lifetime.h:36-148), truncation, GetSigma, Simpson weights
+//                   (SimulationCPU.cpp:178-222) fused; double-double block
+//                   partials, no atomics
+//   k_finalize      fixed-order double-double sum of the block partials
+//
+// All FP64; compiled with -ffp-contract=off.
+#include "es_kernels.h"
+#include "es_rng.h"
+#include "es_trace.h"
+
+namespace es {
+
+namespace {
+
+struct DD {
+    double s, c;
+};
+
+__device__ __forceinline__ void dd_add_d(DD& a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x))
+        a.c += (a.s - t) + x;
+    else
+        a.c += (x - t) + a.s;
+    a.s = t;
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    const double s = a.s + b.s;
+    const double bb = s - a.s;
+    double e = (a.s - (s - bb)) + (b.s - bb);
+    e += a.c + b.c;
+    const double r = s + e;
+    DD out;
+    out.s = r;
+    out.c = e - (r - s);
+    return out;
+}
+
+__device__ __forceinline__ DD dd_wave_reduce(DD v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        DD o;
+        o.s = __shfl_xor(v.s, off, 64);
+        o.c = __shfl_xor(v.c, off, 64);
+        v = dd_add(v, o);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int64_t i64_wave_sum(int64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int64_t i64_wave_max(int64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double simpson_weight(int i, int dim)  // util.h:24-28
+{
+    const double main_multiplier = (i % 2 == 0) ? 2.0 : 4.0;
+    const bool is_edge = (i == 0) || i == (dim - 1);
+    return is_edge ? 1.0 : main_multiplier;
+}
+
+// ------------------------------------------------------------------ generate
+
+__global__ void k_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a, b;
+    esr_impurity(key, (uint64_t)i, lo, hi, &a, &b);
+    x[i] = a;
+    y[i] = b;
+}
+
+// ------------------------------------------------------------------- binning
+
+struct BinGeom {
+    double lo, size, cell_size, r;
+    int cpr;
+};
+
+__device__ __forceinline__ void grid_cell(const BinGeom& g, double x, double y, int& cx, int& cy)
+{
+    cx = (int)((x - g.lo) / g.size * (double)(g.cpr));  // Grid.cpp:182-188
+    cy = (int)((y - g.lo) / g.size * (double)(g.cpr));
+}
+
+__device__ __forceinline__ bool in_grid(int cx, int cy, int cpr)
+{
+    return (cx >= 0 && cx < cpr) && (cy >= 0 && cy < cpr);
+}
+
+// Grid.cpp:112-180 for impurity (x, y): up to 9 distinct target cells.
+__device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
+{
+    int px[5], py[5];
+    grid_cell(g, x, y, px[0], py[0]);
+    grid_cell(g, x + g.r, y, px[1], py[1]);
+    grid_cell(g, x - g.r, y, px[2], py[2]);
+    grid_cell(g, x, y + g.r, px[3], py[3]);
+    grid_cell(g, x, y - g.r, px[4], py[4]);
+    int n = 0;
+    int ax[5], ay[5], na = 0;
+    for (int k = 0; k < 5; k++) {
+        bool seen = false;
+        for (int m = 0; m < na; m++) seen = seen || (ax[m] == px[k] && ay[m] == py[k]);
+        if (!seen && in_grid(px[k], py[k], g.cpr)) {
+            ax[na] = px[k];
+            ay[na] = py[k];
+            na++;
+            out[n++] = py[k] * g.cpr + px[k];
+        }
+    }
+    const double llx = g.lo + px[0] * g.cell_size;  // Grid.h:64-67
+    const double lly = g.lo + py[0] * g.cell_size;
+    const double lrx = llx + g.cell_size, lry = lly + 0.0;
+    const double tlx = llx + 0.0, tly = lly + g.cell_size;
+    const double trx = llx + g.cell_size, try_ = lly + g.cell_size;
+    const double ir2 = g.r * g.r;
+    const int cx = px[0], cy = py[0];
+    double dx, dy;
+    dx = tlx - x; dy = tly - y;
+    if (dx * dx + dy * dy < ir2 && in_grid(cx - 1, cy + 1, g.cpr)) out[n++] = (cy + 1) * g.cpr + (cx - 1);
+    dx = trx - x; dy = try_ - y;
+    if (dx * dx + dy * dy < ir2 && in_grid(cx + 1, cy + 1, g.cpr)) out[n++] = (cy + 1) * g.cpr + (cx + 1);
+    dx = llx - x; dy = lly - y;
+    if (dx * dx + dy * dy < ir2 && in_grid(cx - 1, cy - 1, g.cpr)) out[n++] = (cy - 1) * g.cpr + (cx - 1);
+    dx = lrx - x; dy = lry - y;
+    if (dx * dx + dy * dy < ir2 && in_grid(cx + 1, cy - 1, g.cpr)) out[n++] = (cy - 1) * g.cpr + (cx + 1);
+    return n;
+}
+
+__global__ void k_bin_count(const double* x, const double* y, int64_t n, BinGeom g, int32_t* targets,
+                            uint8_t* ntarg, int32_t* counts)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t t[9];
+    const int k = bin_targets(g, x[i], y[i], t);
+    ntarg[i] = (uint8_t)k;
+    for (int m = 0; m < k; m++) {
+        targets[9 * i + m] = t[m];
+        atomicAdd(&counts[t[m]], 1);
+    }
+}
+
+// Single-block exclusive scan of n int32 -> begin[0..n] (int64).  Optionally
+// also writes inclusive ends as int32 (Grid::imp_index layout).
+__global__ __launch_bounds__(1024) void k_scan(const int32_t* counts, int64_t n, int64_t* begin, int32_t* incl_end)
+{
+    __shared__ int64_t part[1024];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = (b0 + chunk < n) ? b0 + chunk : n;
+    int64_t local = 0;
+    for (int64_t k = b0; k < b1; k++) local += counts[k];
+    part[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = (tid >= off) ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int64_t acc = part[tid] - local;  // exclusive prefix of this chunk
+    for (int64_t k = b0; k < b1; k++) {
+        begin[k] = acc;
+        acc += counts[k];
+        if (incl_end) incl_end[k] = (int32_t)acc;
+    }
+    if (tid == 1023) begin[n] = part[1023];
+}
+
+__global__ void k_bin_fill(int64_t n, const int32_t* targets, const uint8_t* ntarg, const int64_t* begin,
+                           int32_t* fill, int32_t* order)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = ntarg[i];
+    for (int m = 0; m < k; m++) {
+        const int c = targets[9 * i + m];
+        const int slot = atomicAdd(&fill[c], 1);
+        order[begin[c] + slot] = (int32_t)i;
+    }
+}
+
+__device__ void sift_down(int32_t* a, int64_t root, int64_t n)
+{
+    while (true) {
+        int64_t child = 2 * root + 1;
+        if (child >= n) return;
+        if (child + 1 < n && a[child + 1] > a[child]) child++;
+        if (a[root] >= a[child]) return;
+        const int32_t t = a[root];
+        a[root] = a[child];
+        a[child] = t;
+        root = child;
+    }
+}
+
+// Sort each cell's impurity ids ascending: the reference pushes impurities into
+// a cell in index order (Grid.cpp:87-89,129).
+__global__ void k_bin_sort(int64_t ncell, const int64_t* begin, int32_t* order)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int32_t* a = order + begin[c];
+    const int64_t n = begin[c + 1] - begin[c];
+    if (n <= 32) {
+        for (int64_t k = 1; k < n; k++) {
+            const int32_t v = a[k];
+            int64_t m = k - 1;
+            while (m >= 0 && a[m] > v) {
+                a[m + 1] = a[m];
+                m--;
+            }
+            a[m + 1] = v;
+        }
+    } else {
+        for (int64_t r = n / 2 - 1; r >= 0; r--) sift_down(a, r, n);
+        for (int64_t e = n - 1; e > 0; e--) {
+            const int32_t t = a[0];
+            a[0] = a[e];
+            a[e] = t;
+            sift_down(a, 0, e);
+        }
+    }
+}
+
+__global__ void k_bin_gather(int64_t total, const int32_t* order, const double* x, const double* y, double* ix,
+                             double* iy)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const int32_t i = order[k];
+    ix[k] = x[i];
+    iy[k] = y[i];
+}
+
+// ----------------------------------------------------------------- positions
+
+__global__ void k_positions(const double* ix, const double* iy, const int32_t* cell_end, double start, double size,
+                            double radius, int cpr, int ppr, double dist, double offx, double offy, uint8_t* flags,
+                            int32_t* err)
+{
+    const int64_t pidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pidx >= (int64_t)ppr * ppr) return;
+    const int i = (int)(pidx % ppr), j = (int)(pidx / ppr);
+    const double px = (double)i * dist + offx;  // SimulationCPU.cpp:114
+    const double py = (double)j * dist + offy;
+    const int cx = (int)((px - start) / size * (double)(cpr));  // cell_grid.h:52-58
+    const int cy = (int)((py - start) / size * (double)(cpr));
+    if (!(cx >= 0 && cx < cpr && cy >= 0 && cy < cpr)) {
+        flags[pidx] = 2;
+        atomicOr(err, 1);
+        return;
+    }
+    const int ci = cy * cpr + cx;
+    const int s = (ci == 0) ? 0 : cell_end[ci - 1];
+    const int e = cell_end[ci];
+    const double r2 = radius * radius;
+    uint8_t inside = 0;
+    for (int k = s; k < e; k++) {  // InsideImpurity, details.h:208-213
+        const double dx = px - ix[k], dy = py - iy[k];
+        if (r2 > (dx * dx + dy * dy)) {
+            inside = 1;
+            break;
+        }
+    }
+    flags[pidx] = inside;
+}
+
+__global__ __launch_bounds__(1024) void k_compact(const uint8_t* flags, int64_t n, int32_t* free_pos, int32_t* nfree)
+{
+    __shared__ int64_t part[1024];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = (b0 + chunk < n) ? b0 + chunk : n;
+    int64_t local = 0;
+    for (int64_t k = b0; k < b1; k++) local += (flags[k] == 0);
+    part[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = (tid >= off) ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int64_t w = part[tid] - local;
+    for (int64_t k = b0; k < b1; k++)
+        if (flags[k] == 0) free_pos[w++] = (int32_t)k;
+    if (tid == 1023) *nfree = (int32_t)part[1023];
+}
+
+// --------------------------------------------------------------------- trace
+
+template <bool CW, int NT>
+__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
+{
+    constexpr int NQ = 2 * NT + 1;
+    const int b = blockIdx.y;
+    const int ppq = a.ppq;
+    const int ppp = 4 * ppq;
+    const int64_t nitems = (int64_t)a.nfree * ppp;
+    const int64_t P = (int64_t)a.ppr * a.ppr * ppp;
+    const double w = a.w[b];
+    const double ws = CW ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
+
+    DevGrid g;
+    g.ix = a.ix;
+    g.iy = a.iy;
+    g.cell_end = a.cell_end;
+    g.start = a.g_start;
+    g.size = a.g_size;
+    g.L = a.g_L;
+    g.radius = a.g_radius;
+    g.cpr = a.g_cpr;
+
+    // Orbit constants shared by every particle of this field (lifetime.h:48-49).
+    const double R = a.speed / w;
+    const double R2 = R * R;
+    const double radd = R + g.radius, rmin = R - g.radius;
+
+    DD acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q].s = acc[q].c = 0.0;
+    int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0;
+    int m_escaped = 0, m_at_bound = 0, m_cap = 0, m_traced = 0;
+    int64_t m_edge = 0, m_cross = 0, m_sigma = 0;
+
+    const int64_t stride = (int64_t)a.nblk * kBlock;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nitems; k += stride) {
+        const int64_t slot = k / ppp;
+        const int v = (int)(k - slot * ppp);
+        const int q = v / ppq, p = v - (v / ppq) * ppq;
+        const int pidx = a.free_pos[slot];
+        const int j = pidx / a.ppr, i = pidx - (pidx / a.ppr) * a.ppr;
+        const double px = (double)i * a.dist + a.offx;
+        const double py = (double)j * a.dist + a.offy;
+
+        // CreateParticle (lifetime.h:36-58)
+        const double phi = a.phi_start + q * ES_HALF_PI + p * a.phi_step;
+        double sphi, cphi;
+        esm_sincos(phi, &sphi, &cphi);
+        const double vx = cphi * a.speed, vy = sphi * a.speed;
+        const double sx = vy * R / a.speed;
+        const double sy = -vx * R / a.speed;
+        Orb o;
+        o.cx = CW ? (px - sx) : (px + sx);
+        o.cy = CW ? (py - sy) : (py + sy);
+        o.R = R;
+        o.R2 = R2;
+        o.eps_r = 1e-10 * R;
+        o.radd2 = radd * radd;
+        o.rmin2 = rmin * rmin;
+        o.r2 = g.radius * g.radius;
+        o.pa = phi;
+        o.w = w;
+        double bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
+        if (!a.coherent) {
+            const double remaining = d_smod(phi + a.alpha, ES_HALF_PI);
+            const double dphi = CW ? remaining : (2.0 * a.alpha - remaining);
+            bound_time = dphi / w;
+        }
+        const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
+        const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
+
+        TraceCounters tc;
+        tc.escaped = tc.at_bound = tc.cap_hit = tc.edge_hits = tc.cross_calls = 0;
+        const double lt_raw = d_trace<CW>(o, px, py, phi, c0x, c0y, bound_time, g, tc);
+
+        m_cells += tc.cells;
+        m_tested += tc.tested;
+        m_max_cells = tc.cells > m_max_cells ? tc.cells : m_max_cells;
+        m_max_tested = tc.tested > m_max_tested ? tc.tested : m_max_tested;
+        m_escaped += tc.escaped;
+        m_at_bound += tc.at_bound;
+        m_cap += tc.cap_hit;
+        m_traced += 1;
+        m_edge += tc.edge_hits;
+        m_cross += tc.cross_calls;
+        if (a.lifetimes) a.lifetimes[(int64_t)b * P + (int64_t)pidx * ppp + v] = lt_raw;
+        dd_add_d(acc[2 * NT], lt_raw);
+
+        // Truncation, GetSigma, Simpson weight per temperature.  A zero
+        // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
+        const double wgt = simpson_weight(j, a.ppr) * simpson_weight(i, a.ppr) * simpson_weight(p, ppq);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            if (t < a.nT) {
+                const double ml = a.max_lt[t];
+                const double lt = (lt_raw < ml) ? lt_raw : ml;
+                if (lt != 0.0) {
+                    const double tau = a.tau[t];
+                    const double z = esm_exp(-lt / tau);
+                    double sa, ca;
+                    esm_sincos(phi + ws * lt, &sa, &ca);
+                    double r = cphi - ca * z;
+                    r += ws * tau * sa * z;
+                    r -= ws * tau * sphi;
+                    dd_add_d(acc[2 * t], r * cphi * wgt);
+                    dd_add_d(acc[2 * t + 1], r * sphi * wgt);
+                    m_sigma += 1;
+                }
+            }
+        }
+    }
+
+    // Block reduction in a fixed order: wave shuffles, then the 4 waves in LDS.
+    __shared__ double sh_s[NQ][kBlock / 64];
+    __shared__ double sh_c[NQ][kBlock / 64];
+    __shared__ int64_t sh_m[kMetricSlots][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const DD r = dd_wave_reduce(acc[q]);
+        if (lane == 0) {
+            sh_s[q][wave] = r.s;
+            sh_c[q][wave] = r.c;
+        }
+    }
+    int64_t mv[kMetricSlots] = {m_cells, m_tested, m_escaped, m_at_bound, m_cap,  m_max_tested,
+                                m_max_cells, m_traced, m_edge, m_cross, m_sigma};
+#pragma unroll
+    for (int s = 0; s < kMetricSlots; s++) {
+        const int64_t r = (s == 5 || s == 6) ? i64_wave_max(mv[s]) : i64_wave_sum(mv[s]);
+        if (lane == 0) sh_m[s][wave] = r;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)b * a.nblk + blockIdx.x;
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        DD r;
+        r.s = sh_s[q][0];
+        r.c = sh_c[q][0];
+        for (int wv = 1; wv < kBlock / 64; wv++) {
+            DD o;
+            o.s = sh_s[q][wv];
+            o.c = sh_c[q][wv];
+            r = dd_add(r, o);
+        }
+        // Partials are stored with the launch's NQ stride = 2 * nT + 1.
+        const int nq = 2 * a.nT + 1;
+        int dst = -1;
+        if (q < 2 * a.nT) dst = q;
+        else if (q == 2 * NT) dst = 2 * a.nT;
+        if (dst >= 0) {
+            a.dpart[(base * nq + dst) * 2] = r.s;
+            a.dpart[(base * nq + dst) * 2 + 1] = r.c;
+        }
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + kMetricSlots) {
+        const int s = threadIdx.x - 64;
+        int64_t r = sh_m[s][0];
+        for (int wv = 1; wv < kBlock / 64; wv++) {
+            const int64_t o = sh_m[s][wv];
+            r = (s == 5 || s == 6) ? (o > r ? o : r) : r + o;
+        }
+        a.mpart[base * kMetricSlots + s] = r;
+    }
+}
+
+// ------------------------------------------------------------------ finalize
+
+__global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int64_t* mpart, int nblk, int nT,
+                                                  const double* factor, double* out_sig, double* out_lt_sum,
+                                                  int64_t* out_m)
+{
+    const int b = blockIdx.x;
+    const int nq = 2 * nT + 1;
+    __shared__ double sh_s[4], sh_c[4];
+    __shared__ int64_t sh_m[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < nq; q++) {
+        DD acc;
+        acc.s = acc.c = 0.0;
+        for (int blk = threadIdx.x; blk < nblk; blk += 256) {
+            DD o;
+            o.s = dpart[(((int64_t)b * nblk + blk) * nq + q) * 2];
+            o.c = dpart[(((int64_t)b * nblk + blk) * nq + q) * 2 + 1];
+            acc = dd_add(acc, o);
+        }
+        acc = dd_wave_reduce(acc);
+        if (lane == 0) {
+            sh_s[wave] = acc.s;
+            sh_c[wave] = acc.c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DD r;
+            r.s = sh_s[0];
+            r.c = sh_c[0];
+            for (int wv = 1; wv < 4; wv++) {
+                DD o;
+                o.s = sh_s[wv];
+                o.c = sh_c[wv];
+                r = dd_add(r, o);
+            }
+            const double total = r.s + r.c;
+            if (q < 2 * nT) {
+                const int t = q >> 1;
+                out_sig[((int64_t)b * nT + t) * 2 + (q & 1)] = total * factor[(int64_t)b * nT + t];
+            } else {
+                out_lt_sum[b] = total;
+            }
+        }
+        __syncthreads();
+    }
+    for (int s = 0; s < kMetricSlots; s++) {
+        const bool is_max = (s == 5 || s == 6);
+        int64_t acc = 0;
+        for (int blk = threadIdx.x; blk < nblk; blk += 256) {
+            const int64_t o = mpart[((int64_t)b * nblk + blk) * kMetricSlots + s];
+            acc = is_max ? (o > acc ? o : acc) : acc + o;
+        }
+        acc = is_max ? i64_wave_max(acc) : i64_wave_sum(acc);
+        if (lane == 0) sh_m[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t r = sh_m[0];
+            for (int wv = 1; wv < 4; wv++) r = is_max ? (sh_m[wv] > r ? sh_m[wv] : r) : r + sh_m[wv];
+            out_m[(int64_t)b * kMetricSlots + s] = r;
+        }
+        __syncthreads();
+    }
+}
+
+inline unsigned int blocks_for(int64_t n, int bs) { return (unsigned int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, st, key, n, lo, hi, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_count(const double* x, const double* y, int64_t n, const GridDev& g, BinScratch& s,
+                            hipStream_t st)
+{
+    const int64_t ncell = (int64_t)g.cpr * g.cpr;
+    hipError_t e = hipMemsetAsync(s.counts, 0, ncell * sizeof(int32_t), st);
+    if (e != hipSuccess) return e;
+    if (n <= 0) return hipSuccess;
+    BinGeom bg{g.lo, g.size, g.cell_size, g.radius, g.cpr};
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, y, n, bg, s.targets, s.ntarg,
+                       s.counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st)
+{
+    const int64_t ncell = (int64_t)g.cpr * g.cpr;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.counts, ncell, s.begin, g.cell_end);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s, hipStream_t st)
+{
+    const int64_t ncell = (int64_t)g.cpr * g.cpr;
+    hipError_t e = hipMemsetAsync(s.fill, 0, ncell * sizeof(int32_t), st);
+    if (e != hipSuccess) return e;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, s.targets, s.ntarg, s.begin,
+                           s.fill, s.order);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(ncell, 64)), dim3(64), 0, st, ncell, s.begin, s.order);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (g.total > 0) {
+        hipLaunchKernelGGL(k_bin_gather, dim3(blocks_for(g.total, 256)), dim3(256), 0, st, g.total, s.order, x, y,
+                           g.ix, g.iy);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
+                            int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st)
+{
+    const int64_t n = (int64_t)ppr * ppr;
+    hipError_t e = hipMemsetAsync(err, 0, sizeof(int32_t), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_positions, dim3(blocks_for(n, 256)), dim3(256), 0, st, g.ix, g.iy, g.cell_end,
+                       g.lo, g.size, g.radius, g.cpr, ppr, dist, offx, offy, flags, err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, flags, n, free_pos, nfree);
+    return hipGetLastError();
+}
+
+template <bool CW, int NT>
+static hipError_t launch_trace_t(const TraceArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_trace<CW, NT>), dim3(a.nblk, a.nB), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+template <bool CW>
+static hipError_t launch_trace_cw(const TraceArgs& a, hipStream_t st)
+{
+    if (a.nT <= 1) return launch_trace_t<CW, 1>(a, st);
+    if (a.nT <= 2) return launch_trace_t<CW, 2>(a, st);
+    if (a.nT <= 4) return launch_trace_t<CW, 4>(a, st);
+    return launch_trace_t<CW, 8>(a, st);
+}
+
+hipError_t launch_trace(const TraceArgs& a, hipStream_t st)
+{
+    if (a.nT < 0 || a.nT > kMaxT || a.nB <= 0 || a.nblk <= 0) return hipErrorInvalidValue;
+    return a.clockwise ? launch_trace_cw<true>(a, st) : launch_trace_cw<false>(a, st);
+}
+
+hipError_t launch_finalize(const double* dpart, const int64_t* mpart, int nB, int nblk, int nT, const double* factor,
+                           double* out_sig, double* out_lt_sum, int64_t* out_m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(nB), dim3(256), 0, st, dpart, mpart, nblk, nT, factor, out_sig, out_lt_sum,
+                       out_m);
+    return hipGetLastError();
+}
+
+}  // namespace es

@@ -1,0 +1,49 @@
+"""Named workloads of this implementation (synthetic fields from es_rng.h).
+
+Each is a plain dict of reference settings (src/ConfigSettings.h:22-34 keys and
+the Simulation sizes) plus the explicit choices the reference's settings cannot
+express:
+
+* ``cells_per_row``  — explicit cell grid (the reference derives it from an
+  integer target population, Grid.cpp:40);
+* ``incoherent_tau`` — tau used directly for the incoherent pass when the
+  temperature is 0 K (the reference uses HBAR/(KB*T), Simulation.h:90);
+* the spawn square of side L is split as region_size = L/2 with extends = L/4
+  on each side (Grid.cpp:33 spawns in [-extends, region_size + extends]).
+"""
+import math
+
+import numpy as np
+
+# Small FP64 case, CPU-runnable: side 1.0e-6 m, 2,000 disks r = 1.1e-8 m,
+# 32x32 cells, 64x64 positions, 4 windows x 17 Simpson points (alpha = pi/4),
+# B = 20 T, T = 0 K (tau = 1e-12 s direct), v_F = 7e5 m/s, m* = 5 m_e.
+SMALL_FP64 = dict(
+    name="small_fp64_2000imp_64pos_17phi",
+    side=1.0e-6, impurity_count=2000, impurity_radius=1.1e-8, cells_per_row=32,
+    positions_per_row=64, particles_per_quadrant=17, alpha=math.pi / 4,
+    fields=[20.0], temperatures=[0.0], incoherent_tau=1e-12, tau=1e-12,
+    particle_speed=7e5, clockwise=1, seeds=[101],
+)
+
+# Production FP64 sweep: side 2.0e-6 m, 60,000 disks r = 1.1e-8 m, 128x128
+# cells, 256x256 positions, 65 Simpson points per window, 40 fields log-spaced
+# over 1-40 T, T in {2, 40, 100} K, 8 seeded samples.
+PRODUCTION_FP64 = dict(
+    name="production_fp64_60000imp_256pos_65phi_40B_3T_8samples",
+    side=2.0e-6, impurity_count=60000, impurity_radius=1.1e-8, cells_per_row=128,
+    positions_per_row=256, particles_per_quadrant=65, alpha=math.pi / 4,
+    fields=list(np.logspace(0.0, math.log10(40.0), 40)), temperatures=[2.0, 40.0, 100.0], incoherent_tau=0.0,
+    tau=1e-12, particle_speed=7e5, clockwise=1, seeds=[(1 + 100) * (i + 1) for i in range(8)],
+)
+
+WORKLOADS = {w["name"]: w for w in (SMALL_FP64, PRODUCTION_FP64)}
+
+
+def region(w):
+    """(region_size, region_extends) for the workload's spawn side."""
+    return w["side"] / 2.0, w["side"] / 4.0
+
+
+def lifetimes_per_pass(w):
+    return w["positions_per_row"] ** 2 * 4 * w["particles_per_quadrant"]

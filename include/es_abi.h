@@ -1,0 +1,166 @@
+/*
+ * es_abi.h — C ABI of the MI355X (gfx950) implementation of the
+ * ElasticScattering hot path: impurity binning, the per-particle lifetime
+ * trace, the Simpson integration over phi and the reductions to sigma_xx /
+ * sigma_xy.
+ *
+ * It replaces the OpenCL backend behind the reference's operator API,
+ * `class Simulation` (src/sim/Simulation.h:56-147), and the grid it consumes
+ * (`class Grid`, src/sim/Grid.h:44-95).  Plain C types only; every call
+ * returns ES_OK or a negative status (the reference prints and exit(0)s,
+ * src/utils/ErrorMacros.h:22-38); es_last_error() gives the message.
+ *
+ * Threading: one solver per host thread.  A solver owns its device memory and
+ * its HIP stream on the current device at creation.  Host arrays are
+ * caller-owned and are only read/written during the call.
+ */
+#ifndef ES_ABI_H
+#define ES_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ES_OK 0
+#define ES_ERR_INVALID_ARG (-1)    /* bad size, null pointer, unsupported setting */
+#define ES_ERR_HIP (-2)            /* a HIP runtime call failed */
+#define ES_ERR_NO_GRID (-3)        /* no grid set on the solver */
+#define ES_ERR_NO_SAMPLE (-4)      /* es_solver_init_sample not called */
+#define ES_ERR_OOM (-5)            /* device allocation failed or over the size cap */
+#define ES_ERR_OUT_OF_GRID (-6)    /* a start position lies outside the cell grid (reference reads out of bounds, lifetime.h:88-91) */
+#define ES_ERR_NO_DEVICE (-7)      /* no HIP device */
+
+#define ES_PRECISION_FP64 0        /* FP64 throughout; lifetimes bit-identical to the CPU restatement */
+
+typedef struct es_solver es_solver;
+
+/* GridInformation (src/sim/es/settings.h:26-33), plus the cell count. */
+typedef struct es_grid_info {
+    double region_size;
+    double region_extends;
+    double cell_size;
+    int64_t indexed_impurity_count;   /* Grid::GetTotalImpurityCount (with duplicates) */
+    int64_t unique_impurity_count;    /* Grid::GetUniqueImpurityCount */
+    int32_t index_size;               /* cells_per_row^2 */
+    int32_t cells_per_row;
+} es_grid_info;
+
+/* Arguments of the Grid constructors (src/sim/Grid.cpp:28-59). */
+typedef struct es_grid_desc {
+    double region_size;               /* positions span [0, region_size] */
+    double region_extends;            /* spawn square [-extends, region_size + extends] */
+    double impurity_radius;
+    double impurity_density;          /* used by es_grid_generate when count <= 0 */
+    int32_t target_cell_population;   /* cells_per_row = max(round(sqrt(N/target)), 1) (Grid.cpp:40) */
+    int32_t cells_per_row;            /* > 0: explicit cell grid instead of the rule above */
+} es_grid_desc;
+
+/* Settings (src/ConfigSettings.h:22-34) as used by Simulation::InitSample. */
+typedef struct es_settings {
+    double tau;                       /* coherent tau (Simulation.h:118) */
+    double alpha;
+    int32_t is_clockwise;
+    double particle_speed;
+    double incoherent_tau;            /* > 0: incoherent tau instead of HBAR/(KB*T) (Simulation.h:90) */
+} es_settings;
+
+typedef struct es_sigma {
+    double xx;
+    double xy;
+} es_sigma;
+
+/* ParticleMetrics (src/sim/es/particle_metrics.h:22-44) + Metrics (src/Metrics.h:27-44). */
+typedef struct es_metrics {
+    int64_t cells_passed;
+    int64_t impurities_tested;
+    int64_t particles_inside_impurity;
+    int64_t particles_escaped;
+    int64_t particles_at_bound;
+    int64_t max_impurities_tested;
+    int64_t max_cells_passed;
+    int64_t trace_cap_hits;           /* traces stopped after 4*cpr^2+16 cells; 0 on every finite trace */
+    int64_t real_particles;           /* total - inside (SimulationCPU.cpp:41) */
+    double avg_particle_lifetime;     /* Simulation.h:93-106 */
+    double time_elapsed_lifetimes;    /* seconds, device events around the lifetime stage */
+    double time_elapsed_temperatures; /* seconds, the per-T reduction stage */
+    /* Work counters of this implementation (no reference counterpart): the
+       inputs of the algorithmic FP64 work model used for roofline figures. */
+    int64_t traced_particles;         /* particles that start outside every impurity */
+    int64_t edge_hits;                /* accepted cell-edge intersections (cell_grid.h:80-89 calls) */
+    int64_t cross_calls;              /* GetFirstCrossTime calls (details.h:182) */
+    int64_t sigma_evals;              /* GetSigma evaluations with a non-zero truncated lifetime */
+} es_metrics;
+
+/* Simulation(positions_per_row, particles_per_quadrant, GridInformation) (Simulation.h:131-146). */
+typedef struct es_solver_desc {
+    int32_t positions_per_row;
+    int32_t particles_per_quadrant;
+    es_grid_info grid_info;           /* region_size, region_extends, cell_size of sample 0's grid;
+                                         cell_size 0: taken from the first grid set on the solver */
+    int32_t precision;                /* ES_PRECISION_FP64 */
+    int32_t device;                   /* HIP device ordinal */
+} es_solver_desc;
+
+/* Simulation::Simulation — allocate a solver (Simulation.h:131-146). */
+int es_solver_create(const es_solver_desc* desc, es_solver** out);
+
+/* ~SimulationCL (SimulationCL.cpp:477-508). */
+void es_solver_destroy(es_solver* s);
+
+/* Last error message of `s` (or of the last failed create when s is NULL). */
+const char* es_last_error(const es_solver* s);
+
+/*
+ * Grid(std::vector<v2> impurities, region_size, region_extends, radius, target)
+ * (Grid.cpp:48-59): uploads n centres and bins them on the GPU (Grid.cpp:77-180).
+ */
+int es_grid_set_impurities(es_solver* s, const double* x, const double* y, int64_t n,
+                           const es_grid_desc* desc, es_grid_info* out_info);
+
+/*
+ * Grid(seed, region_size, region_extends, density, radius, target)
+ * (Grid.cpp:28-46): draws the centres on the GPU with this implementation's
+ * generator (elasticscattering_amd/csrc/es_rng.h; the reference RNG is
+ * implementation-defined) and bins them.  count <= 0 derives
+ * N = ceil(side^2 * density) as Grid.cpp:35-39 does.
+ */
+int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_desc* desc,
+                     es_grid_info* out_info);
+
+/*
+ * Grid::GetImpurities / Grid::GetIndex (Grid.h:74-75) of the current grid:
+ * cell-ordered AoS (x, y) pairs and inclusive end offsets per cell.
+ * Either pointer may be NULL.
+ */
+int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index);
+
+/* Simulation::InitSample (Simulation.h:111-126). */
+int es_solver_init_sample(es_solver* s, const es_settings* settings, int32_t coherent);
+
+/*
+ * Simulation::ComputeSigmas (Simulation.h:128; SimulationCPU.cpp:33-73):
+ * lifetimes for field B, then sigma for each of the nT temperatures.
+ * out: nT entries.  metrics may be NULL.  lifetimes (may be NULL) receives
+ * the P = ppr^2 * 4 * ppq raw lifetimes in the layout [j][i][q][p]
+ * (Simulation.h:67).
+ */
+int es_solver_compute_sigmas(es_solver* s, double B, const double* T, int32_t nT,
+                             es_sigma* out, es_metrics* metrics, double* lifetimes);
+
+/*
+ * The same for nB fields in one launch (the B loop of SimulationRunner.cpp:129-145).
+ * out: nB * nT entries, B-major.  metrics: nB entries or NULL.
+ */
+int es_solver_compute_sigmas_sweep(es_solver* s, const double* B, int32_t nB, const double* T, int32_t nT,
+                                   es_sigma* out, es_metrics* metrics);
+
+/* Build/version string of the library. */
+const char* es_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ES_ABI_H */

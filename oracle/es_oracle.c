@@ -123,6 +123,8 @@ typedef struct {
     int64_t max_impurities_tested;
     int64_t max_cells_passed;
     int64_t trace_cap_hits;
+    int64_t edge_hits;      /* Update_Best_Intersect calls */
+    int64_t cross_calls;    /* GetFirstCrossTime calls */
 } orc_metrics;
 
 static inline v2 mk2(double x, double y) { v2 r; r.x = x; r.y = y; return r; }
@@ -294,10 +296,14 @@ static int different_point(v2 p1, v2 p2, double L)
     return fabs(p1.x - p2.x) > 1e-6 * L || fabs(p1.y - p2.y) > 1e-6 * L;
 }
 
+/* Work counter of the trace in progress on this thread (not reference state). */
+static _Thread_local int64_t tl_edge_hits;
+
 /* cell_grid.h:80-89 (the Update_Best_Intersect macro) */
 static void update_best_intersect(const orc_orbit* orbit, v2 pt, v2i next_cell, double L,
                                   const orc_intersection* last, orc_intersection* closest)
 {
+    tl_edge_hits += 1;
     const double incident_angle = get_angle(pt, orbit);
     const double dphi = get_cross_angle(last->incident_angle, incident_angle, orbit->clockwise);
     if (dphi < closest->dphi && different_point(pt, last->position, L)) {
@@ -430,6 +436,7 @@ static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is
     }
 
     const int64_t cap = 4LL * is->cells_per_row * is->cells_per_row + 16;
+    tl_edge_hits = 0;
     while (1) {
         orc_intersection entry_point = next_intersection;
         const int next_cell_available = get_next_cell(&p->orbit, p->phi, is, &entry_point, &next_intersection);
@@ -447,6 +454,7 @@ static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is
             if (circles_cross(&p->orbit, impurity, is->impurity_radius)) {
                 const double t = get_first_cross_time(&p->orbit, impurity, is->impurity_radius, p->angular_speed, valid_phi_range);
                 lifetime = (t < lifetime) ? t : lifetime;
+                m->cross_calls += 1;
             }
         }
 
@@ -465,6 +473,7 @@ static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is
     }
     m->cells_passed += cells;
     m->impurities_tested += tested;
+    m->edge_hits += tl_edge_hits;
     *cells_out = cells;
     *tested_out = tested;
 
@@ -956,6 +965,8 @@ static void compute_lifetimes(const orc_state* st, const orc_grid* g, double* li
         out->particles_escaped += jobs[t].m.particles_escaped;
         out->particles_at_bound += jobs[t].m.particles_at_bound;
         out->trace_cap_hits += jobs[t].m.trace_cap_hits;
+        out->edge_hits += jobs[t].m.edge_hits;
+        out->cross_calls += jobs[t].m.cross_calls;
         if (out->max_cells_passed < jobs[t].m.max_cells_passed) out->max_cells_passed = jobs[t].m.max_cells_passed;
         if (out->max_impurities_tested < jobs[t].m.max_impurities_tested) out->max_impurities_tested = jobs[t].m.max_impurities_tested;
     }

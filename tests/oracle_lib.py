@@ -34,6 +34,8 @@ class OrcMetrics(C.Structure):
         ("max_impurities_tested", _I64),
         ("max_cells_passed", _I64),
         ("trace_cap_hits", _I64),
+        ("edge_hits", _I64),
+        ("cross_calls", _I64),
     ]
 
     def as_dict(self):

@@ -1,0 +1,480 @@
+"""Pins the CPU oracle (no GPU needed).
+
+Each function of the oracle is checked against what fixes it independently:
+* the reference's own known-answer tests (tests/es/*.h, tests/sim/*.h, numbers
+  transcribed into tests/golden/reference_kats.json and the .dat fields);
+* the reference's kernel headers compiled from the reference tree
+  (oracle/_ref/libesref.so) on random inputs, bit for bit (glibc build);
+* mathematics: circle equations, closed-form integrals, Simpson exactness,
+  exact disk/square overlap, brute-force first hits on small fields.
+Both oracle builds (glibc math and es_math.h) run the reference tests.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FX = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))
+PI = 3.141592653589793238463
+HALF_PI = 1.570796326794896619231
+PI2 = 6.283185307179586
+E_CH = 1.60217646e-19
+M = 5.0 * 9.10938188e-31
+
+
+@pytest.fixture(params=["libm", "esm"], scope="module")
+def lib(request):
+    lb = O.load(request.param)
+    assert lb is not None, f"oracle build {request.param} missing: run `make -C oracle`"
+    return lb
+
+
+def approx(a, b, eps=1e-6):  # CHECK_APPROX (TestMacros.h:13)
+    return abs(a - b) < eps
+
+
+# ------------------------------------------------------------------ es_math
+
+def _ulps(a, b):
+    if a == b or (a != a and b != b):
+        return 0.0
+    return abs(a - b) / math.ulp(b)
+
+
+@pytest.mark.parametrize("fn,ref,lo,hi", [
+    ("m_sin", math.sin, -7.0, 13.0), ("m_cos", math.cos, -7.0, 13.0),
+    ("m_sin", math.sin, -1e-3, 1e-3), ("m_cos", math.cos, -100.0, 100.0),
+    ("m_asin", math.asin, -1.0, 1.0), ("m_asin", math.asin, 0.97, 1.0),
+    ("m_exp", math.exp, -20.0, 0.0), ("m_exp", math.exp, -745.0, 709.0),
+])
+def test_es_math_within_one_ulp_of_glibc(fn, ref, lo, hi):
+    esm = O.load("esm")
+    f = getattr(esm, fn)
+    xs = np.random.default_rng(3).uniform(lo, hi, 40000)
+    worst = max(_ulps(f(float(x)), ref(float(x))) for x in xs)
+    assert worst <= 1.0, worst
+
+
+def test_es_math_special_values():
+    esm = O.load("esm")
+    assert esm.m_exp(-0.0) == 1.0 and esm.m_exp(0.0) == 1.0
+    assert esm.m_sin(0.0) == 0.0 and math.copysign(1, esm.m_sin(-0.0)) == -1
+    assert esm.m_cos(0.0) == 1.0
+    assert esm.m_asin(1.0) == math.asin(1.0) and esm.m_asin(-1.0) == -math.asin(1.0)
+    assert math.isnan(esm.m_asin(1.0000000000000002)) and math.isnan(esm.m_asin(float("nan")))
+    assert esm.m_exp(-800.0) == 0.0 and math.isinf(esm.m_exp(710.0))
+    for k in range(-16, 17):  # multiples of pi/2: reduction near zero remainders
+        x = k * HALF_PI
+        assert _ulps(esm.m_sin(x), math.sin(x)) <= 1 or abs(esm.m_sin(x) - math.sin(x)) < 1e-31
+
+
+# -------------------------------------------- oracle (glibc) vs reference headers
+
+@pytest.fixture(scope="module")
+def ref():
+    r = O.load("ref")
+    if r is None:
+        pytest.skip("oracle/_ref/libesref.so not built (reference tree absent)")
+    return r
+
+
+def _same(a, b):
+    return a == b or (a != a and b != b)
+
+
+def test_geometry_bit_exact_vs_reference_headers(ref):
+    L = O.load("libm")
+    rng = np.random.default_rng(7)
+    import ctypes as C
+    for k in range(6000):
+        cx, cy = rng.uniform(-2e-6, 2e-6, 2)
+        R = 10 ** rng.uniform(-7.5, -5)
+        cw = int(rng.integers(0, 2))
+        ang = rng.uniform(0, 2 * math.pi)
+        px, py = cx + R * math.cos(ang), cy + R * math.sin(ang)
+        if k % 7 == 0:
+            py = cy + R
+        assert _same(L.get_angle(px, py, cx, cy, R, cw), ref.get_angle(px, py, cx, cy, R, cw))
+        ipx, ipy = px + rng.normal() * 2e-8, py + rng.normal() * 2e-8
+        ir = 10 ** rng.uniform(-9, -7.7)
+        assert L.circles_cross(cx, cy, R, ipx, ipy, ir) == ref.circles_cross(cx, cy, R, ipx, ipy, ir)
+        o1, o2 = (C.c_double * 4)(), (C.c_double * 4)()
+        L.cross_points(cx, cy, R, ipx, ipy, ir, o1)
+        ref.cross_points(cx, cy, R, ipx, ipy, ir, o2)
+        assert all(_same(o1[i], o2[i]) for i in range(4))
+        pa = rng.uniform(0, 2 * math.pi)
+        w = 10 ** rng.uniform(10, 12.5)
+        lo, hi = rng.uniform(0, 2 * math.pi, 2)
+        if k % 5 == 0:
+            hi = lo
+        assert _same(L.first_cross_time(cx, cy, R, cw, pa, ipx, ipy, ir, w, lo, hi),
+                     ref.first_cross_time(cx, cy, R, cw, pa, ipx, ipy, ir, w, lo, hi))
+        phi, alpha = rng.uniform(-1, 7), rng.uniform(0, math.pi / 4)
+        for coh in (0, 1):
+            for el in (0, 1):
+                for fu in (0, 1):
+                    assert _same(L.bound_time(phi, alpha, w, coh, el, fu), ref.bound_time(phi, alpha, w, coh, el, fu))
+        assert _same(L.bound_angle(phi, alpha, cw), ref.bound_angle(phi, alpha, cw))
+        assert L.angle_in_range(pa, lo, hi, cw) == ref.angle_in_range(pa, lo, hi, cw)
+        assert _same(L.cross_angle(pa, lo, cw), ref.cross_angle(pa, lo, cw))
+        lt, tau = rng.uniform(0, 1e-11), 10 ** rng.uniform(-13, -11)
+        assert _same(L.get_sigma(lt, phi, tau, w), ref.get_sigma(lt, phi, tau, w))
+        assert _same(L.simpson_weight(k % 9, 9), ref.simpson_weight(k % 9, 9))
+        start, size, cpr = -rng.uniform(0, 1e-6), rng.uniform(1e-7, 4e-6), int(rng.integers(1, 200))
+        c1, c2 = (C.c_int * 2)(), (C.c_int * 2)()
+        L.get_cell(px, py, start, size, cpr, c1)
+        ref.get_cell(px, py, start, size, cpr, c2)
+        assert list(c1) == list(c2)
+
+
+def test_next_cell_traces_bit_exact_vs_reference_headers(ref):
+    import ctypes as C
+    L = O.load("libm")
+    rng = np.random.default_rng(11)
+    steps = 0
+    for _ in range(1500):
+        cpr = int(rng.integers(1, 40))
+        start = -rng.uniform(1e-7, 1e-6)
+        size = rng.uniform(5e-7, 4e-6)
+        Lc = size / cpr
+        R = 10 ** rng.uniform(-7.3, -5.3)
+        cw = int(rng.integers(0, 2))
+        cell = [int(rng.integers(0, cpr)), int(rng.integers(0, cpr))]
+        px = start + (cell[0] + rng.uniform(0, 1)) * Lc
+        py = start + (cell[1] + rng.uniform(0, 1)) * Lc
+        phi = rng.uniform(0, 2 * math.pi)
+        vx, vy = math.cos(phi), math.sin(phi)
+        cx, cy = (px - vy * R, py + vx * R) if cw else (px + vy * R, py - vx * R)
+        state = {}
+        for nm, lb in (("o", L), ("r", ref)):
+            state[nm] = ((C.c_double * 4)(px, py, phi, 2 * math.pi), (C.c_int * 2)(*cell))
+        for _it in range(300):
+            outs = {}
+            for nm, lb in (("o", L), ("r", ref)):
+                last, lc = state[nm]
+                n, nc = (C.c_double * 4)(), (C.c_int * 2)()
+                ok = lb.next_cell(cx, cy, R, cw, phi, start, size, Lc, cpr, last, lc, n, nc)
+                outs[nm] = (ok, n, nc)
+            steps += 1
+            (ok1, n1, c1), (ok2, n2, c2) = outs["o"], outs["r"]
+            assert ok1 == ok2 and list(c1) == list(c2) and all(_same(n1[i], n2[i]) for i in range(4))
+            if not ok1:
+                break
+            state = {"o": (n1, c1), "r": (n2, c2)}
+    assert steps > 5000
+
+
+# --------------------------------------------- reference known-answer tests
+
+def test_angle_kats(lib):  # tests/es/AngleTests.h
+    g = lib.get_angle
+    assert approx(g(1, 0, 0, 0, 1, 1), HALF_PI) and approx(g(-1, 0, 0, 0, 1, 1), 3 * HALF_PI)
+    assert approx(g(0, 1, 0, 0, 1, 1), PI) and approx(g(0, -1, 0, 0, 1, 1), 0)
+    assert approx(g(-1, 0, 0, 0, 1, 0), HALF_PI) and approx(g(0, 1, 0, 0, 1, 0), 0)
+    assert approx(g(1, 0, 0, 0, 1, 0), 3 * HALF_PI) and approx(g(0, -1, 0, 0, 1, 0), PI)
+    assert approx(g(5, 5, 6, 6, math.sqrt(2), 1), 7 * PI / 4)
+    assert approx(g(5, 5, 6, 6, math.sqrt(2), 0), 3 * PI / 4)
+    air = lib.angle_in_range
+    assert air(1, 0.5, 1.5, 0) and not air(1, 0.5, 1.5, 1) and air(2, 0.5, 1.5, 1) and air(0.1, 0.5, 1.5, 1)
+    assert not air(1, 1.5, 0.5, 0) and air(1, 1.5, 0.5, 1) and air(1, 1, 1, 1) and air(2, 1, 1, 1)
+    assert lib.cross_angle(6.1, 0.1, 1) == 6 and approx(lib.cross_angle(6.1, 0.1, 0), PI2 - 6.0)
+
+
+def test_orbit_detail_kats(lib):  # tests/es/OrbitDetailsTest.h
+    import ctypes as C
+    ins = lib.inside_impurity
+    assert ins(1, 1, 0, 0, 2) and ins(0, 0, 1, 1, 2) and not ins(2, 1, 0, 0, 2) and not ins(1, 0, 0, 0, 1)
+    pos, vel = (1e-6, 1e-6), (7e5, 0.0)
+    wc = E_CH * 2.5 / 9.10938188e-31
+    vf = math.sqrt(vel[0] ** 2)
+    R = vf / wc
+    out = (C.c_double * 2)()
+    lib.orbit_center(*pos, *vel, R, vf, 1, out)
+    assert out[0] == pos[0] and out[1] == pos[1] + R
+    lib.orbit_center(*pos, *vel, R, vf, 0, out)
+    assert out[0] == pos[0] and out[1] == pos[1] - R
+    # GetBoundTime KATs call (phi, w, alpha, ...) against the (phi, alpha, w, ...) signature (:65-115)
+    bt = lib.bound_time
+    assert bt(-0.25, 0.5, 1, 0, 0, 1) == 0.25 and bt(-0.25, 0.5, 1, 0, 1, 1) == 0.75
+    assert bt(-0.25, 0.5, 1, 0, 1, 0) == 0.25 and bt(-0.25, 0.5, 1, 0, 0, 0) == 0.75
+    for phi in (-0.25, PI / 2 - 0.25, PI - 0.25, 3 * PI / 2 - 0.25):
+        assert bt(phi, 0.5, 1, 0, 0, 1) == 0.25
+    assert bt(0.5, 0.5, 1, 0, 0, 1) == 1 and bt(0.5, 0.5, 1, 0, 0, 0) == 0
+    cc = lib.circles_cross
+    assert not cc(1, 2, 1, 5, 5, 1) and not cc(1, 2, 1, 1, 3, 5) and cc(0, 0, 1, 0, 0.5, 0.6) and cc(0, 0, 1000, 2, 0, 1001)
+    cp = (C.c_double * 4)()
+    lib.cross_points(-1, 0, 1.5, 1, 0, 1.5, cp)
+    assert cp[0] == 0 and abs(cp[1]) == math.sqrt(1.5 * 1.5 - 1) and cp[2] == 0 and abs(cp[3]) == math.sqrt(1.5 * 1.5 - 1)
+    for (c, R_, p, r) in (((0, 0), 1, (1, 1), 1.5), ((100, 0), 100, (-1, 0), 1.5), ((100, -0.5), 100, (1, 0.5), 1.5)):
+        lib.cross_points(c[0], c[1], R_, p[0], p[1], r, cp)
+        for k in (0, 2):
+            assert approx(R_ ** 2, (cp[k] - c[0]) ** 2 + (cp[k + 1] - c[1]) ** 2)
+            assert approx(r ** 2, (cp[k] - p[0]) ** 2 + (cp[k + 1] - p[1]) ** 2)
+    # Cross Time (:192-217)
+    pa1 = lib.get_angle(3, 4, 0, 0, 5, 1)
+    pa2 = lib.get_angle(3, 4, 0, 0, 5, 0)
+    t = lib.first_cross_time(0, 0, 5, 1, pa1, 5, 0, 0.1, 2, 0, 0)
+    assert abs(t - 0.907 / 2) < 0.01
+    t2 = lib.first_cross_time(0, 0, 5, 0, pa2, 5, 0, 0.1, 2, -0.0, 0)
+    assert approx(t + t2, (PI2 - (0.1 * 2.0 / 5)) / 2)
+    t = lib.first_cross_time(0, 0, 5, 1, pa1, 5, 0, 0.059, 100, 0, 0)
+    t2 = lib.first_cross_time(0, 0, 5, 0, pa2, 5, 0, 0.059, 100, 0, 0)
+    assert approx(t + t2, (PI2 - (0.059 * 2.0 / 5)) / 100)
+
+
+def test_index_kats(lib):  # tests/es/IndexTests.h
+    import ctypes as C
+    dp = lib.different_point
+    assert not dp(5e-7, 3.063904e-07, 5e-7, 3.063904e-07, 3e-7) and dp(5e-7, 3.063904e-07, 2e-7, 3.063904e-07, 3e-7)
+    w = (C.c_double * 2)()
+    lib.to_world(5, 4, -1e-6, 3e-6 / 10, w)
+    assert w[0] == -1e-6 + 5 / 10 * 3e-6 and abs(w[1] - (-1e-6 + 4 / 10 * 3e-6)) < 1e-13
+    lib.to_world(2, 3, -10, 3.0, w)
+    assert w[0] == -10 + 2 * 3.0 and w[1] == -10 + 3 * 3.0
+    c = (C.c_int * 2)()
+    lib.get_cell(0, 0, 0, 0.4, 6, c)
+    assert list(c) == [0, 0]
+    lib.get_cell(0, 0, -0.1, 0.5, 6, c)
+    assert list(c) == [1, 1]
+    lib.get_cell(0.15, 0.15, -0.1, 0.5, 9, c)
+    assert list(c) == [4, 4]
+    lib.get_cell(0.399, 0.399, -0.1, 0.5, 9, c)
+    assert list(c) == [8, 8]
+    assert lib.get_index(0, 0, 30) == 0 and lib.get_index(3, 2, 30) == 63 and lib.get_index(2, 3, 30) == 92
+    assert lib.get_index(4, 3, 10) == 34
+    for (x, y, ok) in ((-1, 3, 0), (3, -3, 0), (30, 3, 0), (30, 12, 0), (0, 0, 1), (12, 12, 1), (29, 29, 1)):
+        assert lib.within_bounds(x, y, 30) == ok
+
+
+def test_simpson_kats(lib):  # tests/sim/SimpsonWeightsTest.h
+    sw = lib.simpson_weight
+    assert [sw(i, 7) for i in range(7)] == [1, 4, 2, 4, 2, 4, 1]
+    w2 = lambda i, j: sw(i, 7) * sw(j, 7)
+    assert (w2(6, 0), w2(0, 0), w2(1, 0), w2(2, 0), w2(3, 0), w2(0, 1), w2(1, 1), w2(3, 3), w2(1, 2), w2(2, 1),
+            w2(0, 6)) == (1, 1, 4, 2, 4, 4, 16, 16, 8, 8, 1)
+
+
+def test_simpson_exact_for_cubics(lib):
+    for n in (3, 7, 17, 65):
+        xs = np.linspace(0, 2, n)
+        h = xs[1] - xs[0]
+        w = np.array([lib.simpson_weight(i, n) for i in range(n)])
+        f = 1 + 2 * xs - xs ** 2 + 0.5 * xs ** 3
+        assert abs((w @ f) * h / 3 - (2 + 4 - 8 / 3 + 2)) < 1e-12
+
+
+def test_next_cell_reference_traces(lib):  # tests/es/GridMovementTests.h:43-265
+    import ctypes as C
+    nc = FX["next_cell"]
+    start, end = nc["spawn_range"]
+    size = end - start
+    cpr = nc["cells_per_row"]
+    L = size / cpr
+    for s in nc["setups"]:
+        last = (C.c_double * 4)(s["start"][0], s["start"][1], s["incident_angle"], PI2)
+        lc = (C.c_int * 2)(*s["cell"])
+        i = 0
+        while True:
+            n, ncell = (C.c_double * 4)(), (C.c_int * 2)()
+            ok = lib.next_cell(s["center"][0], s["center"][1], s["radius"], int(s["clockwise"]), 0.0, start, size, L,
+                               cpr, last, lc, n, ncell)
+            if not ok:
+                assert i == len(s["expected"])
+                break
+            e = s["expected"][i]
+            assert abs(n[0] - e["x"]) < 1e-13 and abs(n[1] - e["y"]) < 1e-13
+            assert abs(n[2] - e["angle"]) < 1e-13
+            assert list(ncell) == e["cell"]
+            last, lc = n, ncell
+            i += 1
+
+
+def test_lifetime_reference_cases(lib):  # tests/es/LifetimeTests.h:10-135
+    lt = FX["lifetime"]
+    x, y = O.read_points(os.path.join(GOLDEN, lt["impurities"]))
+    g = O.OracleGrid.from_points(lib, x, y, lt["region_size"], lt["region_extends"], lt["impurity_radius"], lt["target"])
+    alpha = PI / 4.0
+    for c in lt["cases"]:
+        w = E_CH * c["B"] / M
+        step = c.get("step", 0.0)
+        combos = [(c["cw"], c["coherent"])] if c["cw"] is not None else [(a, b) for a in (0, 1) for b in (0, 1)]
+        for cw, coh in combos:
+            v = w * lib.particle_lifetime(g.h, c["q"], c["p"], c["pos"][0], c["pos"][1], lt["particle_speed"], w,
+                                          alpha, cw, coh, 0.0, step, None)
+            if "gt" in c:
+                assert v > c["gt"], (c["name"], v)
+            if "lt" in c:
+                assert v < c["lt"], (c["name"], v)
+            if "eq" in c:
+                assert v == c["eq"], (c["name"], v)
+            if c.get("eq_pi_over_4"):
+                assert abs(v - PI / 4) < 1e-13, (c["name"], v)
+
+
+# ------------------------------------------------------------------ binning
+
+def test_grid_generation_invariants(lib):  # tests/sim/GridGenerationTests.h (seeded: this build's RNG)
+    g = O.OracleGrid.generate(lib, 0, 1.0, 0.0, 1e2, 1e-2, int(1e2 / 16))
+    xy, idx = g.csr()
+    assert np.all(np.diff(idx) >= 0)
+    assert O.OracleGrid.generate(lib, 0, 1.0, 0.0, 1e2, 1e-3, 4).cells_per_row ** 2 == 25
+    one = O.OracleGrid.generate(lib, 0, 1.0, 0.0, 1e2, 1e-3, int(1e2 + 1))
+    _, idx1 = one.csr()
+    assert idx1.size == 1 and idx1[0] == 100
+
+
+@pytest.mark.parametrize("first", [(0.5, 0.5), (0.505, 0.4999), (0.508, 0.4999), (0.492, 0.4999), (0.508, 0.501),
+                                   (0.492, 0.501)])
+def test_diagonal_kats(lib, first):  # GridGenerationTests.h:191-245
+    pts = np.array([first, (0.1, 0.1), (0.9, 0.9), (0.9, 0.8)])
+    g = O.OracleGrid.from_points(lib, pts[:, 0], pts[:, 1], 1, 0, 1e-2, 1)
+    assert g.cells_per_row == 2 and g.total_indexed == 7
+
+
+def _disk_square_overlap(cx, cy, r, x0, y0, L):
+    nx = min(max(cx, x0), x0 + L)
+    ny = min(max(cy, y0), y0 + L)
+    return (nx - cx) ** 2 + (ny - cy) ** 2 < r * r
+
+
+def test_binning_equals_exact_overlap_for_small_disks(lib):
+    """For r < cell size the probe rule lists exactly the cells the open disk overlaps
+    (ties on cell edges aside, which random centres avoid)."""
+    rng = np.random.default_rng(5)
+    n, rs, ext, r, cpr = 3000, 1.0, 0.25, 0.02, 30
+    x = rng.uniform(-ext, rs + ext, n)
+    y = rng.uniform(-ext, rs + ext, n)
+    g = O.OracleGrid.from_points(lib, x, y, rs, ext, r, 0, cpr)
+    xy, idx = g.csr()
+    L = g.cell_size
+    start = 0
+    for c in range(cpr * cpr):
+        cx0, cy0 = g.spawn_lo + (c % cpr) * L, g.spawn_lo + (c // cpr) * L
+        members = {tuple(p) for p in xy[start:idx[c]]}
+        start = idx[c]
+        want = {(px, py) for px, py in zip(x, y) if _disk_square_overlap(px, py, r, cx0, cy0, L)}
+        assert members == want, c
+
+
+def test_binning_cell_order_is_impurity_order(lib):
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0, 1, 500)
+    y = rng.uniform(0, 1, 500)
+    g = O.OracleGrid.from_points(lib, x, y, 1.0, 0.0, 0.03, 4)
+    xy, idx = g.csr()
+    order = {(a, b): i for i, (a, b) in enumerate(zip(x, y))}
+    start = 0
+    for c in range(idx.size):
+        ids = [order[tuple(p)] for p in xy[start:idx[c]]]
+        assert ids == sorted(ids)
+        start = idx[c]
+
+
+# ---------------------------------------------------------- trace brute force
+
+def _brute_first_hit(px, py, phi, R, cw, w, imps, r):
+    """Smallest travel time to any impurity disk along the full circle, or 0."""
+    vx, vy = math.cos(phi), math.sin(phi)
+    cx, cy = (px - vy * R, py + vx * R) if cw else (px + vy * R, py - vx * R)
+    a0 = math.atan2(py - cy, px - cx)
+    best = math.inf
+    for ix, iy in imps:
+        d = math.hypot(ix - cx, iy - cy)
+        if not (abs(R - r) < d < R + r):
+            continue
+        a = math.atan2(iy - cy, ix - cx)
+        half = math.acos((d * d + R * R - r * r) / (2 * d * R))
+        for ang in (a - half, a + half):
+            da = (a0 - ang) % (2 * math.pi) if cw else (ang - a0) % (2 * math.pi)
+            best = min(best, da / w)
+    return 0.0 if best == math.inf else best
+
+
+def test_trace_equals_brute_force_first_hit(lib):
+    """Orbits that stay inside the grid: the cell walk finds the global first hit."""
+    rng = np.random.default_rng(21)
+    rs, ext, r = 1e-6, 1e-6, 1.5e-8
+    n = 400
+    x = rng.uniform(-ext, rs + ext, n)
+    y = rng.uniform(-ext, rs + ext, n)
+    g = O.OracleGrid.from_points(lib, x, y, rs, ext, r, 5)
+    imps = list(zip(x, y))
+    speed, B = 1.68e5, 10.0
+    w = E_CH * B / M
+    R = speed / w
+    checked = 0
+    for _ in range(300):
+        px, py = rng.uniform(0.2e-6, 0.8e-6, 2)
+        if any((px - a) ** 2 + (py - b) ** 2 < r * r for a, b in imps):
+            continue
+        phi = rng.uniform(0, 2 * math.pi)
+        cw = int(rng.integers(0, 2))
+        got = lib.particle_lifetime(g.h, 0, 0, px, py, speed, w, math.pi / 4, cw, 1, phi, 0.0, None)
+        want = _brute_first_hit(px, py, phi, R, cw, w, imps, r)
+        assert abs(got - want) <= 1e-9 * max(want, 1e-30) + 1e-22, (px, py, phi, cw, got, want)
+        checked += 1
+    assert checked > 200
+
+
+def test_get_sigma_closed_form(lib):
+    """GetSigma = (1+(w tau)^2)/tau * int_0^lt cos(phi + w t) exp(-t/tau) dt (util.h:34-46)."""
+    from scipy.integrate import quad
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        tau = 10 ** rng.uniform(-13, -11)
+        w = rng.choice([-1, 1]) * 10 ** rng.uniform(10, 12.5)
+        lt = rng.uniform(0, 10 * tau)
+        phi = rng.uniform(-1, 7)
+        val, _ = quad(lambda t: math.cos(phi + w * t) * math.exp(-t / tau), 0, lt, limit=400, epsabs=0,
+                      epsrel=1e-12)
+        want = (1 + (w * tau) ** 2) / tau * val
+        got = lib.get_sigma(lt, phi, tau, w)
+        assert abs(got - want) <= 1e-8 * (1 + (w * tau) ** 2) + 1e-9 * abs(want)
+    assert lib.get_sigma(0.0, 0.3, 1e-12, 5e11) == 0.0
+
+
+# ---------------------------------------------------- golden DataRows (CPU)
+
+@pytest.mark.parametrize("key,ppr,r", [("datarows_file1", 31, 2e-9), ("datarows_file2", 127, 1.11e-9)])
+def test_reference_golden_datarows(lib, key, ppr, r):
+    """tests/sim/SigmaIntegrationTest.h:80-141 at the reference tolerance 1e-5/B (TestMacros.h:17)."""
+    block = FX[key]
+    x, y = O.read_points(os.path.join(GOLDEN, block["impurities"]))
+    g = O.OracleGrid.from_points(lib, x, y, 1e-7, 1e-7, r, 10)
+    prm = O.sim_params(ppr, 7, 1e-7, 1e-7, g.cell_size, 1.0, 0.3, 1, 1.67834e5, nthreads=8)
+    temps = sorted({row["T"] for row in block["rows"]})
+    res = {}
+    for B in sorted({row["B"] for row in block["rows"]}):
+        for coherent in (1, 0):
+            sig, m, _ = O.compute_sigmas(lib, g, prm, coherent, B, temps)
+            assert m.pm.particles_escaped == 0
+            for t, T in enumerate(temps):
+                res[(coherent, B, T)] = sig[t]
+    for row in block["rows"]:
+        tol = 1e-5 / row["B"]
+        for coherent, kx, ky in ((1, "coh_xx", "coh_xy"), (0, "inc_xx", "inc_xy")):
+            s = res[(coherent, row["B"], row["T"])]
+            assert abs((s.xx_naive - row[kx]) / row[kx]) < tol
+            assert abs((s.xy_naive - row[ky]) / row[ky]) < tol
+            assert abs(s.xx_comp - s.xx_naive) <= 1e-10 * abs(s.xx_naive)
+
+
+def test_oracle_builds_agree_end_to_end():
+    """glibc vs es_math oracle: same lifetimes except where a libm ulp flips a branch."""
+    L, E = O.load("libm"), O.load("esm")
+    x, y = O.read_points(os.path.join(GOLDEN, "test_impurities.dat"))
+    out = []
+    for lb in (L, E):
+        g = O.OracleGrid.from_points(lb, x, y, 1e-7, 1e-7, 2e-9, 10)
+        prm = O.sim_params(31, 7, 1e-7, 1e-7, g.cell_size, 1.0, 0.3, 1, 1.67834e5, nthreads=8)
+        out.append(O.compute_sigmas(lb, g, prm, 0, 24.006, [1.0, 10.0], want_lifetimes=True))
+    la, lb_ = out[0][2], out[1][2]
+    rel = np.abs(la - lb_) / np.maximum(np.abs(la), 1e-300)
+    assert np.mean(rel > 1e-9) < 1e-3
+    for s1, s2 in zip(out[0][0], out[1][0]):
+        assert abs(s1.xx_comp - s2.xx_comp) <= 1e-8 * abs(s1.xx_comp)
