@@ -23,21 +23,25 @@ sys.path.insert(0, ROOT)
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic FP64 operations per event of the trace kernel (add, sub, mul,
-# div, sqrt, floor, rint = 1; fma = 2), counted from the operation sequence of
-# the reference functions as restated in es_trace.h / es_math.h.
-WORK_MODEL = {
-    "traced_particles": 91,   # CreateParticle (sincos 58), start cell, Simpson weight, lifetime sum
+# Algorithmic FP64 operations per event (add, sub, mul, div, sqrt, floor,
+# rint = 1; fma = 2), counted from the operation sequence of the reference
+# functions as restated in es_trace.h / es_math.h.  The trace kernel
+# (k_trace) does CreateParticle + TraceOrbit; k_sigma the temperature stage.
+TRACE_WORK = {
+    "traced_particles": 85,   # CreateParticle (sincos 58, orbit centre, bound time), start cell
     "cells_passed": 95,       # GetNextCell: corners, 4 edge projections, snap, stop test
     "edge_hits": 52,          # per accepted edge intersection: sqrt, points, GetAngle (asin), GetCrossAngle
     "impurities_tested": 5,   # CirclesCross
     "cross_calls": 121,       # GetCrossPoints + 2 GetAngle + 2 AngleInRange + times
-    "sigma_evals": 82,        # truncation, exp, sincos, GetSigma, two weighted terms, compensated adds
+}
+SIGMA_WORK = {
+    "traced_particles": 70,   # phi, sincos, Simpson weight, lifetime sum
+    "sigma_evals": 78,        # truncation, exp, sincos, GetSigma, two weighted terms, compensated adds
 }
 
 
-def algorithmic_flops(m):
-    return sum(WORK_MODEL[k] * m[k] for k in WORK_MODEL)
+def algorithmic_flops(m, model):
+    return sum(model[k] * m[k] for k in model)
 
 
 def cpu_baseline(w, seconds_target=15.0):
@@ -128,13 +132,15 @@ def main():
             sig, ms = sim.compute_sweep(fields, temps)
             if stats is not None and coherent == 0:
                 stats["launch_s"].append(sum(m["time_elapsed_lifetimes"] for m in ms))
-                stats["flops"].append(sum(algorithmic_flops(m) for m in ms))
+                stats["flops"].append(sum(algorithmic_flops(m, TRACE_WORK) for m in ms))
+                stats["sigma_flops"].append(sum(algorithmic_flops(m, SIGMA_WORK) for m in ms))
+                stats["sigma_s"].append(sum(m["time_elapsed_temperatures"] for m in ms))
                 stats["traced"] += sum(m["traced_particles"] for m in ms)
                 stats["sigma"].append(sig)
 
     for s in range(args.warmup):
         step(s, None)
-    stats = {"launch_s": [], "flops": [], "traced": 0, "sigma": []}
+    stats = {"launch_s": [], "flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0, "sigma": []}
     t0 = time.perf_counter()
     for s in range(args.steps):
         step(s, stats)
@@ -144,6 +150,11 @@ def main():
     avg_launch = sum(stats["launch_s"]) / len(stats["launch_s"])
     avg_flops = sum(stats["flops"]) / len(stats["flops"])
     achieved = avg_flops / avg_launch / 1e12
+    sigma_tf = (sum(stats["sigma_flops"]) / sum(stats["sigma_s"]) / 1e12) if sum(stats["sigma_s"]) > 0 else None
+    traffic = None
+    tf_path = os.path.join(ROOT, "profiles", "k_trace_production_fp64_traffic.json")
+    if w is PRODUCTION_FP64 and os.path.exists(tf_path):
+        traffic = json.load(open(tf_path)).get("hbm_bytes_per_launch")
     line = {
         "metric": "orbit lifetimes/sec + sigma(B) sweep wall-time at <=1e-3 rel err vs FP64",
         "value": lifetimes / dt,
@@ -168,14 +179,15 @@ def main():
         "traced_lifetimes_per_s": stats["traced"] / dt,
         "roofline": {
             "bound": "fp64-valu",
-            "kernel": "k_trace (lifetime + sigma, fused)",
+            "kernel": "k_trace (CreateParticle + TraceOrbit, one launch per sample and pass, all fields)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": avg_launch * 1e3,
             "algorithmic_flops_per_launch": avg_flops,
+            "sigma_stage_tflops": sigma_tf,
         },
     }
     if not args.no_cpu_baseline:

@@ -49,14 +49,14 @@ struct es_solver {
     bool has_grid = false;
     es::GridDev g;
     int64_t unique = 0;
-    DevBuf in_x, in_y, ix, iy, cell_end;
-    DevBuf b_targets, b_ntarg, b_counts, b_fill, b_begin, b_order;
+    DevBuf in_x, in_y, ixy, cell_end;
+    DevBuf b_targets, b_ntarg, b_counts, b_fill, b_begin, b_bsum, b_order;
 
     // positions (per grid and solver)
     bool pos_valid = false;
     int nfree = 0;
     int64_t n_inside_positions = 0;
-    DevBuf flags, free_pos, small_i32;  // small_i32: [0] err, [1] nfree
+    DevBuf flags, bcount, free_pos, small_i32;  // small_i32: [0] err, [1] nfree
 
     // sample (InitSample)
     bool has_sample = false;
@@ -66,7 +66,10 @@ struct es_solver {
 
     // per call
     DevBuf w, factor, dpart, mpart, sig, ltsum, mout, lifetimes;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    DevBuf lt_item, m_item, f_item, queue;
+    int cus = 0;                      // compute units of the device
+    int trace_bpc[2] = {0, 0};        // resident trace blocks per CU (counter-clockwise, clockwise)
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -153,10 +156,11 @@ int bin_uploaded(es_solver* s, int64_t n, const es_grid_desc* d, es_grid_info* o
     if ((rc = ensure(s, s->b_counts, ncell * sizeof(int32_t), "cell counts"))) return rc;
     if ((rc = ensure(s, s->b_fill, ncell * sizeof(int32_t), "cell fill"))) return rc;
     if ((rc = ensure(s, s->b_begin, (ncell + 1) * sizeof(int64_t), "cell begin"))) return rc;
+    if ((rc = ensure(s, s->b_bsum, ((ncell + 255) / 256) * sizeof(int64_t), "scan block sums"))) return rc;
     if ((rc = ensure(s, s->cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
     g.cell_end = P<int32_t>(s->cell_end);
     es::BinScratch bs{P<int32_t>(s->b_targets), P<uint8_t>(s->b_ntarg), P<int32_t>(s->b_counts),
-                      P<int32_t>(s->b_fill), P<int64_t>(s->b_begin), nullptr};
+                      P<int32_t>(s->b_fill), P<int64_t>(s->b_begin), P<int64_t>(s->b_bsum), nullptr};
     const double* x = P<double>(s->in_x);
     const double* y = P<double>(s->in_y);
     ES_HIP(s, es::launch_bin_count(x, y, n, g, bs, s->stream));
@@ -167,11 +171,9 @@ int bin_uploaded(es_solver* s, int64_t n, const es_grid_desc* d, es_grid_info* o
     ES_HIP(s, hipStreamSynchronize(s->stream));
     if (total > INT32_MAX) return fail(s, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
     if ((rc = ensure(s, s->b_order, total * sizeof(int32_t), "cell order"))) return rc;
-    if ((rc = ensure(s, s->ix, total * sizeof(double), "impurity x"))) return rc;
-    if ((rc = ensure(s, s->iy, total * sizeof(double), "impurity y"))) return rc;
+    if ((rc = ensure(s, s->ixy, total * sizeof(double2), "impurities"))) return rc;
     bs.order = P<int32_t>(s->b_order);
-    g.ix = P<double>(s->ix);
-    g.iy = P<double>(s->iy);
+    g.ixy = P<double2>(s->ixy);
     g.total = total;
     ES_HIP(s, es::launch_bin_place(x, y, n, g, bs, s->stream));
     ES_HIP(s, hipStreamSynchronize(s->stream));
@@ -210,14 +212,15 @@ int ensure_positions(es_solver* s)
     int rc;
     if ((rc = ensure(s, s->flags, n, "position flags"))) return rc;
     if ((rc = ensure(s, s->free_pos, n * sizeof(int32_t), "free positions"))) return rc;
+    if ((rc = ensure(s, s->bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
     if ((rc = ensure(s, s->small_i32, 2 * sizeof(int32_t), "flags"))) return rc;
     // Simulation.h:142-143
     const double dist = s->desc.grid_info.region_size / (double)(ppr - 1);
     const double offx = s->desc.grid_info.cell_size * 0.01;
     const double offy = s->desc.grid_info.cell_size * 0.005;
     int32_t* small = P<int32_t>(s->small_i32);
-    ES_HIP(s, es::launch_positions(s->g, ppr, dist, offx, offy, P<uint8_t>(s->flags), small, P<int32_t>(s->free_pos),
-                                   small + 1, s->stream));
+    ES_HIP(s, es::launch_positions(s->g, ppr, dist, offx, offy, P<uint8_t>(s->flags), P<int32_t>(s->bcount), small,
+                                   P<int32_t>(s->free_pos), small + 1, s->stream));
     int32_t host[2] = {0, 0};
     ES_HIP(s, hipMemcpyAsync(host, small, sizeof(host), hipMemcpyDeviceToHost, s->stream));
     ES_HIP(s, hipStreamSynchronize(s->stream));
@@ -278,9 +281,11 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     const int ppr = s->desc.positions_per_row, ppq = s->desc.particles_per_quadrant, ppp = 4 * ppq;
     const int64_t P_ = (int64_t)ppr * ppr * ppp;
     const int64_t items = (int64_t)s->nfree * ppp;
-    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((items + es::kBlock - 1) / es::kBlock, 2048));
+    const int64_t total_items = items * nB;
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((items + es::kBlock - 1) / es::kBlock, 1024));
     const int nTc = std::max(1, std::min(nT, es::kMaxT));
     const int nq = 2 * nTc + 1;
+    const int cw = s->st.is_clockwise ? 1 : 0;
 
     if ((rc = ensure(s, s->w, nB * sizeof(double), "fields"))) return rc;
     if ((rc = ensure(s, s->factor, (size_t)nB * es::kMaxT * sizeof(double), "factors"))) return rc;
@@ -289,18 +294,26 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     if ((rc = ensure(s, s->sig, (size_t)nB * es::kMaxT * 2 * sizeof(double), "sigma out"))) return rc;
     if ((rc = ensure(s, s->ltsum, nB * sizeof(double), "lifetime sums"))) return rc;
     if ((rc = ensure(s, s->mout, (size_t)nB * es::kMetricSlots * sizeof(int64_t), "metrics out"))) return rc;
+    if ((rc = ensure(s, s->lt_item, total_items * sizeof(double), "item lifetimes"))) return rc;
+    if ((rc = ensure(s, s->m_item, total_items * sizeof(int4), "item counters"))) return rc;
+    if ((rc = ensure(s, s->f_item, total_items, "item flags"))) return rc;
+    if ((rc = ensure(s, s->queue, sizeof(unsigned long long), "work queue"))) return rc;
     if (lifetimes_host) {
         if ((rc = ensure(s, s->lifetimes, (size_t)nB * P_ * sizeof(double), "lifetimes"))) return rc;
         ES_HIP(s, hipMemsetAsync(s->lifetimes.p, 0, (size_t)nB * P_ * sizeof(double), s->stream));
+    }
+    if (s->trace_bpc[cw] <= 0) {
+        ES_HIP(s, es::trace_blocks_per_cu(cw, &s->trace_bpc[cw]));
+        if (s->trace_bpc[cw] <= 0) s->trace_bpc[cw] = 1;
     }
 
     std::vector<double> w(nB);
     for (int b = 0; b < nB; b++) w[b] = kE * B[b] / kM;  // SimulationCPU.cpp:107
     ES_HIP(s, hipMemcpyAsync(s->w.p, w.data(), nB * sizeof(double), hipMemcpyHostToDevice, s->stream));
 
+    // Lifetime stage: every (field, free position, phi) item once.
     es::TraceArgs a{};
-    a.ix = s->g.ix;
-    a.iy = s->g.iy;
+    a.ixy = s->g.ixy;
     a.cell_end = s->g.cell_end;
     a.g_start = s->g.lo;
     a.g_size = s->g.size;
@@ -319,39 +332,66 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     a.alpha = s->st.alpha;
     a.speed = s->st.particle_speed;
     a.coherent = s->coherent;
-    a.clockwise = s->st.is_clockwise ? 1 : 0;
+    a.clockwise = cw;
     a.nB = nB;
     a.w = P<double>(s->w);
-    a.dpart = P<double>(s->dpart);
-    a.mpart = P<int64_t>(s->mpart);
-    a.nblk = nblk;
+    a.queue = P<unsigned long long>(s->queue);
+    a.chunk = 128;
+    a.lt_item = P<double>(s->lt_item);
+    a.m_item = P<int4>(s->m_item);
+    a.f_item = P<uint8_t>(s->f_item);
+    a.lifetimes = lifetimes_host ? P<double>(s->lifetimes) : nullptr;
+    const int64_t want_blocks = (total_items + es::kBlock - 1) / es::kBlock;
+    const int tblocks = (int)std::max<int64_t>(1, std::min<int64_t>(want_blocks, (int64_t)s->trace_bpc[cw] * s->cus));
+
+    ES_HIP(s, hipEventRecord(s->ev[0], s->stream));
+    if (total_items > 0) ES_HIP(s, es::launch_trace(a, tblocks, s->stream));
+    ES_HIP(s, hipEventRecord(s->ev[1], s->stream));
+
+    // Temperature stage, at most kMaxT temperatures per launch.
+    es::SigmaArgs sa{};
+    sa.lt_item = a.lt_item;
+    sa.m_item = a.m_item;
+    sa.f_item = a.f_item;
+    sa.free_pos = a.free_pos;
+    sa.nfree = s->nfree;
+    sa.ppr = ppr;
+    sa.ppq = ppq;
+    sa.phi_start = s->phi_start;
+    sa.phi_step = s->phi_step;
+    sa.clockwise = cw;
+    sa.nB = nB;
+    sa.w = a.w;
+    sa.dpart = P<double>(s->dpart);
+    sa.mpart = P<int64_t>(s->mpart);
+    sa.nblk = nblk;
 
     std::vector<double> sig((size_t)nB * es::kMaxT * 2);
     std::vector<double> ltsum(nB);
     std::vector<int64_t> mh((size_t)nB * es::kMetricSlots);
-    double t_life = 0, t_temp = 0;
+    double t_temp = 0;
 
     const int nchunks = nT == 0 ? 1 : (nT + es::kMaxT - 1) / es::kMaxT;
     for (int c = 0; c < nchunks; c++) {
         const int t0 = c * es::kMaxT;
         const int nt = nT == 0 ? 0 : std::min(es::kMaxT, nT - t0);
-        a.nT = nt;
+        sa.nT = nt;
+        sa.with_metrics = (c == 0) ? 1 : 0;
         for (int t = 0; t < es::kMaxT; t++) {
-            a.tau[t] = t < nt ? taus[t0 + t] : 1.0;
-            a.max_lt[t] = 15.0 * a.tau[t];  // Simulation.h:91
+            sa.tau[t] = t < nt ? taus[t0 + t] : 1.0;
+            sa.max_lt[t] = 15.0 * sa.tau[t];  // Simulation.h:91
         }
-        a.lifetimes = (lifetimes_host && c == 0) ? P<double>(s->lifetimes) : nullptr;
         std::vector<double> fac((size_t)nB * std::max(nt, 1), 0.0);
         for (int b = 0; b < nB; b++)
             for (int t = 0; t < nt; t++) fac[(size_t)b * nt + t] = integrand_factor(s, w[b], taus[t0 + t]);
         ES_HIP(s, hipMemcpyAsync(s->factor.p, fac.data(), fac.size() * sizeof(double), hipMemcpyHostToDevice,
                                  s->stream));
-        ES_HIP(s, hipEventRecord(s->ev[0], s->stream));
-        ES_HIP(s, es::launch_trace(a, s->stream));
-        ES_HIP(s, hipEventRecord(s->ev[1], s->stream));
-        ES_HIP(s, es::launch_finalize(P<double>(s->dpart), P<int64_t>(s->mpart), nB, nblk, nt, P<double>(s->factor),
-                                      P<double>(s->sig), P<double>(s->ltsum), P<int64_t>(s->mout), s->stream));
         ES_HIP(s, hipEventRecord(s->ev[2], s->stream));
+        ES_HIP(s, es::launch_sigma(sa, s->stream));
+        ES_HIP(s, es::launch_finalize(P<double>(s->dpart), P<int64_t>(s->mpart), nB, nblk, nt, sa.with_metrics,
+                                      P<double>(s->factor), P<double>(s->sig), P<double>(s->ltsum), P<int64_t>(s->mout),
+                                      s->stream));
+        ES_HIP(s, hipEventRecord(s->ev[3], s->stream));
         ES_HIP(s, hipMemcpyAsync(sig.data(), s->sig.p, (size_t)nB * std::max(nt, 1) * 2 * sizeof(double),
                                  hipMemcpyDeviceToHost, s->stream));
         if (c == 0) {
@@ -363,10 +403,8 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
                                          hipMemcpyDeviceToHost, s->stream));
         }
         ES_HIP(s, hipStreamSynchronize(s->stream));
-        float ms1 = 0, ms2 = 0;
-        ES_HIP(s, hipEventElapsedTime(&ms1, s->ev[0], s->ev[1]));
-        ES_HIP(s, hipEventElapsedTime(&ms2, s->ev[1], s->ev[2]));
-        t_life += ms1 * 1e-3;
+        float ms2 = 0;
+        ES_HIP(s, hipEventElapsedTime(&ms2, s->ev[2], s->ev[3]));
         t_temp += ms2 * 1e-3;
         for (int b = 0; b < nB; b++)
             for (int t = 0; t < nt; t++) {
@@ -374,6 +412,9 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
                 out[(size_t)b * nT + t0 + t].xy = sig[((size_t)b * nt + t) * 2 + 1];
             }
     }
+    float ms1 = 0;
+    ES_HIP(s, hipEventElapsedTime(&ms1, s->ev[0], s->ev[1]));
+    const double t_life = ms1 * 1e-3;
     if (metrics) {
         const int64_t inside = s->n_inside_positions * ppp;
         for (int b = 0; b < nB; b++) {
@@ -389,7 +430,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
             o.particles_inside_impurity = inside;
             o.real_particles = P_ - inside;
             o.avg_particle_lifetime = P_ ? ltsum[b] / (double)P_ : 0.0;
-            o.time_elapsed_lifetimes = t_life / nB;
+            o.time_elapsed_lifetimes = t_life / nB;  // one launch for all fields: its share
             o.time_elapsed_temperatures = t_temp / nB;
             o.traced_particles = m[7];
             o.edge_hits = m[8];
@@ -446,7 +487,8 @@ int es_solver_create(const es_solver_desc* desc, es_solver** out)
     if (const char* cap = std::getenv("ES_MAX_DEVICE_BYTES")) s->byte_cap = (size_t)std::strtoull(cap, nullptr, 10);
     hipError_t e = hipSetDevice(s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventCreate(&s->ev[i]);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&s->ev[i]);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
     if (e != hipSuccess) {
         g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
         es_solver_destroy(s);
@@ -461,9 +503,10 @@ void es_solver_destroy(es_solver* s)
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    DevBuf* bufs[] = {&s->in_x, &s->in_y, &s->ix, &s->iy, &s->cell_end, &s->b_targets, &s->b_ntarg, &s->b_counts,
-                      &s->b_fill, &s->b_begin, &s->b_order, &s->flags, &s->free_pos, &s->small_i32, &s->w,
-                      &s->factor, &s->dpart, &s->mpart, &s->sig, &s->ltsum, &s->mout, &s->lifetimes};
+    DevBuf* bufs[] = {&s->in_x, &s->in_y, &s->ixy, &s->cell_end, &s->b_targets, &s->b_ntarg, &s->b_counts,
+                      &s->b_fill, &s->b_begin, &s->b_bsum, &s->b_order, &s->flags, &s->bcount, &s->free_pos,
+                      &s->small_i32, &s->w, &s->factor, &s->dpart, &s->mpart, &s->sig, &s->ltsum, &s->mout,
+                      &s->lifetimes, &s->lt_item, &s->m_item, &s->f_item, &s->queue};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t& ev : s->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -519,14 +562,8 @@ int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index)
     ES_HIP(s, hipSetDevice(s->device));
     const int64_t total = s->g.total, ncell = (int64_t)s->g.cpr * s->g.cpr;
     if (ordered_xy && total > 0) {
-        std::vector<double> x(total), y(total);
-        ES_HIP(s, hipMemcpyAsync(x.data(), s->g.ix, total * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-        ES_HIP(s, hipMemcpyAsync(y.data(), s->g.iy, total * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        ES_HIP(s, hipMemcpyAsync(ordered_xy, s->g.ixy, total * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
         ES_HIP(s, hipStreamSynchronize(s->stream));
-        for (int64_t k = 0; k < total; k++) {
-            ordered_xy[2 * k] = x[k];
-            ordered_xy[2 * k + 1] = y[k];
-        }
     }
     if (imp_index) {
         ES_HIP(s, hipMemcpyAsync(imp_index, s->g.cell_end, ncell * sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));

@@ -2,17 +2,23 @@
 //
 //   k_generate      impurity centres from es_rng.h (Grid.cpp:61-75 analogue)
 //   k_bin_count     per impurity: target cells (Grid.cpp:112-180), per-cell counts
-//   k_bin_scan      exclusive scan -> cell begin offsets + reference inclusive ends
+//   k_scan_block /
+//   k_scan_add      two-level exclusive scan -> cell begin offsets + the
+//                   reference's inclusive ends
 //   k_bin_fill      scatter impurity ids into their cells
-//   k_bin_sort      ascending impurity id inside each cell (the reference's order)
-//   k_bin_gather    cell-ordered SoA x[], y[]
+//   k_bin_sort      ascending impurity id inside each cell (the reference's
+//                   order), one wave per cell
+//   k_bin_gather    cell-ordered (x, y) pairs
 //   k_positions     start cell + "inside an impurity" test per position
 //                   (lifetime.h:84-99), hoisted out of the phi and B loops
 //   k_compact       ordered list of free positions (deterministic)
-//   k_trace<CW,NT>  one particle per work item: CreateParticle + TraceOrbit
-//                   (lifetime.h:36-148), truncation, GetSigma, Simpson weights
-//                   (SimulationCPU.cpp:178-222) fused; double-double block
-//                   partials, no atomics
+//   k_trace<CW>     CreateParticle + TraceOrbit (lifetime.h:36-148) per work
+//                   item; a lane whose trace ends takes the next item of its
+//                   wave's queue chunk, so long and short traces share waves;
+//                   per-item raw lifetime + work counters
+//   k_sigma<NT>     truncation, GetSigma, Simpson weights (SimulationCPU.cpp:
+//                   143-222) per item and temperature; double-double block
+//                   partials per field, no atomics
 //   k_finalize      fixed-order double-double sum of the block partials
 //
 // All FP64; compiled with -ffp-contract=off.
@@ -171,32 +177,63 @@ __global__ void k_bin_count(const double* x, const double* y, int64_t n, BinGeom
     }
 }
 
-// Single-block exclusive scan of n int32 -> begin[0..n] (int64).  Optionally
-// also writes inclusive ends as int32 (Grid::imp_index layout).
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* counts, int64_t n, int64_t* begin, int32_t* incl_end)
+// Block-local exclusive scan of 256 int64 values held one per thread.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* sh /* [256] */, int64_t& total)
 {
-    __shared__ int64_t part[1024];
     const int tid = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b0 = tid * chunk;
-    const int64_t b1 = (b0 + chunk < n) ? b0 + chunk : n;
-    int64_t local = 0;
-    for (int64_t k = b0; k < b1; k++) local += counts[k];
-    part[tid] = local;
+    sh[tid] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int64_t v = (tid >= off) ? part[tid - off] : 0;
+    for (int off = 1; off < 256; off <<= 1) {
+        const int64_t add = (tid >= off) ? sh[tid - off] : 0;
         __syncthreads();
-        part[tid] += v;
+        sh[tid] += add;
         __syncthreads();
     }
-    int64_t acc = part[tid] - local;  // exclusive prefix of this chunk
-    for (int64_t k = b0; k < b1; k++) {
-        begin[k] = acc;
-        acc += counts[k];
-        if (incl_end) incl_end[k] = (int32_t)acc;
+    total = sh[255];
+    const int64_t r = sh[tid] - v;
+    __syncthreads();
+    return r;
+}
+
+// Sum of bsum[0 .. nb) by one block (fixed order per thread, then LDS tree).
+__device__ __forceinline__ int64_t block_prefix_of(const int64_t* bsum, int nb, int64_t* sh)
+{
+    int64_t acc = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) acc += bsum[k];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
     }
-    if (tid == 1023) begin[n] = part[1023];
+    const int64_t r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan_block(const int32_t* counts, int64_t n, int64_t* begin, int64_t* bsum)
+{
+    __shared__ int64_t sh[256];
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t v = (c < n) ? counts[c] : 0;
+    int64_t total;
+    const int64_t ex = block_exclusive_scan(v, sh, total);
+    if (c < n) begin[c] = ex;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(const int32_t* counts, int64_t n, int64_t* begin,
+                                                  const int64_t* bsum, int32_t* incl_end)
+{
+    __shared__ int64_t sh[256];
+    const int64_t off = block_prefix_of(bsum, blockIdx.x, sh);
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c < n) {
+        const int64_t b = begin[c] + off;
+        begin[c] = b;
+        incl_end[c] = (int32_t)(b + counts[c]);
+        if (c == n - 1) begin[n] = b + counts[c];
+    }
 }
 
 __global__ void k_bin_fill(int64_t n, const int32_t* targets, const uint8_t* ntarg, const int64_t* begin,
@@ -226,25 +263,26 @@ __device__ void sift_down(int32_t* a, int64_t root, int64_t n)
     }
 }
 
-// Sort each cell's impurity ids ascending: the reference pushes impurities into
-// a cell in index order (Grid.cpp:87-89,129).
-__global__ void k_bin_sort(int64_t ncell, const int64_t* begin, int32_t* order)
+// Ascending impurity ids inside each cell: the reference pushes impurities into
+// a cell in index order (Grid.cpp:87-89,129).  One wave per cell; ids in a
+// cell are distinct, so a lane's rank is the count of smaller ids.
+__global__ __launch_bounds__(256) void k_bin_sort(int64_t ncell, const int64_t* begin, int32_t* order)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (c >= ncell) return;
     int32_t* a = order + begin[c];
     const int64_t n = begin[c + 1] - begin[c];
-    if (n <= 32) {
-        for (int64_t k = 1; k < n; k++) {
-            const int32_t v = a[k];
-            int64_t m = k - 1;
-            while (m >= 0 && a[m] > v) {
-                a[m + 1] = a[m];
-                m--;
-            }
-            a[m + 1] = v;
+    if (n <= 1) return;
+    if (n <= 64) {
+        const int32_t v = (lane < n) ? a[lane] : INT32_MAX;
+        int rank = 0;
+        for (int k = 0; k < (int)n; k++) {
+            const int32_t o = __shfl(v, k, 64);
+            rank += (o < v) ? 1 : 0;
         }
-    } else {
+        if (lane < n) a[rank] = v;
+    } else if (lane == 0) {
         for (int64_t r = n / 2 - 1; r >= 0; r--) sift_down(a, r, n);
         for (int64_t e = n - 1; e > 0; e--) {
             const int32_t t = a[0];
@@ -255,170 +293,260 @@ __global__ void k_bin_sort(int64_t ncell, const int64_t* begin, int32_t* order)
     }
 }
 
-__global__ void k_bin_gather(int64_t total, const int32_t* order, const double* x, const double* y, double* ix,
-                             double* iy)
+__global__ void k_bin_gather(int64_t total, const int32_t* order, const double* x, const double* y, double2* ixy)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     const int32_t i = order[k];
-    ix[k] = x[i];
-    iy[k] = y[i];
+    ixy[k] = make_double2(x[i], y[i]);
 }
 
 // ----------------------------------------------------------------- positions
 
-__global__ void k_positions(const double* ix, const double* iy, const int32_t* cell_end, double start, double size,
-                            double radius, int cpr, int ppr, double dist, double offx, double offy, uint8_t* flags,
-                            int32_t* err)
+__global__ __launch_bounds__(256) void k_positions(const double2* ixy, const int32_t* cell_end, double start,
+                                                   double size, double radius, int cpr, int ppr, double dist,
+                                                   double offx, double offy, uint8_t* flags, int32_t* bcount,
+                                                   int32_t* err)
 {
-    const int64_t pidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pidx >= (int64_t)ppr * ppr) return;
-    const int i = (int)(pidx % ppr), j = (int)(pidx / ppr);
-    const double px = (double)i * dist + offx;  // SimulationCPU.cpp:114
-    const double py = (double)j * dist + offy;
-    const int cx = (int)((px - start) / size * (double)(cpr));  // cell_grid.h:52-58
-    const int cy = (int)((py - start) / size * (double)(cpr));
-    if (!(cx >= 0 && cx < cpr && cy >= 0 && cy < cpr)) {
-        flags[pidx] = 2;
-        atomicOr(err, 1);
-        return;
-    }
-    const int ci = cy * cpr + cx;
-    const int s = (ci == 0) ? 0 : cell_end[ci - 1];
-    const int e = cell_end[ci];
-    const double r2 = radius * radius;
-    uint8_t inside = 0;
-    for (int k = s; k < e; k++) {  // InsideImpurity, details.h:208-213
-        const double dx = px - ix[k], dy = py - iy[k];
-        if (r2 > (dx * dx + dy * dy)) {
-            inside = 1;
-            break;
+    __shared__ int sh[4];
+    const int64_t pidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n = (int64_t)ppr * ppr;
+    uint8_t f = 1;
+    if (pidx < n) {
+        const int i = (int)(pidx % ppr), j = (int)(pidx / ppr);
+        const double px = (double)i * dist + offx;  // SimulationCPU.cpp:114
+        const double py = (double)j * dist + offy;
+        const int cx = (int)((px - start) / size * (double)(cpr));  // cell_grid.h:52-58
+        const int cy = (int)((py - start) / size * (double)(cpr));
+        if (!(cx >= 0 && cx < cpr && cy >= 0 && cy < cpr)) {
+            f = 2;
+            atomicOr(err, 1);
+        } else {
+            const int ci = cy * cpr + cx;
+            const int s = (ci == 0) ? 0 : cell_end[ci - 1];
+            const int e = cell_end[ci];
+            const double r2 = radius * radius;
+            f = 0;
+            for (int k = s; k < e; k++) {  // InsideImpurity, details.h:208-213
+                const double2 q = ixy[k];
+                const double dx = px - q.x, dy = py - q.y;
+                if (r2 > (dx * dx + dy * dy)) {
+                    f = 1;
+                    break;
+                }
+            }
         }
+        flags[pidx] = f;
     }
-    flags[pidx] = inside;
+    const unsigned long long m = __ballot(pidx < n && f == 0);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ __launch_bounds__(1024) void k_compact(const uint8_t* flags, int64_t n, int32_t* free_pos, int32_t* nfree)
+__global__ __launch_bounds__(256) void k_compact(const uint8_t* flags, int64_t n, const int32_t* bcount, int nblocks,
+                                                 int32_t* free_pos, int32_t* nfree)
 {
-    __shared__ int64_t part[1024];
-    const int tid = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b0 = tid * chunk;
-    const int64_t b1 = (b0 + chunk < n) ? b0 + chunk : n;
-    int64_t local = 0;
-    for (int64_t k = b0; k < b1; k++) local += (flags[k] == 0);
-    part[tid] = local;
+    __shared__ int64_t sh[256];
+    __shared__ int wsum[4];
+    int64_t acc = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) acc += bcount[k];
+    sh[threadIdx.x] = acc;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int64_t v = (tid >= off) ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    int64_t w = part[tid] - local;
-    for (int64_t k = b0; k < b1; k++)
-        if (flags[k] == 0) free_pos[w++] = (int32_t)k;
-    if (tid == 1023) *nfree = (int32_t)part[1023];
+    const int64_t base = sh[0];
+    const int64_t pidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool is_free = pidx < n && flags[pidx] == 0;
+    const unsigned long long m = __ballot(is_free);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wave; w++) woff += wsum[w];
+    const unsigned long long lower = (lane == 0) ? 0ull : (m & (~0ull >> (64 - lane)));
+    if (is_free) free_pos[base + woff + __popcll(lower)] = (int32_t)pidx;
+    if (blockIdx.x == nblocks - 1 && threadIdx.x == 0)
+        *nfree = (int32_t)(base + wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
 // --------------------------------------------------------------------- trace
 
-template <bool CW, int NT>
+template <bool CW>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
 {
-    constexpr int NQ = 2 * NT + 1;
-    const int b = blockIdx.y;
     const int ppq = a.ppq;
     const int ppp = 4 * ppq;
     const int64_t nitems = (int64_t)a.nfree * ppp;
+    const int64_t total = nitems * a.nB;
     const int64_t P = (int64_t)a.ppr * a.ppr * ppp;
-    const double w = a.w[b];
-    const double ws = CW ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
 
     DevGrid g;
-    g.ix = a.ix;
-    g.iy = a.iy;
+    g.ixy = a.ixy;
     g.cell_end = a.cell_end;
     g.start = a.g_start;
     g.size = a.g_size;
     g.L = a.g_L;
     g.radius = a.g_radius;
     g.cpr = a.g_cpr;
+    const int cap = 4 * g.cpr * g.cpr + 16;
 
-    // Orbit constants shared by every particle of this field (lifetime.h:48-49).
-    const double R = a.speed / w;
-    const double R2 = R * R;
-    const double radd = R + g.radius, rmin = R - g.radius;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    int64_t q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
+    bool exhausted = false;
+    bool active = false;
+    int64_t item = 0, out_idx = 0;
+    Orb o;
+    double phi0 = 0.0, bound_time = 0.0;
+    TraceState st;
+
+    while (true) {
+        // Give every idle lane the next item of the wave's chunk.
+        unsigned long long need = __ballot(!active);
+        while (need != 0ull && !exhausted) {
+            if (q_next >= q_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = __shfl(base, 0, 64);
+                if ((int64_t)base >= total) {
+                    exhausted = true;
+                    break;
+                }
+                q_next = (int64_t)base;
+                q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
+            }
+            const int64_t left = q_end - q_next;
+            const int n = __popcll(need);
+            const int take = (int)((int64_t)n < left ? (int64_t)n : left);
+            if (!active) {
+                const int rank = __popcll(need & below);
+                if (rank < take) {
+                    item = q_next + rank;
+                    const int b = (int)(item / nitems);
+                    const int64_t k = item - (int64_t)b * nitems;
+                    const int64_t slot = k / ppp;
+                    const int v = (int)(k - slot * ppp);
+                    const int q = v / ppq, p = v - (v / ppq) * ppq;
+                    const int pidx = a.free_pos[slot];
+                    const int j = pidx / a.ppr, i = pidx - (pidx / a.ppr) * a.ppr;
+                    out_idx = (int64_t)b * P + (int64_t)pidx * ppp + v;
+                    const double px = (double)i * a.dist + a.offx;  // SimulationCPU.cpp:114
+                    const double py = (double)j * a.dist + a.offy;
+                    const double w = a.w[b];
+                    // CreateParticle (lifetime.h:36-58)
+                    const double phi = a.phi_start + q * ES_HALF_PI + p * a.phi_step;
+                    double sphi, cphi;
+                    esm_sincos(phi, &sphi, &cphi);
+                    const double vx = cphi * a.speed, vy = sphi * a.speed;
+                    const double R = a.speed / w;
+                    const double sx = vy * R / a.speed;
+                    const double sy = -vx * R / a.speed;
+                    const double radd = R + g.radius, rmin = R - g.radius;
+                    o.cx = CW ? (px - sx) : (px + sx);
+                    o.cy = CW ? (py - sy) : (py + sy);
+                    o.R = R;
+                    o.R2 = R * R;
+                    o.eps_r = 1e-10 * R;
+                    o.radd2 = radd * radd;
+                    o.rmin2 = rmin * rmin;
+                    o.r2 = g.radius * g.radius;
+                    o.pa = phi;
+                    o.w = w;
+                    phi0 = phi;
+                    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
+                    if (!a.coherent) {
+                        const double remaining = d_smod(phi + a.alpha, ES_HALF_PI);
+                        const double dphi = CW ? remaining : (2.0 * a.alpha - remaining);
+                        bound_time = dphi / w;
+                    }
+                    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
+                    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
+                    d_trace_begin(st, px, py, phi, c0x, c0y);
+                    active = true;
+                }
+            }
+            q_next += take;
+            need = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;  // queue drained and every lane done
+
+        if (active) {
+            int flags = 0;
+            if (d_step<CW>(o, phi0, g, cap, st, flags)) {
+                double lt = st.lifetime;
+                if (lt > bound_time) {  // lifetime.h:142-146
+                    lt = bound_time;
+                    flags |= ES_F_AT_BOUND;
+                }
+                a.lt_item[item] = lt;
+                a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
+                a.f_item[item] = (uint8_t)flags;
+                if (a.lifetimes) a.lifetimes[out_idx] = lt;
+                active = false;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------- sigma
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
+{
+    constexpr int NQ = 2 * NT + 1;
+    const int b = blockIdx.y;
+    const int ppq = a.ppq;
+    const int ppp = 4 * ppq;
+    const int64_t nitems = (int64_t)a.nfree * ppp;
+    const double w = a.w[b];
+    const double ws = a.clockwise ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
 
     DD acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q].s = acc[q].c = 0.0;
-    int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0;
-    int m_escaped = 0, m_at_bound = 0, m_cap = 0, m_traced = 0;
-    int64_t m_edge = 0, m_cross = 0, m_sigma = 0;
+    int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0, m_edge = 0, m_cross = 0, m_sigma = 0;
+    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0;
 
     const int64_t stride = (int64_t)a.nblk * kBlock;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nitems; k += stride) {
+        const int64_t gi = (int64_t)b * nitems + k;
+        const double lt_raw = a.lt_item[gi];
         const int64_t slot = k / ppp;
         const int v = (int)(k - slot * ppp);
         const int q = v / ppq, p = v - (v / ppq) * ppq;
         const int pidx = a.free_pos[slot];
         const int j = pidx / a.ppr, i = pidx - (pidx / a.ppr) * a.ppr;
-        const double px = (double)i * a.dist + a.offx;
-        const double py = (double)j * a.dist + a.offy;
+        if (a.with_metrics) {
+            const int4 m = a.m_item[gi];
+            const int f = a.f_item[gi];
+            m_cells += m.x;
+            m_tested += m.y;
+            m_edge += m.z;
+            m_cross += m.w;
+            m_max_cells = m.x > m_max_cells ? m.x : m_max_cells;
+            m_max_tested = m.y > m_max_tested ? m.y : m_max_tested;
+            m_escaped += (f & ES_F_ESCAPED) ? 1 : 0;
+            m_at_bound += (f & ES_F_AT_BOUND) ? 1 : 0;
+            m_cap += (f & ES_F_CAP) ? 1 : 0;
+            m_items += 1;
+        }
+        dd_add_d(acc[2 * NT], lt_raw);
 
-        // CreateParticle (lifetime.h:36-58)
+        // ApplySigma + IntegrateResult terms (SimulationCPU.cpp:178-222): a zero
+        // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
         const double phi = a.phi_start + q * ES_HALF_PI + p * a.phi_step;
         double sphi, cphi;
         esm_sincos(phi, &sphi, &cphi);
-        const double vx = cphi * a.speed, vy = sphi * a.speed;
-        const double sx = vy * R / a.speed;
-        const double sy = -vx * R / a.speed;
-        Orb o;
-        o.cx = CW ? (px - sx) : (px + sx);
-        o.cy = CW ? (py - sy) : (py + sy);
-        o.R = R;
-        o.R2 = R2;
-        o.eps_r = 1e-10 * R;
-        o.radd2 = radd * radd;
-        o.rmin2 = rmin * rmin;
-        o.r2 = g.radius * g.radius;
-        o.pa = phi;
-        o.w = w;
-        double bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
-        if (!a.coherent) {
-            const double remaining = d_smod(phi + a.alpha, ES_HALF_PI);
-            const double dphi = CW ? remaining : (2.0 * a.alpha - remaining);
-            bound_time = dphi / w;
-        }
-        const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
-        const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
-
-        TraceCounters tc;
-        tc.escaped = tc.at_bound = tc.cap_hit = tc.edge_hits = tc.cross_calls = 0;
-        const double lt_raw = d_trace<CW>(o, px, py, phi, c0x, c0y, bound_time, g, tc);
-
-        m_cells += tc.cells;
-        m_tested += tc.tested;
-        m_max_cells = tc.cells > m_max_cells ? tc.cells : m_max_cells;
-        m_max_tested = tc.tested > m_max_tested ? tc.tested : m_max_tested;
-        m_escaped += tc.escaped;
-        m_at_bound += tc.at_bound;
-        m_cap += tc.cap_hit;
-        m_traced += 1;
-        m_edge += tc.edge_hits;
-        m_cross += tc.cross_calls;
-        if (a.lifetimes) a.lifetimes[(int64_t)b * P + (int64_t)pidx * ppp + v] = lt_raw;
-        dd_add_d(acc[2 * NT], lt_raw);
-
-        // Truncation, GetSigma, Simpson weight per temperature.  A zero
-        // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
         const double wgt = simpson_weight(j, a.ppr) * simpson_weight(i, a.ppr) * simpson_weight(p, ppq);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             if (t < a.nT) {
                 const double ml = a.max_lt[t];
-                const double lt = (lt_raw < ml) ? lt_raw : ml;
+                const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
                 if (lt != 0.0) {
                     const double tau = a.tau[t];
                     const double z = esm_exp(-lt / tau);
@@ -448,8 +576,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
             sh_c[q][wave] = r.c;
         }
     }
-    int64_t mv[kMetricSlots] = {m_cells, m_tested, m_escaped, m_at_bound, m_cap,  m_max_tested,
-                                m_max_cells, m_traced, m_edge, m_cross, m_sigma};
+    int64_t mv[kMetricSlots] = {m_cells, m_tested, m_escaped, m_at_bound, m_cap, m_max_tested,
+                                m_max_cells, m_items, m_edge, m_cross, m_sigma};
 #pragma unroll
     for (int s = 0; s < kMetricSlots; s++) {
         const int64_t r = (s == 5 || s == 6) ? i64_wave_max(mv[s]) : i64_wave_sum(mv[s]);
@@ -468,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
             o.c = sh_c[q][wv];
             r = dd_add(r, o);
         }
-        // Partials are stored with the launch's NQ stride = 2 * nT + 1.
+        // Partials are stored with the launch's stride 2 * nT + 1.
         const int nq = 2 * a.nT + 1;
         int dst = -1;
         if (q < 2 * a.nT) dst = q;
@@ -488,11 +616,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
     }
 }
 
+__global__ void k_reset(unsigned long long* q) { *q = 0ull; }
+
 // ------------------------------------------------------------------ finalize
 
 __global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int64_t* mpart, int nblk, int nT,
-                                                  const double* factor, double* out_sig, double* out_lt_sum,
-                                                  int64_t* out_m)
+                                                  int with_metrics, const double* factor, double* out_sig,
+                                                  double* out_lt_sum, int64_t* out_m)
 {
     const int b = blockIdx.x;
     const int nq = 2 * nT + 1;
@@ -534,6 +664,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int
         }
         __syncthreads();
     }
+    if (!with_metrics) return;
     for (int s = 0; s < kMetricSlots; s++) {
         const bool is_max = (s == 5 || s == 6);
         int64_t acc = 0;
@@ -580,7 +711,11 @@ hipError_t launch_bin_count(const double* x, const double* y, int64_t n, const G
 hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st)
 {
     const int64_t ncell = (int64_t)g.cpr * g.cpr;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.counts, ncell, s.begin, g.cell_end);
+    const unsigned int nb = blocks_for(ncell, 256);
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum, g.cell_end);
     return hipGetLastError();
 }
 
@@ -594,56 +729,71 @@ hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev
                            s.fill, s.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(ncell, 64)), dim3(64), 0, st, ncell, s.begin, s.order);
+    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(ncell, 4)), dim3(256), 0, st, ncell, s.begin, s.order);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (g.total > 0) {
         hipLaunchKernelGGL(k_bin_gather, dim3(blocks_for(g.total, 256)), dim3(256), 0, st, g.total, s.order, x, y,
-                           g.ix, g.iy);
+                           g.ixy);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
-                            int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st)
+                            int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st)
 {
     const int64_t n = (int64_t)ppr * ppr;
+    const unsigned int nb = blocks_for(n, 256);
     hipError_t e = hipMemsetAsync(err, 0, sizeof(int32_t), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_positions, dim3(blocks_for(n, 256)), dim3(256), 0, st, g.ix, g.iy, g.cell_end,
-                       g.lo, g.size, g.radius, g.cpr, ppr, dist, offx, offy, flags, err);
+    hipLaunchKernelGGL(k_positions, dim3(nb), dim3(256), 0, st, g.ixy, g.cell_end, g.lo, g.size, g.radius, g.cpr, ppr,
+                       dist, offx, offy, flags, bcount, err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, flags, n, free_pos, nfree);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, st, flags, n, bcount, (int)nb, free_pos, nfree);
     return hipGetLastError();
 }
 
-template <bool CW, int NT>
-static hipError_t launch_trace_t(const TraceArgs& a, hipStream_t st)
+hipError_t launch_trace(const TraceArgs& a, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_trace<CW, NT>), dim3(a.nblk, a.nB), dim3(kBlock), 0, st, a);
+    if (a.nB <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.clockwise)
+        hipLaunchKernelGGL((k_trace<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_trace<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
-template <bool CW>
-static hipError_t launch_trace_cw(const TraceArgs& a, hipStream_t st)
+hipError_t trace_blocks_per_cu(int clockwise, int* blocks)
 {
-    if (a.nT <= 1) return launch_trace_t<CW, 1>(a, st);
-    if (a.nT <= 2) return launch_trace_t<CW, 2>(a, st);
-    if (a.nT <= 4) return launch_trace_t<CW, 4>(a, st);
-    return launch_trace_t<CW, 8>(a, st);
+    if (clockwise)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace<true>, kBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace<false>, kBlock, 0);
 }
 
-hipError_t launch_trace(const TraceArgs& a, hipStream_t st)
+template <int NT>
+static hipError_t launch_sigma_t(const SigmaArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_sigma<NT>), dim3(a.nblk, a.nB), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma(const SigmaArgs& a, hipStream_t st)
 {
     if (a.nT < 0 || a.nT > kMaxT || a.nB <= 0 || a.nblk <= 0) return hipErrorInvalidValue;
-    return a.clockwise ? launch_trace_cw<true>(a, st) : launch_trace_cw<false>(a, st);
+    if (a.nT <= 1) return launch_sigma_t<1>(a, st);
+    if (a.nT <= 2) return launch_sigma_t<2>(a, st);
+    if (a.nT <= 4) return launch_sigma_t<4>(a, st);
+    return launch_sigma_t<8>(a, st);
 }
 
-hipError_t launch_finalize(const double* dpart, const int64_t* mpart, int nB, int nblk, int nT, const double* factor,
-                           double* out_sig, double* out_lt_sum, int64_t* out_m, hipStream_t st)
+hipError_t launch_finalize(const double* dpart, const int64_t* mpart, int nB, int nblk, int nT, int with_metrics,
+                           const double* factor, double* out_sig, double* out_lt_sum, int64_t* out_m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(nB), dim3(256), 0, st, dpart, mpart, nblk, nT, factor, out_sig, out_lt_sum,
-                       out_m);
+    hipLaunchKernelGGL(k_finalize, dim3(nB), dim3(256), 0, st, dpart, mpart, nblk, nT, with_metrics, factor, out_sig,
+                       out_lt_sum, out_m);
     return hipGetLastError();
 }
 

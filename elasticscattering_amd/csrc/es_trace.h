@@ -2,10 +2,17 @@
 //
 // The floating-point operations and their order are those of the reference
 // kernel library (src/sim/es/details.h, cell_grid.h, lifetime.h); the code is
-// organised for the GPU: per-orbit constants are computed once per particle,
-// the rotation sense is a template parameter, impurities are read from SoA
-// arrays, and the start-cell "inside an impurity" test is hoisted out (it does
-// not depend on phi or B and runs once per position in es_kernels.hip).
+// organised for the GPU:
+//   * per-orbit constants are computed once per particle;
+//   * the rotation sense is a template parameter;
+//   * the four cell edges and the two crossing points are loops (one copy of
+//     the asin-bearing code, which keeps the register count down);
+//   * a trace advances one cell per call of d_step(), so a wave can give a
+//     lane a new particle as soon as its trace ends (es_kernels.hip);
+//   * impurities are read as 16-byte (x, y) pairs, four cells' worth of
+//     loads in flight;
+//   * the start-cell "inside an impurity" test is hoisted out: it depends on
+//     neither phi nor B and runs once per position (k_positions).
 // Compiled with -ffp-contract=off so no multiply-add is fused.
 #pragma once
 
@@ -21,14 +28,13 @@
 
 // Device view of the binned impurity field (Grid::GetSettings, Grid.h:82-91).
 struct DevGrid {
-    const double* ix;       // cell-ordered x of indexed impurities
-    const double* iy;       // cell-ordered y
+    const double2* ixy;       // cell-ordered (x, y) of indexed impurities
     const int32_t* cell_end;  // inclusive end offset per cell (Grid::imp_index)
-    double start;           // spawn_region_start
-    double size;            // spawn_region_size
-    double L;               // cell_size
-    double radius;          // impurity_radius
-    int cpr;                // cells_per_row
+    double start;             // spawn_region_start
+    double size;              // spawn_region_size
+    double L;                 // cell_size
+    double radius;            // impurity_radius
+    int cpr;                  // cells_per_row
 };
 
 // One orbit (details.h:31-65) with the constants its tests reuse.
@@ -99,67 +105,23 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
     const double ys = sqrt(o.R2 - xs * xs);
     const double ux = (px - o.cx) / dist, uy = (py - o.cy) / dist;
     const double bx = o.cx + ux * xs, by = o.cy + uy * xs;
-    const double p1x = bx + uy * ys, p1y = by + (-ux) * ys;
-    const double p2x = bx + (-uy) * ys, p2y = by + ux * ys;
-    const double phi1 = d_angle<CW>(p1x, p1y, o);
-    const double phi2 = d_angle<CW>(p2x, p2y, o);
     double tt = ES_INF;
-    if (d_in_range<CW>(phi1, rx, ry)) {
-        const double t = d_cross_angle<CW>(o.pa, phi1) / o.w;
-        tt = (tt < t) ? tt : t;
-    }
-    if (d_in_range<CW>(phi2, rx, ry)) {
-        const double t = d_cross_angle<CW>(o.pa, phi2) / o.w;
-        tt = (tt < t) ? tt : t;
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+        // point 1: (bx + uy*ys, by + (-ux)*ys); point 2: (bx + (-uy)*ys, by + ux*ys)
+        const double ex = k ? -uy : uy;
+        const double ey = k ? ux : -ux;
+        const double phi_k = d_angle<CW>(bx + ex * ys, by + ey * ys, o);
+        if (d_in_range<CW>(phi_k, rx, ry)) {
+            const double t = d_cross_angle<CW>(o.pa, phi_k) / o.w;
+            tt = (tt < t) ? tt : t;
+        }
     }
     return tt;
 }
 
-// Update_Best_Intersect (cell_grid.h:80-89).
-template <bool CW>
-__device__ __forceinline__ void d_update_best(const Orb& o, double x, double y, int ncx, int ncy, double dp_eps,
-                                              const Isect& last, Isect& best, double& best_dphi)
-{
-    const double ia = d_angle<CW>(x, y, o);
-    const double dphi = d_cross_angle<CW>(last.ang, ia);
-    if (dphi < best_dphi && (fabs(x - last.x) > dp_eps || fabs(y - last.y) > dp_eps)) {
-        best.x = x;
-        best.y = y;
-        best.cx = ncx;
-        best.cy = ncy;
-        best.ang = ia;
-        best_dphi = dphi;
-    }
-}
-
-// UpdateFirstBoundaryIntersect (cell_grid.h:91-129) for the edge p1 -> p2.
-template <bool CW>
-__device__ __forceinline__ void d_edge(const Orb& o, int offx, int offy, double p1x, double p1y, double p2x,
-                                       double p2y, double L, double dp_eps, const Isect& last, Isect& best,
-                                       double& best_dphi, int& edge_hits)
-{
-    const double ux = (p2x - p1x) / L, uy = (p2y - p1y) / L;
-    const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
-    const double projx = p1x + ux * pd, projy = p1y + uy * pd;
-    const double dx = projx - o.cx, dy = projy - o.cy;
-    const double pcd = dx * dx + dy * dy;
-    if (pcd >= o.R2) return;
-    const double s = sqrt(o.R2 - pcd);
-    const double tx = ux * s, ty = uy * s;
-    const double i1x = projx + tx, i1y = projy + ty;
-    const double i2x = projx - tx, i2y = projy - ty;
-    const bool horizontal = fabs(ux) > fabs(uy);
-    const double a1 = horizontal ? i1x : i1y, a2 = horizontal ? i2x : i2y;
-    const double l0 = horizontal ? p1x : p1y, l1 = horizontal ? p2x : p2y;
-    const bool v1 = (a1 > l0 && a1 < l1) || (a1 < l0 && a1 > l1);
-    const bool v2 = (a2 > l0 && a2 < l1) || (a2 < l0 && a2 > l1);
-    const int ncx = last.cx + offx, ncy = last.cy + offy;
-    if (v1) d_update_best<CW>(o, i1x, i1y, ncx, ncy, dp_eps, last, best, best_dphi);
-    if (v2) d_update_best<CW>(o, i2x, i2y, ncx, ncy, dp_eps, last, best, best_dphi);
-    edge_hits += (int)v1 + (int)v2;
-}
-
-// GetNextCell (cell_grid.h:132-181).  `phi0` is the particle's start angle.
+// GetNextCell (cell_grid.h:132-181) with UpdateFirstBoundaryIntersect
+// (:91-129) and Update_Best_Intersect (:80-89).  `phi0`: the start angle.
 template <bool CW>
 __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const DevGrid& g, const Isect& last, Isect& next,
                                             int& edge_hits)
@@ -174,10 +136,46 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     const double trx = llx + L, try_ = lly + L;
     const double tlx = llx + 0.0, tly = lly + L;
 
-    d_edge<CW>(o, 0, 1, tlx, tly, trx, try_, L, dp_eps, last, next, best_dphi, edge_hits);
-    d_edge<CW>(o, 0, -1, llx, lly, lrx, lry, L, dp_eps, last, next, best_dphi, edge_hits);
-    d_edge<CW>(o, 1, 0, lrx, lry, trx, try_, L, dp_eps, last, next, best_dphi, edge_hits);
-    d_edge<CW>(o, -1, 0, llx, lly, tlx, tly, L, dp_eps, last, next, best_dphi, edge_hits);
+    // Edges in the reference order: top (0,+1), bottom (0,-1), right (+1,0), left (-1,0).
+#pragma unroll 1
+    for (int e = 0; e < 4; e++) {
+        const double p1x = (e == 0) ? tlx : (e == 2) ? lrx : llx;
+        const double p1y = (e == 0) ? tly : (e == 2) ? lry : lly;
+        const double p2x = (e == 0 || e == 2) ? trx : (e == 1) ? lrx : tlx;
+        const double p2y = (e == 0 || e == 2) ? try_ : (e == 1) ? lry : tly;
+        const int ncx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
+        const int ncy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
+
+        const double ux = (p2x - p1x) / L, uy = (p2y - p1y) / L;
+        const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
+        const double projx = p1x + ux * pd, projy = p1y + uy * pd;
+        const double dx = projx - o.cx, dy = projy - o.cy;
+        const double pcd = dx * dx + dy * dy;
+        if (pcd >= o.R2) continue;
+        const double s = sqrt(o.R2 - pcd);
+        const double tx = ux * s, ty = uy * s;
+        const bool horizontal = fabs(ux) > fabs(uy);
+        const double l0 = horizontal ? p1x : p1y, l1 = horizontal ? p2x : p2y;
+#pragma unroll 1
+        for (int k = 0; k < 2; k++) {
+            const double ix = k ? (projx - tx) : (projx + tx);
+            const double iy = k ? (projy - ty) : (projy + ty);
+            const double a = horizontal ? ix : iy;
+            if ((a > l0 && a < l1) || (a < l0 && a > l1)) {
+                edge_hits += 1;
+                const double ia = d_angle<CW>(ix, iy, o);
+                const double dphi = d_cross_angle<CW>(last.ang, ia);
+                if (dphi < best_dphi && (fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps)) {
+                    next.x = ix;
+                    next.y = iy;
+                    next.cx = ncx;
+                    next.cy = ncy;
+                    next.ang = ia;
+                    best_dphi = dphi;
+                }
+            }
+        }
+    }
 
     const double delta = 1e-7 * L;
     const bool is_xlow = (next.x - llx) < delta;
@@ -197,67 +195,87 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     return !same_cell && inside && !d_in_range<CW>(phi0, last.ang, next.ang);
 }
 
-struct TraceCounters {
-    int64_t cells;
-    int64_t tested;
-    int escaped;
-    int at_bound;
-    int cap_hit;
-    int edge_hits;     // accepted cell-edge intersections (Update_Best_Intersect calls)
-    int cross_calls;   // GetFirstCrossTime calls (CirclesCross true)
+// Per-particle trace state and work counters.
+struct TraceState {
+    Isect nx;            // entry point of the next cell to visit
+    double lifetime;     // running minimum (INF until a hit)
+    int cells;           // cells visited
+    int tested;          // impurities tested
+    int edge_hits;       // accepted cell-edge intersections (Update_Best_Intersect calls)
+    int cross_calls;     // GetFirstCrossTime calls (CirclesCross true)
 };
 
-// TraceOrbit (lifetime.h:78-148) after the start-cell test, from cell (c0x, c0y).
-template <bool CW>
-__device__ double d_trace(const Orb& o, double px, double py, double phi0, int c0x, int c0y, double bound_time,
-                          const DevGrid& g, TraceCounters& tc)
+// Trace outcome flags.
+#define ES_F_ESCAPED 1
+#define ES_F_AT_BOUND 2
+#define ES_F_CAP 4
+
+__device__ __forceinline__ void d_trace_begin(TraceState& st, double px, double py, double phi0, int c0x, int c0y)
 {
-    double lifetime = ES_INF;
-    Isect nx;
-    nx.x = px;
-    nx.y = py;
-    nx.ang = phi0;
-    nx.cx = c0x;
-    nx.cy = c0y;
-    const int64_t cap = 4LL * g.cpr * g.cpr + 16;
-    int64_t cells = 0, tested = 0;
-    while (true) {
-        const Isect entry = nx;
-        const bool avail = d_next_cell<CW>(o, phi0, g, entry, nx, tc.edge_hits);
-        const double rx = entry.ang;
-        const double ry = avail ? nx.ang : phi0;
-        const int ci = entry.cy * g.cpr + entry.cx;
-        const int s = (ci > 0) ? g.cell_end[ci - 1] : 0;
-        const int e = g.cell_end[ci];
-        cells += 1;
-        tested += (e - s);
-        for (int i = s; i < e; i++) {
-            const double ipx = g.ix[i], ipy = g.iy[i];
-            const double qx = o.cx - ipx, qy = o.cy - ipy;
+    st.nx.x = px;
+    st.nx.y = py;
+    st.nx.ang = phi0;
+    st.nx.cx = c0x;
+    st.nx.cy = c0y;
+    st.lifetime = ES_INF;
+    st.cells = st.tested = st.edge_hits = st.cross_calls = 0;
+}
+
+// One iteration of the TraceOrbit cell loop (lifetime.h:101-140).  Returns
+// true when the trace has ended; then *flags holds ES_F_ESCAPED / ES_F_CAP
+// and st.lifetime the raw lifetime before the bound (lifetime.h:142-147).
+template <bool CW>
+__device__ __forceinline__ bool d_step(const Orb& o, double phi0, const DevGrid& g, int cap, TraceState& st,
+                                       int& flags)
+{
+    const Isect entry = st.nx;
+    const bool avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
+    const double rx = entry.ang;
+    const double ry = avail ? st.nx.ang : phi0;
+    const int ci = entry.cy * g.cpr + entry.cx;
+    const int s = (ci > 0) ? g.cell_end[ci - 1] : 0;
+    const int e = g.cell_end[ci];
+    st.cells += 1;
+    st.tested += (e - s);
+    double lifetime = st.lifetime;
+    for (int i = s; i < e; i += 4) {
+        // Four impurities' loads in flight, CirclesCross on each (details.h:141-151),
+        // then GetFirstCrossTime for the crossing ones in impurity order.
+        double2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = (i + u < e) ? g.ixy[i + u] : make_double2(0.0, 0.0);
+        int hits = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
             const double d2 = qx * qx + qy * qy;
-            if ((d2 < o.radd2) && (d2 > o.rmin2)) {
-                const double t = d_first_cross_time<CW>(o, ipx, ipy, rx, ry);
-                lifetime = (t < lifetime) ? t : lifetime;
-                tc.cross_calls += 1;
-            }
+            hits |= ((i + u < e) && (d2 < o.radd2) && (d2 > o.rmin2)) ? (1 << u) : 0;
         }
-        if (lifetime < ES_INF) break;
-        if (!avail) {
-            tc.escaped += 1;
-            lifetime = 0.0;
-            break;
-        }
-        if (cells >= cap) {
-            tc.cap_hit += 1;
-            lifetime = 0.0;
-            break;
+#pragma unroll 1
+        while (hits) {
+            const int u = __builtin_ctz(hits);
+            hits &= hits - 1;
+            const double hx = (u == 0) ? p[0].x : (u == 1) ? p[1].x : (u == 2) ? p[2].x : p[3].x;
+            const double hy = (u == 0) ? p[0].y : (u == 1) ? p[1].y : (u == 2) ? p[2].y : p[3].y;
+            const double t = d_first_cross_time<CW>(o, hx, hy, rx, ry);
+            lifetime = (t < lifetime) ? t : lifetime;
+            st.cross_calls += 1;
         }
     }
-    tc.cells = cells;
-    tc.tested = tested;
-    if (lifetime > bound_time) {
-        tc.at_bound += 1;
-        return bound_time;
+    st.lifetime = lifetime;
+    if (lifetime < ES_INF) {
+        flags = 0;
+        return true;
     }
-    return lifetime;
+    if (!avail) {
+        flags = ES_F_ESCAPED;
+        st.lifetime = 0.0;
+        return true;
+    }
+    if (st.cells >= cap) {
+        flags = ES_F_CAP;
+        st.lifetime = 0.0;
+        return true;
+    }
+    return false;
 }
