@@ -18,7 +18,7 @@ import numpy as np
 
 __all__ = [
     "EsError", "GridInfo", "GridDesc", "Settings", "Sigma", "Metrics", "SolverDesc",
-    "Simulation", "load_library", "LIB_PATH", "EXPORTED_SYMBOLS",
+    "Simulation", "load_library", "parse_config", "run", "LIB_PATH", "EXPORTED_SYMBOLS",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -128,6 +128,9 @@ _SIGNATURES = {
     "es_solver_compute_sigmas_sweep": (C.c_int, [_P, _PD, C.c_int32, _PD, C.c_int32, C.POINTER(Sigma),
                                                  C.POINTER(Metrics)]),
     "es_version": (C.c_char_p, []),
+    # include/es_runner.h
+    "es_config_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
+    "es_run": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 
@@ -150,6 +153,29 @@ def load_library(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+def parse_config(path):
+    """The .config file as the reference parses it (SimulationConfiguration.h:77-229), as a dict."""
+    import json
+
+    lib = load_library()
+    buf = C.create_string_buffer(1 << 16)
+    rc = lib.es_config_parse(str(path).encode(), buf, len(buf))
+    d = json.loads(buf.value.decode())
+    if rc != ES_OK:
+        raise EsError(rc, d.get("error", "parse failed"))
+    return d
+
+
+def run(config_path, verbose=False):
+    """SimulationRunner::Run (SimulationRunner.cpp:32-52) on GPU 0; returns the output directory."""
+    lib = load_library()
+    buf = C.create_string_buffer(4096)
+    rc = lib.es_run(str(config_path).encode(), int(bool(verbose)), buf, len(buf))
+    if rc != ES_OK:
+        raise EsError(rc, "es_run failed (see stderr with verbose=True)")
+    return buf.value.decode()
 
 
 def _dptr(a):

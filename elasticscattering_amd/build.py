@@ -22,7 +22,8 @@ LIB = os.path.join(PKG, "libes_hip.so")
 RUNNER = os.path.join(PKG, "es_run")
 
 HIP_SOURCES = ["es_kernels.hip", "es_abi.hip"]
-HOST_SOURCES = []
+HOST_SOURCES = ["host/es_config.cpp", "host/es_runner.cpp"]
+RUNNER_MAIN = "host/es_run_main.cpp"
 
 ARCH = os.environ.get("ES_OFFLOAD_ARCH", "gfx950")
 
@@ -58,6 +59,11 @@ def build_library(verbose=False):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmds.append([cc, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, os.path.basename(os.path.splitext(src)[0]) + ".o")
+        objs.append(obj)
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                     "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
@@ -67,6 +73,11 @@ def build_library(verbose=False):
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # es_run: the command-line runner, linked against the library next to it.
+    tmp = RUNNER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, RUNNER_MAIN), "-o", tmp,
+          "-L", PKG, "-les_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, RUNNER)
     return LIB
 
 
