@@ -32,7 +32,7 @@ TRACE_WORK = {
     "cells_passed": 95,       # GetNextCell: corners, 4 edge projections, snap, stop test
     "edge_hits": 52,          # per accepted edge intersection: sqrt, points, GetAngle (asin), GetCrossAngle
     "impurities_tested": 5,   # CirclesCross
-    "cross_calls": 121,       # GetCrossPoints + 2 GetAngle + 2 AngleInRange + times
+    "cross_exact": 121,       # GetCrossPoints + 2 GetAngle + 2 AngleInRange + times (every crossing disk)
 }
 SIGMA_WORK = {
     "traced_particles": 70,   # phi, sincos, Simpson weight, lifetime sum
@@ -136,11 +136,14 @@ def main():
                 stats["sigma_flops"].append(sum(algorithmic_flops(m, SIGMA_WORK) for m in ms))
                 stats["sigma_s"].append(sum(m["time_elapsed_temperatures"] for m in ms))
                 stats["traced"] += sum(m["traced_particles"] for m in ms)
+                stats["cross"] += sum(m["cross_calls"] for m in ms)
+                stats["exact"] += sum(m["cross_exact"] for m in ms)
                 stats["sigma"].append(sig)
 
     for s in range(args.warmup):
         step(s, None)
-    stats = {"launch_s": [], "flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0, "sigma": []}
+    stats = {"launch_s": [], "flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0, "sigma": [], "cross": 0,
+             "exact": 0}
     t0 = time.perf_counter()
     for s in range(args.steps):
         step(s, stats)
@@ -177,6 +180,7 @@ def main():
         },
         "sweep_wall_s": dt * len(w["seeds"]) / args.steps,
         "traced_lifetimes_per_s": stats["traced"] / dt,
+        "exact_cross_solves_fraction": stats["exact"] / max(stats["cross"], 1),
         "roofline": {
             "bound": "fp64-valu",
             "kernel": "k_trace (CreateParticle + TraceOrbit, one launch per sample and pass, all fields)",

@@ -96,6 +96,7 @@ class Metrics(C.Structure):
         ("edge_hits", C.c_int64),
         ("cross_calls", C.c_int64),
         ("sigma_evals", C.c_int64),
+        ("cross_exact", C.c_int64),
     ]
 
     def as_dict(self):
@@ -138,11 +139,11 @@ _lib = None
 
 
 def load_library(path=None):
-    """Loads libes_hip.so (raises if it has not been built)."""
+    """Loads libes_hip.so (raises if it has not been built); ES_LIBRARY overrides the path."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("ES_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise ImportError(f"{p} is missing: build it with `python -m elasticscattering_amd.build`")
     lib = C.CDLL(p)

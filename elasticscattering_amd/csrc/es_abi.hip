@@ -66,7 +66,7 @@ struct es_solver {
 
     // per call
     DevBuf w, factor, dpart, mpart, sig, ltsum, mout, lifetimes;
-    DevBuf lt_item, m_item, f_item, queue;
+    DevBuf lt_item, m_item, x_item, f_item, queue;
     int cus = 0;                      // compute units of the device
     int trace_bpc[2] = {0, 0};        // resident trace blocks per CU (counter-clockwise, clockwise)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -297,6 +297,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     if ((rc = ensure(s, s->lt_item, total_items * sizeof(double), "item lifetimes"))) return rc;
     if ((rc = ensure(s, s->m_item, total_items * sizeof(int4), "item counters"))) return rc;
     if ((rc = ensure(s, s->f_item, total_items, "item flags"))) return rc;
+    if ((rc = ensure(s, s->x_item, total_items * sizeof(int32_t), "item exact solves"))) return rc;
     if ((rc = ensure(s, s->queue, sizeof(unsigned long long), "work queue"))) return rc;
     if (lifetimes_host) {
         if ((rc = ensure(s, s->lifetimes, (size_t)nB * P_ * sizeof(double), "lifetimes"))) return rc;
@@ -339,6 +340,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     a.chunk = 128;
     a.lt_item = P<double>(s->lt_item);
     a.m_item = P<int4>(s->m_item);
+    a.x_item = P<int32_t>(s->x_item);
     a.f_item = P<uint8_t>(s->f_item);
     a.lifetimes = lifetimes_host ? P<double>(s->lifetimes) : nullptr;
     const int64_t want_blocks = (total_items + es::kBlock - 1) / es::kBlock;
@@ -352,6 +354,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
     es::SigmaArgs sa{};
     sa.lt_item = a.lt_item;
     sa.m_item = a.m_item;
+    sa.x_item = a.x_item;
     sa.f_item = a.f_item;
     sa.free_pos = a.free_pos;
     sa.nfree = s->nfree;
@@ -436,6 +439,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
             o.edge_hits = m[8];
             o.cross_calls = m[9];
             o.sigma_evals = m[10];
+            o.cross_exact = m[11];
         }
     }
     return ES_OK;
@@ -506,7 +510,7 @@ void es_solver_destroy(es_solver* s)
     DevBuf* bufs[] = {&s->in_x, &s->in_y, &s->ixy, &s->cell_end, &s->b_targets, &s->b_ntarg, &s->b_counts,
                       &s->b_fill, &s->b_begin, &s->b_bsum, &s->b_order, &s->flags, &s->bcount, &s->free_pos,
                       &s->small_i32, &s->w, &s->factor, &s->dpart, &s->mpart, &s->sig, &s->ltsum, &s->mout,
-                      &s->lifetimes, &s->lt_item, &s->m_item, &s->f_item, &s->queue};
+                      &s->lifetimes, &s->lt_item, &s->m_item, &s->x_item, &s->f_item, &s->queue};
     for (DevBuf* b : bufs) release(*b);
     for (hipEvent_t& ev : s->ev)
         if (ev) (void)hipEventDestroy(ev);

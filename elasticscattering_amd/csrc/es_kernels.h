@@ -10,8 +10,8 @@ constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kMaxT = 8;           // temperatures per sigma launch
 // int64 metric partials per block: 0 cells, 1 tested, 2 escaped, 3 at_bound,
 // 4 cap hits, 5 max tested (max), 6 max cells (max), 7 traced particles,
-// 8 edge hits, 9 cross-time calls, 10 sigma evaluations.
-constexpr int kMetricSlots = 11;
+// 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time solves.
+constexpr int kMetricSlots = 12;
 
 // Binned field on the device (cell-ordered (x, y) pairs; reference index layout).
 struct GridDev {
@@ -42,6 +42,7 @@ struct TraceArgs {
     // outputs per item g = b * nitems + k
     double* lt_item;               // raw lifetime
     int4* m_item;                  // cells, tested, edge hits, cross calls
+    int32_t* x_item;               // exact cross-time solves
     uint8_t* f_item;               // ES_F_* flags
     double* lifetimes;             // [nB][P] reference layout, or null
 };
@@ -50,6 +51,7 @@ struct TraceArgs {
 struct SigmaArgs {
     const double* lt_item;
     const int4* m_item;
+    const int32_t* x_item;
     const uint8_t* f_item;
     const int32_t* free_pos;
     int nfree;

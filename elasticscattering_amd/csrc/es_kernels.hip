@@ -374,8 +374,13 @@ __global__ __launch_bounds__(256) void k_compact(const uint8_t* flags, int64_t n
 
 // --------------------------------------------------------------------- trace
 
+#ifndef ES_TRACE_WAVES_PER_EU
+#define ES_TRACE_WAVES_PER_EU 3
+#endif
+
 template <bool CW>
-__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ES_TRACE_WAVES_PER_EU)))
+void k_trace(TraceArgs a)
 {
     const int ppq = a.ppq;
     const int ppp = 4 * ppq;
@@ -456,6 +461,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
                     o.r2 = g.radius * g.radius;
                     o.pa = phi;
                     o.w = w;
+                    o.u0x = px - o.cx;
+                    o.u0y = py - o.cy;
+                    o.r = (float)g.radius;
                     phi0 = phi;
                     bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
                     if (!a.coherent) {
@@ -484,6 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a)
                 }
                 a.lt_item[item] = lt;
                 a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
+                a.x_item[item] = st.cross_exact;
                 a.f_item[item] = (uint8_t)flags;
                 if (a.lifetimes) a.lifetimes[out_idx] = lt;
                 active = false;
@@ -509,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q].s = acc[q].c = 0.0;
     int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0, m_edge = 0, m_cross = 0, m_sigma = 0;
-    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0;
+    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0;
 
     const int64_t stride = (int64_t)a.nblk * kBlock;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nitems; k += stride) {
@@ -527,6 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
             m_tested += m.y;
             m_edge += m.z;
             m_cross += m.w;
+            m_exact += a.x_item[gi];
             m_max_cells = m.x > m_max_cells ? m.x : m_max_cells;
             m_max_tested = m.y > m_max_tested ? m.y : m_max_tested;
             m_escaped += (f & ES_F_ESCAPED) ? 1 : 0;
@@ -577,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
         }
     }
     int64_t mv[kMetricSlots] = {m_cells, m_tested, m_escaped, m_at_bound, m_cap, m_max_tested,
-                                m_max_cells, m_items, m_edge, m_cross, m_sigma};
+                                m_max_cells, m_items, m_edge, m_cross, m_sigma, m_exact};
 #pragma unroll
     for (int s = 0; s < kMetricSlots; s++) {
         const int64_t r = (s == 5 || s == 6) ? i64_wave_max(mv[s]) : i64_wave_sum(mv[s]);

@@ -21,6 +21,14 @@
 
 #include "es_math.h"
 
+// ES_TRACE_SCREEN=1 solves the crossing disks of a cell in order of a lower
+// bound on their travel angle and skips those that cannot lower the minimum
+// (results identical).  Per-lane skips only pay when a whole wave skips, so it
+// is off by default.
+#ifndef ES_TRACE_SCREEN
+#define ES_TRACE_SCREEN 0
+#endif
+
 #define ES_PI 3.141592653589793238463
 #define ES_PI2 6.283185307179586
 #define ES_HALF_PI 1.570796326794896619231
@@ -46,6 +54,8 @@ struct Orb {
     double r2;              // r * r (details.h:160)
     double pa;              // particle_angle = phi
     double w;               // angular speed (unsigned)
+    double u0x, u0y;        // start position minus centre (screening only)
+    float r;                // impurity radius (screening only)
 };
 
 // Cell-boundary crossing (cell_grid.h:31-46); dphi lives in the search only.
@@ -195,6 +205,38 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     return !same_cell && inside && !d_in_range<CW>(phi0, last.ang, next.ang);
 }
 
+// Screening bound (not a reference operation): a lower bound on the travel
+// angle, in the direction of motion, from the start point P0 to any point where
+// the orbit meets the disk of radius r around C = (px, py), or 0 when no bound
+// is available.
+//  * The crossing points lie within asin(r/d) <= 1.0473 r/d (r/d <= 1/2) of
+//    the direction of C seen from the orbit centre O (tangent lines from O).
+//  * If |P0 - C| >= r + 1e-6 R, P0 is outside the disk, every crossing point X
+//    has |P0 - X| >= |P0 - C| - r >= 1e-6 R, so its travel angle lies in
+//    [1e-6, 2pi - 1e-6]: the FP64 crossing angle (error ~1e-8 rad near
+//    asin(+-1)) cannot wrap through 0, and the bound a - beta needs no wrap test.
+//  * eps covers the FP32 error of a and beta and the FP64 error of the exact
+//    crossing angle.
+// A disk whose bound reaches the current minimum travel angle cannot lower the
+// minimum, so its exact GetFirstCrossTime can be skipped without changing any
+// result.
+template <bool CW>
+__device__ __forceinline__ float d_travel_lb(const Orb& o, double px, double py, double d2)
+{
+    const double vx = px - o.cx, vy = py - o.cy;
+    const double gx = o.u0x - vx, gy = o.u0y - vy;  // P0 - C
+    const double gap = (double)o.r + 1e-6 * o.R;
+    if (!(gx * gx + gy * gy >= gap * gap)) return 0.0f;
+    const float ratio = o.r * rsqrtf((float)d2);
+    if (!(ratio <= 0.5f)) return 0.0f;
+    const double crs = o.u0x * vy - o.u0y * vx;
+    const double dot = o.u0x * vx + o.u0y * vy;
+    float a = atan2f((float)(CW ? -crs : crs), (float)dot);  // (-pi, pi]
+    a = (a < 0.0f) ? a + 6.2831855f : a;
+    const float lo = a - 1.0473f * ratio - (2e-6f + 1e-5f * a);
+    return lo > 0.0f ? lo : 0.0f;
+}
+
 // Per-particle trace state and work counters.
 struct TraceState {
     Isect nx;            // entry point of the next cell to visit
@@ -202,7 +244,8 @@ struct TraceState {
     int cells;           // cells visited
     int tested;          // impurities tested
     int edge_hits;       // accepted cell-edge intersections (Update_Best_Intersect calls)
-    int cross_calls;     // GetFirstCrossTime calls (CirclesCross true)
+    int cross_calls;     // GetFirstCrossTime calls of the reference (CirclesCross true)
+    int cross_exact;     // of those, evaluated exactly here (the rest are screened out)
 };
 
 // Trace outcome flags.
@@ -218,7 +261,7 @@ __device__ __forceinline__ void d_trace_begin(TraceState& st, double px, double 
     st.nx.cx = c0x;
     st.nx.cy = c0y;
     st.lifetime = ES_INF;
-    st.cells = st.tested = st.edge_hits = st.cross_calls = 0;
+    st.cells = st.tested = st.edge_hits = st.cross_calls = st.cross_exact = 0;
 }
 
 // One iteration of the TraceOrbit cell loop (lifetime.h:101-140).  Returns
@@ -237,10 +280,85 @@ __device__ __forceinline__ bool d_step(const Orb& o, double phi0, const DevGrid&
     const int e = g.cell_end[ci];
     st.cells += 1;
     st.tested += (e - s);
+#if ES_TRACE_SCREEN
+    // The reference tests every impurity of the cell and keeps the smallest
+    // GetFirstCrossTime (lifetime.h:117-123); a minimum does not depend on the
+    // order, so: pass 1 runs CirclesCross (details.h:141-151) on all of them and
+    // finds the crossing disk with the smallest screening bound; that one is
+    // solved first, then pass 2 solves every other crossing disk whose bound
+    // does not already exceed the current minimum.
+    double lifetime = st.lifetime;
+    float best_lb = 3.0e38f;
+    int best_i = -1, ncross = 0;
+    for (int i = s; i < e; i += 4) {
+        double2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = (i + u < e) ? g.ixy[i + u] : make_double2(0.0, 0.0);
+        int hits = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
+            const double d2 = qx * qx + qy * qy;
+            hits |= ((i + u < e) && (d2 < o.radd2) && (d2 > o.rmin2)) ? (1 << u) : 0;
+        }
+#pragma unroll 1
+        while (hits) {
+            const int u = __builtin_ctz(hits);
+            hits &= hits - 1;
+            const double hx = (u == 0) ? p[0].x : (u == 1) ? p[1].x : (u == 2) ? p[2].x : p[3].x;
+            const double hy = (u == 0) ? p[0].y : (u == 1) ? p[1].y : (u == 2) ? p[2].y : p[3].y;
+            const double qx = o.cx - hx, qy = o.cy - hy;
+            const float lb = d_travel_lb<CW>(o, hx, hy, qx * qx + qy * qy);
+            ncross += 1;
+            if (lb < best_lb) {
+                best_lb = lb;
+                best_i = i + u;
+            }
+        }
+    }
+    st.cross_calls += ncross;
+    // Pass 2.  Its first round (i = s - 4) holds only the best-bound disk, which
+    // is always solved; later rounds skip it.  One call site of the solver.
+    const int last_i = (ncross > 1) ? e : s;
+    for (int i = s - 4; best_i >= 0 && i < last_i; i += 4) {
+        const bool first = (i < s);
+        double2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int k = first ? best_i : i + u;
+            p[u] = (first ? (u == 0) : (k < e)) ? g.ixy[k] : make_double2(0.0, 0.0);
+        }
+        int hits = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
+            const double d2 = qx * qx + qy * qy;
+            const bool cand = first ? (u == 0)
+                                    : ((i + u < e) && (i + u != best_i) && (d2 < o.radd2) && (d2 > o.rmin2));
+            hits |= cand ? (1 << u) : 0;
+        }
+#pragma unroll 1
+        while (hits) {
+            const int u = __builtin_ctz(hits);
+            hits &= hits - 1;
+            const double hx = (u == 0) ? p[0].x : (u == 1) ? p[1].x : (u == 2) ? p[2].x : p[3].x;
+            const double hy = (u == 0) ? p[0].y : (u == 1) ? p[1].y : (u == 2) ? p[2].y : p[3].y;
+            if (!first) {
+                const double qx = o.cx - hx, qy = o.cy - hy;
+                const float lb = d_travel_lb<CW>(o, hx, hy, qx * qx + qy * qy);
+                if ((double)lb >= o.w * lifetime) continue;
+            }
+            const double t = d_first_cross_time<CW>(o, hx, hy, rx, ry);
+            lifetime = (t < lifetime) ? t : lifetime;
+            st.cross_exact += 1;
+        }
+    }
+#else
+    // Every impurity of the cell: CirclesCross (details.h:141-151), then
+    // GetFirstCrossTime for the crossing ones in impurity order, keeping the
+    // smallest time (lifetime.h:117-123).  Four impurities' loads in flight.
     double lifetime = st.lifetime;
     for (int i = s; i < e; i += 4) {
-        // Four impurities' loads in flight, CirclesCross on each (details.h:141-151),
-        // then GetFirstCrossTime for the crossing ones in impurity order.
         double2 p[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) p[u] = (i + u < e) ? g.ixy[i + u] : make_double2(0.0, 0.0);
@@ -260,8 +378,10 @@ __device__ __forceinline__ bool d_step(const Orb& o, double phi0, const DevGrid&
             const double t = d_first_cross_time<CW>(o, hx, hy, rx, ry);
             lifetime = (t < lifetime) ? t : lifetime;
             st.cross_calls += 1;
+            st.cross_exact += 1;
         }
     }
+#endif
     st.lifetime = lifetime;
     if (lifetime < ES_INF) {
         flags = 0;

@@ -91,6 +91,8 @@ typedef struct es_metrics {
     int64_t edge_hits;                /* accepted cell-edge intersections (cell_grid.h:80-89 calls) */
     int64_t cross_calls;              /* GetFirstCrossTime calls (details.h:182) */
     int64_t sigma_evals;              /* GetSigma evaluations with a non-zero truncated lifetime */
+    int64_t cross_exact;              /* GetFirstCrossTime calls evaluated exactly; the others were
+                                         screened out by a bound that proves they cannot lower the minimum */
 } es_metrics;
 
 /* Simulation(positions_per_row, particles_per_quadrant, GridInformation) (Simulation.h:131-146). */
