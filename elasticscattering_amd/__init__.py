@@ -7,6 +7,10 @@ Python mirror of the reference's operator API over the C ABI in
   ``init_sample`` / ``compute_sigmas`` like ``InitSample`` / ``ComputeSigmas``;
   its ``set_impurities`` / ``generate_impurities`` are the two ``Grid``
   constructors (src/sim/Grid.h:93-94), binned on the GPU and kept there.
+* :class:`Batch` — the sample loop of SimulationRunner::RunSimulation
+  (src/SimulationRunner.cpp:88-98) as one launch per stage over many samples
+  of uneven size, optionally replayed from a captured hipGraph, with a tuner
+  for the trace launch shape.
 
 The HIP library ``libes_hip.so`` is built in-tree by
 ``python -m elasticscattering_amd.build``; importing the API without it raises.
@@ -18,7 +22,8 @@ import numpy as np
 
 __all__ = [
     "EsError", "GridInfo", "GridDesc", "Settings", "Sigma", "Metrics", "SolverDesc",
-    "Simulation", "load_library", "parse_config", "run", "LIB_PATH", "EXPORTED_SYMBOLS",
+    "BatchSample", "BatchShape", "Simulation", "Batch", "load_library", "parse_config", "run", "LIB_PATH",
+    "EXPORTED_SYMBOLS",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -113,6 +118,31 @@ class SolverDesc(C.Structure):
     ]
 
 
+class BatchSample(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("impurity_count", C.c_int64),
+        ("grid", GridDesc),
+        ("grid_cell_size", C.c_double),
+        ("positions_per_row", C.c_int32),
+        ("particles_per_quadrant", C.c_int32),
+        ("settings", Settings),
+        ("coherent", C.c_int32),
+        ("share_field_of", C.c_int32),
+        ("fields", C.POINTER(C.c_double)),
+        ("n_fields", C.c_int32),
+        ("temperatures", C.POINTER(C.c_double)),
+        ("n_temperatures", C.c_int32),
+    ]
+
+
+class BatchShape(C.Structure):
+    _fields_ = [("waves_per_eu", C.c_int32), ("chunk", C.c_int32), ("grid_mult", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _PD = C.POINTER(C.c_double)
 _P = C.c_void_p
 
@@ -128,6 +158,17 @@ _SIGNATURES = {
     "es_solver_compute_sigmas": (C.c_int, [_P, C.c_double, _PD, C.c_int32, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_solver_compute_sigmas_sweep": (C.c_int, [_P, _PD, C.c_int32, _PD, C.c_int32, C.POINTER(Sigma),
                                                  C.POINTER(Metrics)]),
+    "es_batch_create": (C.c_int, [C.c_int32, C.POINTER(_P)]),
+    "es_batch_destroy": (None, [_P]),
+    "es_batch_last_error": (C.c_char_p, [_P]),
+    "es_batch_add_sample": (C.c_int, [_P, C.POINTER(BatchSample), C.POINTER(C.c_int32)]),
+    "es_batch_set_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
+    "es_batch_get_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
+    "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
+    "es_batch_run": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
+    "es_batch_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int64)]),
+    "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
     "es_version": (C.c_char_p, []),
     # include/es_runner.h
     "es_config_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
@@ -276,6 +317,118 @@ class Simulation:
     def close(self):
         if getattr(self, "h", None):
             self.lib.es_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many samples of uneven size traced in one launch per stage.
+
+    Each :meth:`add` is one pass (coherent or incoherent) of one field: its
+    impurities are drawn with the es_rng.h generator and binned on the GPU
+    (Grid.cpp:28-59), or taken from an earlier entry (``share_field_of``).
+    :meth:`run` returns, per entry, sigma [n_fields, n_temperatures, 2] — the
+    values ``Simulation.compute_sweep`` gives for the same sample.
+    """
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.es_batch_create(int(device), C.byref(h))
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_batch_last_error(None).decode())
+        self.h = h
+        self.entries = []  # (n_fields, n_temperatures)
+
+    def _check(self, rc):
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_batch_last_error(self.h).decode())
+
+    def add(self, seed, region_size, region_extends, impurity_radius, positions_per_row, particles_per_quadrant,
+            fields, temperatures, tau, alpha, is_clockwise, particle_speed, coherent, incoherent_tau=0.0,
+            density=0.0, count=0, target_cell_population=15, cells_per_row=0, grid_cell_size=0.0,
+            share_field_of=-1):
+        """Adds one sample pass; returns its entry index."""
+        Bs = np.ascontiguousarray(fields, dtype=np.float64)
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        s = BatchSample()
+        s.seed = int(seed)
+        s.impurity_count = int(count)
+        s.grid = GridDesc(region_size, region_extends, impurity_radius, density, int(target_cell_population),
+                          int(cells_per_row))
+        s.grid_cell_size = float(grid_cell_size)
+        s.positions_per_row = int(positions_per_row)
+        s.particles_per_quadrant = int(particles_per_quadrant)
+        s.settings = Settings(tau, alpha, int(bool(is_clockwise)), particle_speed, incoherent_tau)
+        s.coherent = int(bool(coherent))
+        s.share_field_of = int(share_field_of)
+        s.fields = _dptr(Bs)
+        s.n_fields = len(Bs)
+        s.temperatures = _dptr(T) if len(T) else None
+        s.n_temperatures = len(T)
+        idx = C.c_int32(-1)
+        self._check(self.lib.es_batch_add_sample(self.h, C.byref(s), C.byref(idx)))
+        self.entries.append((len(Bs), len(T)))
+        return idx.value
+
+    def counts(self):
+        ns, npr, no = C.c_int32(), C.c_int32(), C.c_int32()
+        ni = C.c_int64()
+        self._check(self.lib.es_batch_counts(self.h, C.byref(ns), C.byref(npr), C.byref(no), C.byref(ni)))
+        return {"samples": ns.value, "pairs": npr.value, "outputs": no.value, "traced_items": ni.value}
+
+    @property
+    def shape(self):
+        sh = BatchShape()
+        self._check(self.lib.es_batch_get_shape(self.h, C.byref(sh)))
+        return sh.as_dict()
+
+    def set_shape(self, waves_per_eu=3, chunk=128, grid_mult=1):
+        sh = BatchShape(int(waves_per_eu), int(chunk), int(grid_mult))
+        self._check(self.lib.es_batch_set_shape(self.h, C.byref(sh)))
+
+    def set_capture(self, enable):
+        self._check(self.lib.es_batch_set_capture(self.h, int(bool(enable))))
+
+    def tune(self):
+        """Times every trace launch shape on this batch; keeps and returns the fastest (shape, seconds)."""
+        sh = BatchShape()
+        t = C.c_double()
+        self._check(self.lib.es_batch_tune(self.h, C.byref(sh), C.byref(t)))
+        return sh.as_dict(), t.value
+
+    def run(self, metrics=True):
+        """([sigma [nB, nT, 2] per entry], [[metrics dict per field] per entry] | None, device seconds)."""
+        nout = sum(b * t for b, t in self.entries)
+        npairs = sum(b for b, _ in self.entries)
+        out = (Sigma * max(nout, 1))()
+        ms = (Metrics * max(npairs, 1))() if metrics else None
+        secs = C.c_double()
+        self._check(self.lib.es_batch_run(self.h, out, ms, C.byref(secs)))
+        flat = np.frombuffer(out, dtype=np.float64, count=2 * nout) if nout else np.empty(0)
+        sig, mets, o, p = [], [] if metrics else None, 0, 0
+        for nB, nT in self.entries:
+            sig.append(flat[2 * o: 2 * (o + nB * nT)].reshape(nB, nT, 2).copy())
+            o += nB * nT
+            if metrics:
+                mets.append([ms[p + b].as_dict() for b in range(nB)])
+            p += nB
+        return sig, mets, secs.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.es_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,7 +1,13 @@
-// es_abi.hip — the C ABI (include/es_abi.h): a solver that owns device
-// memory and one HIP stream, and drives the kernels of es_kernels.hip in the
-// order of the reference's Simulation::ComputeSigmas
-// (src/sim/SimulationCPU.cpp:33-73; src/sim/SimulationCL.cpp:72-91).
+// es_abi.hip — the C ABI (include/es_abi.h).
+//
+// es_solver: one sample at a time, the operator API of the reference's
+//   Simulation (src/sim/Simulation.h:56-147) in the order of
+//   Simulation::ComputeSigmas (src/sim/SimulationCPU.cpp:33-73).
+// es_batch: many samples of uneven size (own fields, grids, fields B, T) in one
+//   launch of each stage, optionally replayed from a captured hipGraph, with a
+//   tuner for the trace kernel's launch shape.
+// Both own their device memory and one HIP stream, and share the stage runner
+// below (binning, start-cell test, trace, temperature stage, final sums).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,114 +41,173 @@ struct DevBuf {
     size_t cap = 0;
 };
 
-}  // namespace
-
-struct es_solver {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    es_solver_desc desc{};
-    std::string err;
-    size_t byte_cap = 0;  // 0: unlimited; else refuse allocations beyond this total
-    size_t bytes_held = 0;
-
-    // grid
-    bool has_grid = false;
-    es::GridDev g;
-    int64_t unique = 0;
-    DevBuf in_x, in_y, ixy, cell_end;
-    DevBuf b_targets, b_ntarg, b_counts, b_fill, b_begin, b_bsum, b_order;
-
-    // positions (per grid and solver)
-    bool pos_valid = false;
-    int nfree = 0;
-    int64_t n_inside_positions = 0;
-    DevBuf flags, bcount, free_pos, small_i32;  // small_i32: [0] err, [1] nfree
-
-    // sample (InitSample)
-    bool has_sample = false;
-    es_settings st{};
-    int coherent = 0;
-    double integrand_angle_area = 0, phi_integrand_factor = 0, phi_start = 0, phi_step = 0;
-
-    // per call
-    DevBuf w, factor, dpart, mpart, sig, ltsum, mout, lifetimes;
-    DevBuf lt_item, m_item, x_item, f_item, queue;
-    int cus = 0;                      // compute units of the device
-    int trace_bpc[2] = {0, 0};        // resident trace blocks per CU (counter-clockwise, clockwise)
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-};
-
-namespace {
-
-int fail(es_solver* s, int code, const std::string& msg)
-{
-    if (s) s->err = msg;
-    return code;
-}
-
-int hip_fail(es_solver* s, hipError_t e, const char* what)
-{
-    return fail(s, ES_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define ES_HIP(s, call)                                 \
-    do {                                                \
-        hipError_t _e = (call);                         \
-        if (_e != hipSuccess) return hip_fail(s, _e, #call); \
-    } while (0)
-
-int ensure(es_solver* s, DevBuf& b, size_t bytes, const char* what)
-{
-    if (bytes == 0) bytes = 8;
-    if (b.cap >= bytes) return ES_OK;
-    if (s->byte_cap && s->bytes_held - b.cap + bytes > s->byte_cap)
-        return fail(s, ES_ERR_OOM, std::string("device size cap exceeded allocating ") + what);
-    if (b.p) {
-        (void)hipFree(b.p);
-        s->bytes_held -= b.cap;
-        b.p = nullptr;
-        b.cap = 0;
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(s, ES_ERR_OOM, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
-    }
-    b.cap = bytes;
-    s->bytes_held += bytes;
-    return ES_OK;
-}
-
-void release(DevBuf& b)
-{
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-}
-
 template <typename T>
 T* P(DevBuf& b)
 {
     return reinterpret_cast<T*>(b.p);
 }
 
+// Stream, events, allocation accounting and the last error of one owner.
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t byte_cap = 0;  // 0: unlimited; else refuse allocations beyond this total
+    size_t bytes_held = 0;
+    int cus = 0;
+    int trace_bpc[2][4] = {{0}};  // resident trace blocks per CU [clockwise][waves_per_eu]
+    long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
+    std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
+};
+
+
+
+int fail(Ctx& c, int code, const std::string& msg)
+{
+    c.err = msg;
+    return code;
+}
+
+int hip_fail(Ctx& c, hipError_t e, const char* what)
+{
+    return fail(c, ES_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Grows the event pool to n events (not while a stream is being captured).
+int ctx_events(Ctx& c, size_t n)
+{
+    while (c.ev.size() < n) {
+        hipEvent_t e = nullptr;
+        const hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return fail(c, ES_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(r));
+        c.ev.push_back(e);
+    }
+    return ES_OK;
+}
+
+#define ES_HIP(c, call)                                        \
+    do {                                                       \
+        hipError_t _e = (call);                                \
+        if (_e != hipSuccess) return hip_fail(c, _e, #call);   \
+    } while (0)
+
+int ensure(Ctx& c, DevBuf& b, size_t bytes, const char* what)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return ES_OK;
+    if (c.byte_cap && c.bytes_held - b.cap + bytes > c.byte_cap)
+        return fail(c, ES_ERR_OOM, std::string("device size cap exceeded allocating ") + what);
+    if (b.p) {
+        (void)hipFree(b.p);
+        c.bytes_held -= b.cap;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, ES_ERR_OOM, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+    }
+    b.cap = bytes;
+    c.bytes_held += bytes;
+    return ES_OK;
+}
+
+void release(Ctx& c, DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    c.bytes_held -= std::min(c.bytes_held, b.cap);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int ctx_init(Ctx& c, int device, std::string& err)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        err = "no HIP device";
+        return ES_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        err = "device ordinal out of range";
+        return ES_ERR_INVALID_ARG;
+    }
+    c.device = device;
+    if (const char* cap = std::getenv("ES_MAX_DEVICE_BYTES")) c.byte_cap = (size_t)std::strtoull(cap, nullptr, 10);
+    if (const char* si = std::getenv("ES_SEGMENT_ITEMS")) {
+        const long long v = std::strtoll(si, nullptr, 10);
+        if (v > 0) c.seg_items = v;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreate(&ev);
+        if (e == hipSuccess) c.ev.push_back(ev);
+    }
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        err = std::string("HIP setup failed: ") + hipGetErrorString(e);
+        return ES_ERR_HIP;
+    }
+    return ES_OK;
+}
+
+void ctx_destroy(Ctx& c)
+{
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (hipEvent_t ev : c.ev)
+        if (ev) (void)hipEventDestroy(ev);
+    c.ev.clear();
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
 // Grid.cpp:40,54 with the optional explicit count.
 int cells_per_row_rule(int64_t unique, int target, int override_cpr)
 {
     if (override_cpr > 0) return override_cpr;
-    const int c = (int)std::round(std::sqrt(unique / (double)target));
-    return std::max(c, 1);
+    const int cpr = (int)std::round(std::sqrt(unique / (double)target));
+    return std::max(cpr, 1);
 }
 
-// Bins the centres in s->in_x / s->in_y (n of them) into the current grid.
-int bin_uploaded(es_solver* s, int64_t n, const es_grid_desc* d, es_grid_info* out)
+// ------------------------------------------------------------------ field
+
+// One binned impurity field and its start-cell test (Grid + the position part
+// of TraceOrbit, lifetime.h:84-99, hoisted: it depends on neither phi nor B).
+struct Field {
+    DevBuf in_x, in_y, ixy, cell_end;
+    DevBuf b_targets, b_ntarg, b_counts, b_fill, b_begin, b_bsum, b_order;
+    DevBuf flags, bcount, free_pos, small;  // small: [0] out-of-grid flag, [1] nfree
+    es::GridDev g;
+    int64_t unique = 0;
+    bool has_grid = false;
+    // start test of the current positions
+    bool pos_valid = false;
+    int pos_ppr = 0;
+    double pos_dist = 0, pos_offx = 0, pos_offy = 0;
+    int nfree = 0;
+    int64_t n_inside = 0;
+};
+
+void field_release(Ctx& c, Field& f)
+{
+    DevBuf* bufs[] = {&f.in_x, &f.in_y, &f.ixy, &f.cell_end, &f.b_targets, &f.b_ntarg, &f.b_counts, &f.b_fill,
+                      &f.b_begin, &f.b_bsum, &f.b_order, &f.flags, &f.bcount, &f.free_pos, &f.small};
+    for (DevBuf* b : bufs) release(c, *b);
+}
+
+// Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180).
+int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* out)
 {
     if (d->target_cell_population <= 0 && d->cells_per_row <= 0)
-        return fail(s, ES_ERR_INVALID_ARG, "target_cell_population must be > 0");
-    if (!(d->impurity_radius >= 0)) return fail(s, ES_ERR_INVALID_ARG, "impurity_radius must be >= 0");
+        return fail(c, ES_ERR_INVALID_ARG, "target_cell_population must be > 0");
+    if (!(d->impurity_radius >= 0)) return fail(c, ES_ERR_INVALID_ARG, "impurity_radius must be >= 0");
+    if (!(d->region_size + 2 * d->region_extends > 0)) return fail(c, ES_ERR_INVALID_ARG, "empty spawn region");
     const int cpr = cells_per_row_rule(n, d->target_cell_population, d->cells_per_row);
-    if ((int64_t)cpr * cpr > (int64_t)1 << 28) return fail(s, ES_ERR_INVALID_ARG, "cell grid too large");
-    es::GridDev& g = s->g;
+    if ((int64_t)cpr * cpr > (int64_t)1 << 28) return fail(c, ES_ERR_INVALID_ARG, "cell grid too large");
+    es::GridDev& g = f.g;
     g.cpr = cpr;
     g.lo = -d->region_extends;                       // Grid.cpp:52
     g.hi = d->region_size + d->region_extends;
@@ -151,46 +216,34 @@ int bin_uploaded(es_solver* s, int64_t n, const es_grid_desc* d, es_grid_info* o
     g.radius = d->impurity_radius;
     const int64_t ncell = (int64_t)cpr * cpr;
     int rc;
-    if ((rc = ensure(s, s->b_targets, 9 * n * sizeof(int32_t), "bin targets"))) return rc;
-    if ((rc = ensure(s, s->b_ntarg, n, "bin target counts"))) return rc;
-    if ((rc = ensure(s, s->b_counts, ncell * sizeof(int32_t), "cell counts"))) return rc;
-    if ((rc = ensure(s, s->b_fill, ncell * sizeof(int32_t), "cell fill"))) return rc;
-    if ((rc = ensure(s, s->b_begin, (ncell + 1) * sizeof(int64_t), "cell begin"))) return rc;
-    if ((rc = ensure(s, s->b_bsum, ((ncell + 255) / 256) * sizeof(int64_t), "scan block sums"))) return rc;
-    if ((rc = ensure(s, s->cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
-    g.cell_end = P<int32_t>(s->cell_end);
-    es::BinScratch bs{P<int32_t>(s->b_targets), P<uint8_t>(s->b_ntarg), P<int32_t>(s->b_counts),
-                      P<int32_t>(s->b_fill), P<int64_t>(s->b_begin), P<int64_t>(s->b_bsum), nullptr};
-    const double* x = P<double>(s->in_x);
-    const double* y = P<double>(s->in_y);
-    ES_HIP(s, es::launch_bin_count(x, y, n, g, bs, s->stream));
-    ES_HIP(s, es::launch_bin_scan(g, bs, s->stream));
+    if ((rc = ensure(c, f.b_targets, 9 * n * sizeof(int32_t), "bin targets"))) return rc;
+    if ((rc = ensure(c, f.b_ntarg, n, "bin target counts"))) return rc;
+    if ((rc = ensure(c, f.b_counts, ncell * sizeof(int32_t), "cell counts"))) return rc;
+    if ((rc = ensure(c, f.b_fill, ncell * sizeof(int32_t), "cell fill"))) return rc;
+    if ((rc = ensure(c, f.b_begin, (ncell + 1) * sizeof(int64_t), "cell begin"))) return rc;
+    if ((rc = ensure(c, f.b_bsum, ((ncell + 255) / 256) * sizeof(int64_t), "scan block sums"))) return rc;
+    if ((rc = ensure(c, f.cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
+    g.cell_end = P<int32_t>(f.cell_end);
+    es::BinScratch bs{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts), P<int32_t>(f.b_fill),
+                      P<int64_t>(f.b_begin),   P<int64_t>(f.b_bsum),   nullptr};
+    const double* x = P<double>(f.in_x);
+    const double* y = P<double>(f.in_y);
+    ES_HIP(c, es::launch_bin_count(x, y, n, g, bs, c.stream));
+    ES_HIP(c, es::launch_bin_scan(g, bs, c.stream));
     int64_t total = 0;
-    ES_HIP(s, hipMemcpyAsync(&total, P<int64_t>(s->b_begin) + ncell, sizeof(int64_t), hipMemcpyDeviceToHost,
-                             s->stream));
-    ES_HIP(s, hipStreamSynchronize(s->stream));
-    if (total > INT32_MAX) return fail(s, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
-    if ((rc = ensure(s, s->b_order, total * sizeof(int32_t), "cell order"))) return rc;
-    if ((rc = ensure(s, s->ixy, total * sizeof(double2), "impurities"))) return rc;
-    bs.order = P<int32_t>(s->b_order);
-    g.ixy = P<double2>(s->ixy);
+    ES_HIP(c, hipMemcpyAsync(&total, P<int64_t>(f.b_begin) + ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    if (total > INT32_MAX) return fail(c, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
+    if ((rc = ensure(c, f.b_order, total * sizeof(int32_t), "cell order"))) return rc;
+    if ((rc = ensure(c, f.ixy, total * sizeof(double2), "impurities"))) return rc;
+    bs.order = P<int32_t>(f.b_order);
+    g.ixy = P<double2>(f.ixy);
     g.total = total;
-    ES_HIP(s, es::launch_bin_place(x, y, n, g, bs, s->stream));
-    ES_HIP(s, hipStreamSynchronize(s->stream));
-    s->unique = n;
-    s->has_grid = true;
-    s->pos_valid = false;
-    // GridInformation of sample 0's grid when the solver was created without one
-    // (SimulationRunner.cpp:75-84 builds the Simulation from the first grid).
-    if (!(s->desc.grid_info.cell_size > 0)) {
-        s->desc.grid_info.cell_size = g.cell_size;
-        s->desc.grid_info.region_size = d->region_size;
-        s->desc.grid_info.region_extends = d->region_extends;
-        s->desc.grid_info.indexed_impurity_count = total;
-        s->desc.grid_info.unique_impurity_count = n;
-        s->desc.grid_info.index_size = (int32_t)ncell;
-        s->desc.grid_info.cells_per_row = cpr;
-    }
+    ES_HIP(c, es::launch_bin_place(x, y, n, g, bs, c.stream));
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    f.unique = n;
+    f.has_grid = true;
+    f.pos_valid = false;
     if (out) {
         out->region_size = d->region_size;
         out->region_extends = d->region_extends;
@@ -203,36 +256,73 @@ int bin_uploaded(es_solver* s, int64_t n, const es_grid_desc* d, es_grid_info* o
     return ES_OK;
 }
 
-// Start-cell test per position, once per grid (lifetime.h:84-99).
-int ensure_positions(es_solver* s)
+int field_upload(Ctx& c, Field& f, const double* x, const double* y, int64_t n, const es_grid_desc* d,
+                 es_grid_info* out)
 {
-    if (s->pos_valid) return ES_OK;
-    const int ppr = s->desc.positions_per_row;
+    if (!d || n < 0 || (n > 0 && (!x || !y))) return fail(c, ES_ERR_INVALID_ARG, "bad impurity arguments");
+    if (n > 10000000) return fail(c, ES_ERR_INVALID_ARG, "more than 1e7 impurities (Grid.cpp:36)");
+    int rc;
+    if ((rc = ensure(c, f.in_x, n * sizeof(double), "impurity input x"))) return rc;
+    if ((rc = ensure(c, f.in_y, n * sizeof(double), "impurity input y"))) return rc;
+    if (n > 0) {
+        ES_HIP(c, hipMemcpyAsync(f.in_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        ES_HIP(c, hipMemcpyAsync(f.in_y.p, y, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+    return field_bin(c, f, n, d, out);
+}
+
+int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid_desc* d, es_grid_info* out)
+{
+    if (!d) return fail(c, ES_ERR_INVALID_ARG, "null grid description");
+    const double lo = -d->region_extends, hi = d->region_size + d->region_extends;
+    int64_t n = count;
+    if (n <= 0) {  // Grid.cpp:33-39
+        const double area_length = hi - lo;
+        const double proposal = area_length * area_length * d->impurity_density;
+        if (proposal > 1e7) return fail(c, ES_ERR_INVALID_ARG, "impurity count proposal > 1e7 (Grid.cpp:36)");
+        if (!(proposal >= 1)) return fail(c, ES_ERR_INVALID_ARG, "no impurities (Grid.cpp:37)");
+        n = std::max<int64_t>(1, (int64_t)std::ceil(proposal));
+    }
+    if (n > 10000000) return fail(c, ES_ERR_INVALID_ARG, "more than 1e7 impurities (Grid.cpp:36)");
+    int rc;
+    if ((rc = ensure(c, f.in_x, n * sizeof(double), "impurity input x"))) return rc;
+    if ((rc = ensure(c, f.in_y, n * sizeof(double), "impurity input y"))) return rc;
+    ES_HIP(c, es::launch_generate(esr_key(seed), n, lo, hi, P<double>(f.in_x), P<double>(f.in_y), c.stream));
+    return field_bin(c, f, n, d, out);
+}
+
+// Start cell + inside test for every position of a ppr x ppr grid.
+int field_positions(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
+{
+    if (f.pos_valid && f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy)
+        return ES_OK;
     const int64_t n = (int64_t)ppr * ppr;
     int rc;
-    if ((rc = ensure(s, s->flags, n, "position flags"))) return rc;
-    if ((rc = ensure(s, s->free_pos, n * sizeof(int32_t), "free positions"))) return rc;
-    if ((rc = ensure(s, s->bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
-    if ((rc = ensure(s, s->small_i32, 2 * sizeof(int32_t), "flags"))) return rc;
-    // Simulation.h:142-143
-    const double dist = s->desc.grid_info.region_size / (double)(ppr - 1);
-    const double offx = s->desc.grid_info.cell_size * 0.01;
-    const double offy = s->desc.grid_info.cell_size * 0.005;
-    int32_t* small = P<int32_t>(s->small_i32);
-    ES_HIP(s, es::launch_positions(s->g, ppr, dist, offx, offy, P<uint8_t>(s->flags), P<int32_t>(s->bcount), small,
-                                   P<int32_t>(s->free_pos), small + 1, s->stream));
+    if ((rc = ensure(c, f.flags, n, "position flags"))) return rc;
+    if ((rc = ensure(c, f.free_pos, n * sizeof(int32_t), "free positions"))) return rc;
+    if ((rc = ensure(c, f.bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
+    if ((rc = ensure(c, f.small, 2 * sizeof(int32_t), "flags"))) return rc;
+    int32_t* small = P<int32_t>(f.small);
+    ES_HIP(c, es::launch_positions(f.g, ppr, dist, offx, offy, P<uint8_t>(f.flags), P<int32_t>(f.bcount), small,
+                                   P<int32_t>(f.free_pos), small + 1, c.stream));
     int32_t host[2] = {0, 0};
-    ES_HIP(s, hipMemcpyAsync(host, small, sizeof(host), hipMemcpyDeviceToHost, s->stream));
-    ES_HIP(s, hipStreamSynchronize(s->stream));
+    ES_HIP(c, hipMemcpyAsync(host, small, sizeof(host), hipMemcpyDeviceToHost, c.stream));
+    ES_HIP(c, hipStreamSynchronize(c.stream));
     if (host[0] != 0)
-        return fail(s, ES_ERR_OUT_OF_GRID,
+        return fail(c, ES_ERR_OUT_OF_GRID,
                     "a start position lies outside the impurity cell grid (region_extends smaller than the "
                     "position offset); the reference would read out of bounds (lifetime.h:88-91)");
-    s->nfree = host[1];
-    s->n_inside_positions = n - host[1];
-    s->pos_valid = true;
+    f.nfree = host[1];
+    f.n_inside = n - host[1];
+    f.pos_ppr = ppr;
+    f.pos_dist = dist;
+    f.pos_offx = offx;
+    f.pos_offy = offy;
+    f.pos_valid = true;
     return ES_OK;
 }
+
+// ----------------------------------------------------------------- stages
 
 // Simulation.h:69-76
 double sigma_factor(double speed, double w, double tau)
@@ -245,202 +335,459 @@ double sigma_factor(double speed, double w, double tau)
 }
 
 // Simulation.h:88
-double integrand_factor(const es_solver* s, double w, double tau)
+double integrand_factor(int ppr, double phi_integrand_factor, double speed, double w, double tau)
 {
-    const double a = 1.0 / (3.0 * (s->desc.positions_per_row - 1));
-    return a * a * s->phi_integrand_factor * sigma_factor(s->st.particle_speed, w, tau) * 1e-8;
+    const double a = 1.0 / (3.0 * (ppr - 1));
+    return a * a * phi_integrand_factor * sigma_factor(speed, w, tau) * 1e-8;
 }
 
-int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_sigma* out, es_metrics* metrics,
-            double* lifetimes_host)
+// InitSample quantities (Simulation.h:111-126).
+struct PassSettings {
+    double integrand_angle_area, phi_integrand_factor, phi_start, phi_step;
+};
+
+PassSettings pass_settings(int ppq, double alpha, int coherent)
 {
-    if (!s) return ES_ERR_INVALID_ARG;
-    if (!s->has_grid) return fail(s, ES_ERR_NO_GRID, "no grid set");
-    if (!s->has_sample) return fail(s, ES_ERR_NO_SAMPLE, "es_solver_init_sample not called");
-    if (!(s->desc.grid_info.region_size > 0)) return fail(s, ES_ERR_INVALID_ARG, "grid_info.region_size must be > 0");
-    if (nB <= 0 || !B) return fail(s, ES_ERR_INVALID_ARG, "need at least one field");
-    if (nT < 0 || (nT > 0 && (!T || !out))) return fail(s, ES_ERR_INVALID_ARG, "bad temperature arguments");
-    for (int b = 0; b < nB; b++)
-        if (!(B[b] > 0) || !std::isfinite(B[b])) return fail(s, ES_ERR_INVALID_ARG, "magnetic field must be > 0");
-    // GetTau (Simulation.h:90) for each T.
-    std::vector<double> taus(nT);
+    PassSettings p;
+    const double base_area = alpha * 2.0;
+    p.integrand_angle_area = !coherent ? base_area : (kHALF_PI - base_area);
+    p.phi_integrand_factor = p.integrand_angle_area / ((ppq - 1) * 3.0);
+    p.phi_start = !coherent ? -alpha : alpha;
+    p.phi_step = p.integrand_angle_area / (ppq - 1);
+    return p;
+}
+
+// GetTau (Simulation.h:90) with the explicit incoherent tau.
+int taus_for(Ctx& c, const es_settings& st, int coherent, const double* T, int nT, std::vector<double>& taus)
+{
+    taus.resize(nT);
     for (int t = 0; t < nT; t++) {
-        if (s->coherent)
-            taus[t] = s->st.tau;
-        else if (s->st.incoherent_tau > 0)
-            taus[t] = s->st.incoherent_tau;
+        if (coherent)
+            taus[t] = st.tau;
+        else if (st.incoherent_tau > 0)
+            taus[t] = st.incoherent_tau;
         else {
-            if (!(T[t] > 0)) return fail(s, ES_ERR_INVALID_ARG, "temperature must be > 0 (or set incoherent_tau)");
+            if (!(T[t] > 0)) return fail(c, ES_ERR_INVALID_ARG, "temperature must be > 0 (or set incoherent_tau)");
             taus[t] = kHBAR / (kKB * T[t]);
         }
-        if (!(taus[t] > 0) || !std::isfinite(taus[t])) return fail(s, ES_ERR_INVALID_ARG, "tau must be finite and > 0");
+        if (!(taus[t] > 0) || !std::isfinite(taus[t])) return fail(c, ES_ERR_INVALID_ARG, "tau must be finite and > 0");
     }
+    return ES_OK;
+}
+
+struct Shape {
+    int waves_per_eu = 3;
+    int chunk = 128;
+    int grid_mult = 1;
+};
+
+// Device buffers and the optional captured graph of the stage kernels.
+struct Stages {
+    DevBuf samples, pairs, factor, dpart, mpart, sig, ltsum, mout, lt_item, m_item, x_item, f_item, queue, lifetimes;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void stages_release(Ctx& c, Stages& s)
+{
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.graph) (void)hipGraphDestroy(s.graph);
+    s.exec = nullptr;
+    s.graph = nullptr;
+    DevBuf* bufs[] = {&s.samples, &s.pairs, &s.factor, &s.dpart,  &s.mpart,  &s.sig,    &s.ltsum,
+                      &s.mout,    &s.lt_item, &s.m_item, &s.x_item, &s.f_item, &s.queue, &s.lifetimes};
+    for (DevBuf* b : bufs) release(c, *b);
+}
+
+// A run of whole pairs launched together; item and block offsets of its pairs
+// count from the segment's start, so one set of per-item buffers serves all.
+struct Segment {
+    int p0 = 0, p1 = 0;       // pairs [p0, p1)
+    long long items = 0;
+    int blocks = 0;           // temperature-stage blocks
+};
+
+// Host description of one stage run: samples, pairs (sorted by item and block
+// offsets), the integrand factors [out][kMaxT], and the derived sizes.
+struct Plan {
+    std::vector<es::SampleDev> samples;
+    std::vector<es::PairDev> pairs;
+    std::vector<double> factor;
+    std::vector<Segment> segs;
+    int nout = 0;
+    int clockwise = 0;
+    int NT = 1;
+    long long total_items = 0;   // over all segments
+    long long max_items = 0;     // largest segment
+    int max_blocks = 0;
+    long long P = 0;  // lifetimes per pass, when raw lifetimes are returned
+};
+
+// Fills item / block offsets and cuts the pairs into segments of at most
+// seg_items work items (a pair larger than that is a segment of its own).
+void plan_offsets(Plan& pl, long long seg_items)
+{
+    long long item = 0;
+    int blk = 0;
+    int maxT = 1;
+    pl.segs.clear();
+    pl.total_items = pl.max_items = 0;
+    pl.max_blocks = 0;
+    Segment sg;
+    for (size_t i = 0; i < pl.pairs.size(); i++) {
+        es::PairDev& p = pl.pairs[i];
+        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nitems + es::kBlock - 1) / es::kBlock, 1024));
+        if ((int)i > sg.p0 && item + p.nitems > seg_items) {
+            sg.p1 = (int)i;
+            sg.items = item;
+            sg.blocks = blk;
+            pl.segs.push_back(sg);
+            sg = Segment();
+            sg.p0 = (int)i;
+            item = 0;
+            blk = 0;
+        }
+        p.item_off = item;
+        item += p.nitems;
+        p.nblk = nblk;
+        p.blk_off = blk;
+        blk += nblk;
+        pl.total_items += p.nitems;
+        maxT = std::max(maxT, pl.samples[p.sample].nT);
+    }
+    sg.p1 = (int)pl.pairs.size();
+    sg.items = item;
+    sg.blocks = blk;
+    pl.segs.push_back(sg);
+    for (const Segment& g : pl.segs) {
+        pl.max_items = std::max(pl.max_items, g.items);
+        pl.max_blocks = std::max(pl.max_blocks, g.blocks);
+    }
+    pl.NT = maxT <= 1 ? 1 : maxT <= 2 ? 2 : maxT <= 4 ? 4 : 8;
+}
+
+int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
+{
+    const int NQ = 2 * pl.NT + 1;
+    const size_t npairs = pl.pairs.size();
     int rc;
-    if ((rc = ensure_positions(s))) return rc;
+    if ((rc = ensure(c, s.samples, pl.samples.size() * sizeof(es::SampleDev), "sample table"))) return rc;
+    if ((rc = ensure(c, s.pairs, npairs * sizeof(es::PairDev), "pair table"))) return rc;
+    if ((rc = ensure(c, s.factor, (size_t)pl.nout * es::kMaxT * sizeof(double), "factors"))) return rc;
+    if ((rc = ensure(c, s.dpart, (size_t)pl.max_blocks * NQ * 2 * sizeof(double), "partials"))) return rc;
+    if ((rc = ensure(c, s.mpart, (size_t)pl.max_blocks * es::kMetricSlots * sizeof(int64_t), "metric partials")))
+        return rc;
+    if ((rc = ensure(c, s.sig, (size_t)pl.nout * es::kMaxT * 2 * sizeof(double), "sigma out"))) return rc;
+    if ((rc = ensure(c, s.ltsum, (size_t)pl.nout * sizeof(double), "lifetime sums"))) return rc;
+    if ((rc = ensure(c, s.mout, (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t), "metrics out"))) return rc;
+    const size_t items = (size_t)std::max<long long>(pl.max_items, 1);
+    if ((rc = ensure(c, s.lt_item, items * sizeof(double), "item lifetimes"))) return rc;
+    if ((rc = ensure(c, s.m_item, items * sizeof(int4), "item counters"))) return rc;
+    if ((rc = ensure(c, s.x_item, items * sizeof(int32_t), "item exact solves"))) return rc;
+    if ((rc = ensure(c, s.f_item, items, "item flags"))) return rc;
+    if ((rc = ensure(c, s.queue, sizeof(unsigned long long), "work queue"))) return rc;
+    if ((rc = ctx_events(c, 4 * pl.segs.size()))) return rc;
+    ES_HIP(c, hipMemcpyAsync(s.samples.p, pl.samples.data(), pl.samples.size() * sizeof(es::SampleDev),
+                             hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemcpyAsync(s.pairs.p, pl.pairs.data(), npairs * sizeof(es::PairDev), hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemcpyAsync(s.factor.p, pl.factor.data(), pl.factor.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c.stream));
+    return ES_OK;
+}
 
-    const int ppr = s->desc.positions_per_row, ppq = s->desc.particles_per_quadrant, ppp = 4 * ppq;
+// Trace grid size of every segment: enough resident blocks to fill the chip
+// (times grid_mult), never more blocks than the segment has items for.
+int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<int>& nblocks)
+{
+    const int w = (sh.waves_per_eu == 2) ? 2 : 3;
+    if (c.trace_bpc[cw][w] <= 0) {
+        ES_HIP(c, es::trace_blocks_per_cu(cw, w, &c.trace_bpc[cw][w]));
+        if (c.trace_bpc[cw][w] <= 0) c.trace_bpc[cw][w] = 1;
+    }
+    nblocks.clear();
+    for (const Segment& g : pl.segs) {
+        const long long want = (g.items + es::kBlock - 1) / es::kBlock;
+        nblocks.push_back((int)std::max<long long>(
+            1, std::min<long long>(want, (long long)c.trace_bpc[cw][w] * c.cus * std::max(1, sh.grid_mult))));
+    }
+    return ES_OK;
+}
+
+// Enqueues the stage kernels (no host copies: the sequence can be captured).
+// Enqueues the stage kernels of every segment (no host copies or memsets: the
+// sequence can be captured in a graph).  Events (4 per segment) only when not capturing.
+int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trace, int with_metrics,
+                   double* lifetimes_dev, const std::vector<int>& nblocks, bool events = true)
+{
+    for (size_t si = 0; si < pl.segs.size(); si++) {
+        const Segment& g = pl.segs[si];
+        es::PairDev* pairs = P<es::PairDev>(s.pairs) + g.p0;
+        const int npairs = g.p1 - g.p0;
+        if (trace) {
+            es::TraceArgs a{};
+            a.samples = P<es::SampleDev>(s.samples);
+            a.pairs = pairs;
+            a.npairs = npairs;
+            a.total_items = g.items;
+            a.queue = P<unsigned long long>(s.queue);
+            a.chunk = sh.chunk;
+            a.lt_item = P<double>(s.lt_item);
+            a.m_item = P<int4>(s.m_item);
+            a.x_item = P<int32_t>(s.x_item);
+            a.f_item = P<uint8_t>(s.f_item);
+            a.lifetimes = lifetimes_dev;
+            a.P = pl.P;
+            if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si], c.stream));
+            ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, nblocks[si], c.stream));
+            if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 1], c.stream));
+        }
+        es::SigmaArgs sa{};
+        sa.samples = P<es::SampleDev>(s.samples);
+        sa.pairs = pairs;
+        sa.npairs = npairs;
+        sa.total_blocks = g.blocks;
+        sa.clockwise = pl.clockwise;
+        sa.lt_item = P<double>(s.lt_item);
+        sa.m_item = P<int4>(s.m_item);
+        sa.x_item = P<int32_t>(s.x_item);
+        sa.f_item = P<uint8_t>(s.f_item);
+        sa.with_metrics = with_metrics;
+        sa.dpart = P<double>(s.dpart);
+        sa.mpart = P<int64_t>(s.mpart);
+        if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 2], c.stream));
+        ES_HIP(c, es::launch_sigma(sa, pl.NT, c.stream));
+        ES_HIP(c, es::launch_finalize(sa.samples, sa.pairs, npairs, pl.NT, with_metrics, sa.dpart, sa.mpart,
+                                      P<double>(s.factor), P<double>(s.sig), P<double>(s.ltsum),
+                                      P<int64_t>(s.mout), c.stream));
+        if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 3], c.stream));
+    }
+    return ES_OK;
+}
+
+struct StageOut {
+    std::vector<double> sig;     // [out][kMaxT][2]
+    std::vector<double> ltsum;   // [out]
+    std::vector<int64_t> m;      // [out][kMetricSlots]
+    double t_trace = 0, t_temp = 0;
+};
+
+int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, StageOut& o, bool events = true)
+{
+    o.sig.resize((size_t)pl.nout * es::kMaxT * 2);
+    o.ltsum.resize(pl.nout);
+    o.m.resize((size_t)pl.nout * es::kMetricSlots);
+    ES_HIP(c, hipMemcpyAsync(o.sig.data(), s.sig.p, o.sig.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (metrics) {
+        ES_HIP(c, hipMemcpyAsync(o.ltsum.data(), s.ltsum.p, o.ltsum.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c.stream));
+        ES_HIP(c, hipMemcpyAsync(o.m.data(), s.mout.p, o.m.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    }
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    float ms = 0;
+    o.t_trace = o.t_temp = 0;
+    if (!events) return ES_OK;
+    for (size_t si = 0; si < pl.segs.size(); si++) {
+        if (trace) {
+            ES_HIP(c, hipEventElapsedTime(&ms, c.ev[4 * si], c.ev[4 * si + 1]));
+            o.t_trace += ms * 1e-3;
+        }
+        ES_HIP(c, hipEventElapsedTime(&ms, c.ev[4 * si + 2], c.ev[4 * si + 3]));
+        o.t_temp += ms * 1e-3;
+    }
+    return ES_OK;
+}
+
+void fill_metrics(es_metrics& o, const int64_t* m, double ltsum, int64_t P, int64_t inside, double t_life,
+                  double t_temp)
+{
+    o.cells_passed = m[0];
+    o.impurities_tested = m[1];
+    o.particles_escaped = m[2];
+    o.particles_at_bound = m[3];
+    o.trace_cap_hits = m[4];
+    o.max_impurities_tested = m[5];
+    o.max_cells_passed = m[6];
+    o.particles_inside_impurity = inside;
+    o.real_particles = P - inside;
+    o.avg_particle_lifetime = P ? ltsum / (double)P : 0.0;
+    o.time_elapsed_lifetimes = t_life;
+    o.time_elapsed_temperatures = t_temp;
+    o.traced_particles = m[7];
+    o.edge_hits = m[8];
+    o.cross_calls = m[9];
+    o.sigma_evals = m[10];
+    o.cross_exact = m[11];
+}
+
+// One sample / pass as a SampleDev (tau and nT are filled by the caller).
+es::SampleDev sample_dev(Field& f, int ppr, int ppq, double dist, double offx, double offy, const PassSettings& ps,
+                         const es_settings& st, int coherent)
+{
+    es::SampleDev d{};
+    d.ixy = f.g.ixy;
+    d.cell_end = f.g.cell_end;
+    d.start = f.g.lo;
+    d.size = f.g.size;
+    d.L = f.g.size / (double)f.g.cpr;  // ImpuritySettings.cell_size (Grid.h:89)
+    d.radius = f.g.radius;
+    d.cpr = f.g.cpr;
+    d.free_pos = P<int32_t>(f.free_pos);
+    d.nfree = f.nfree;
+    d.ppr = ppr;
+    d.ppq = ppq;
+    d.dist = dist;
+    d.offx = offx;
+    d.offy = offy;
+    d.phi_start = ps.phi_start;
+    d.phi_step = ps.phi_step;
+    d.alpha = st.alpha;
+    d.speed = st.particle_speed;
+    d.coherent = coherent;
+    return d;
+}
+
+int check_settings(Ctx& c, const es_settings* st)
+{
+    if (!st) return fail(c, ES_ERR_INVALID_ARG, "null settings");
+    if (!(st->particle_speed > 0)) return fail(c, ES_ERR_INVALID_ARG, "particle_speed must be > 0");
+    if (!(st->alpha >= 0) || !(st->alpha <= kPI / 4 + 1e-12))
+        return fail(c, ES_ERR_INVALID_ARG, "alpha must lie in [0, pi/4]");
+    return ES_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- solver
+
+struct es_solver {
+    Ctx c;
+    es_solver_desc desc{};
+    Field f;
+    Stages s;
+    bool has_sample = false;
+    es_settings st{};
+    int coherent = 0;
+    PassSettings ps{};
+};
+
+struct es_batch {
+    Ctx c;
+    struct Entry {
+        int field;               // index into fields
+        int ppr, ppq;
+        double dist, offx, offy;
+        es_settings st;
+        int coherent;
+        PassSettings ps;
+        std::vector<double> B, T, taus;
+    };
+    std::vector<Field*> fields;
+    std::vector<Entry> entries;
+    Stages s;
+    Shape shape;
+    bool capture = false;
+    bool graph_ready = false;
+    std::vector<int> graph_blocks;
+    Plan plan;
+    bool plan_ready = false;
+};
+
+namespace {
+
+// ComputeSigmas for nB fields of the solver's current sample.
+int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int nT, es_sigma* out, es_metrics* metrics,
+                   double* lifetimes_host)
+{
+    Ctx& c = sv->c;
+    if (!sv->f.has_grid) return fail(c, ES_ERR_NO_GRID, "no grid set");
+    if (!sv->has_sample) return fail(c, ES_ERR_NO_SAMPLE, "es_solver_init_sample not called");
+    if (!(sv->desc.grid_info.region_size > 0)) return fail(c, ES_ERR_INVALID_ARG, "grid_info.region_size must be > 0");
+    if (nB <= 0 || !B) return fail(c, ES_ERR_INVALID_ARG, "need at least one field");
+    if (nT < 0 || (nT > 0 && (!T || !out))) return fail(c, ES_ERR_INVALID_ARG, "bad temperature arguments");
+    for (int b = 0; b < nB; b++)
+        if (!(B[b] > 0) || !std::isfinite(B[b])) return fail(c, ES_ERR_INVALID_ARG, "magnetic field must be > 0");
+    std::vector<double> taus;
+    int rc;
+    if ((rc = taus_for(c, sv->st, sv->coherent, T, nT, taus))) return rc;
+
+    const int ppr = sv->desc.positions_per_row, ppq = sv->desc.particles_per_quadrant, ppp = 4 * ppq;
+    const double dist = sv->desc.grid_info.region_size / (double)(ppr - 1);  // Simulation.h:142-143
+    const double offx = sv->desc.grid_info.cell_size * 0.01;
+    const double offy = sv->desc.grid_info.cell_size * 0.005;
+    if ((rc = field_positions(c, sv->f, ppr, dist, offx, offy))) return rc;
     const int64_t P_ = (int64_t)ppr * ppr * ppp;
-    const int64_t items = (int64_t)s->nfree * ppp;
-    const int64_t total_items = items * nB;
-    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((items + es::kBlock - 1) / es::kBlock, 1024));
-    const int nTc = std::max(1, std::min(nT, es::kMaxT));
-    const int nq = 2 * nTc + 1;
-    const int cw = s->st.is_clockwise ? 1 : 0;
 
-    if ((rc = ensure(s, s->w, nB * sizeof(double), "fields"))) return rc;
-    if ((rc = ensure(s, s->factor, (size_t)nB * es::kMaxT * sizeof(double), "factors"))) return rc;
-    if ((rc = ensure(s, s->dpart, (size_t)nB * nblk * nq * 2 * sizeof(double), "partials"))) return rc;
-    if ((rc = ensure(s, s->mpart, (size_t)nB * nblk * es::kMetricSlots * sizeof(int64_t), "metric partials"))) return rc;
-    if ((rc = ensure(s, s->sig, (size_t)nB * es::kMaxT * 2 * sizeof(double), "sigma out"))) return rc;
-    if ((rc = ensure(s, s->ltsum, nB * sizeof(double), "lifetime sums"))) return rc;
-    if ((rc = ensure(s, s->mout, (size_t)nB * es::kMetricSlots * sizeof(int64_t), "metrics out"))) return rc;
-    if ((rc = ensure(s, s->lt_item, total_items * sizeof(double), "item lifetimes"))) return rc;
-    if ((rc = ensure(s, s->m_item, total_items * sizeof(int4), "item counters"))) return rc;
-    if ((rc = ensure(s, s->f_item, total_items, "item flags"))) return rc;
-    if ((rc = ensure(s, s->x_item, total_items * sizeof(int32_t), "item exact solves"))) return rc;
-    if ((rc = ensure(s, s->queue, sizeof(unsigned long long), "work queue"))) return rc;
-    if (lifetimes_host) {
-        if ((rc = ensure(s, s->lifetimes, (size_t)nB * P_ * sizeof(double), "lifetimes"))) return rc;
-        ES_HIP(s, hipMemsetAsync(s->lifetimes.p, 0, (size_t)nB * P_ * sizeof(double), s->stream));
-    }
-    if (s->trace_bpc[cw] <= 0) {
-        ES_HIP(s, es::trace_blocks_per_cu(cw, &s->trace_bpc[cw]));
-        if (s->trace_bpc[cw] <= 0) s->trace_bpc[cw] = 1;
-    }
-
+    Plan pl;
+    pl.samples.push_back(sample_dev(sv->f, ppr, ppq, dist, offx, offy, sv->ps, sv->st, sv->coherent));
+    pl.clockwise = sv->st.is_clockwise ? 1 : 0;
+    pl.nout = nB;
     std::vector<double> w(nB);
-    for (int b = 0; b < nB; b++) w[b] = kE * B[b] / kM;  // SimulationCPU.cpp:107
-    ES_HIP(s, hipMemcpyAsync(s->w.p, w.data(), nB * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    for (int b = 0; b < nB; b++) {
+        w[b] = kE * B[b] / kM;  // SimulationCPU.cpp:107
+        es::PairDev p{};
+        p.sample = 0;
+        p.out = b;
+        p.w = w[b];
+        p.nitems = (long long)sv->f.nfree * ppp;
+        pl.pairs.push_back(p);
+    }
+    if (lifetimes_host) {
+        pl.P = P_;
+        if ((rc = ensure(c, sv->s.lifetimes, (size_t)nB * P_ * sizeof(double), "lifetimes"))) return rc;
+        ES_HIP(c, hipMemsetAsync(sv->s.lifetimes.p, 0, (size_t)nB * P_ * sizeof(double), c.stream));
+    }
 
-    // Lifetime stage: every (field, free position, phi) item once.
-    es::TraceArgs a{};
-    a.ixy = s->g.ixy;
-    a.cell_end = s->g.cell_end;
-    a.g_start = s->g.lo;
-    a.g_size = s->g.size;
-    a.g_L = s->g.size / (double)s->g.cpr;  // ImpuritySettings.cell_size (Grid.h:89)
-    a.g_radius = s->g.radius;
-    a.g_cpr = s->g.cpr;
-    a.free_pos = P<int32_t>(s->free_pos);
-    a.nfree = s->nfree;
-    a.ppr = ppr;
-    a.ppq = ppq;
-    a.dist = s->desc.grid_info.region_size / (double)(ppr - 1);
-    a.offx = s->desc.grid_info.cell_size * 0.01;
-    a.offy = s->desc.grid_info.cell_size * 0.005;
-    a.phi_start = s->phi_start;
-    a.phi_step = s->phi_step;
-    a.alpha = s->st.alpha;
-    a.speed = s->st.particle_speed;
-    a.coherent = s->coherent;
-    a.clockwise = cw;
-    a.nB = nB;
-    a.w = P<double>(s->w);
-    a.queue = P<unsigned long long>(s->queue);
-    a.chunk = 128;
-    a.lt_item = P<double>(s->lt_item);
-    a.m_item = P<int4>(s->m_item);
-    a.x_item = P<int32_t>(s->x_item);
-    a.f_item = P<uint8_t>(s->f_item);
-    a.lifetimes = lifetimes_host ? P<double>(s->lifetimes) : nullptr;
-    const int64_t want_blocks = (total_items + es::kBlock - 1) / es::kBlock;
-    const int tblocks = (int)std::max<int64_t>(1, std::min<int64_t>(want_blocks, (int64_t)s->trace_bpc[cw] * s->cus));
-
-    ES_HIP(s, hipEventRecord(s->ev[0], s->stream));
-    if (total_items > 0) ES_HIP(s, es::launch_trace(a, tblocks, s->stream));
-    ES_HIP(s, hipEventRecord(s->ev[1], s->stream));
-
-    // Temperature stage, at most kMaxT temperatures per launch.
-    es::SigmaArgs sa{};
-    sa.lt_item = a.lt_item;
-    sa.m_item = a.m_item;
-    sa.x_item = a.x_item;
-    sa.f_item = a.f_item;
-    sa.free_pos = a.free_pos;
-    sa.nfree = s->nfree;
-    sa.ppr = ppr;
-    sa.ppq = ppq;
-    sa.phi_start = s->phi_start;
-    sa.phi_step = s->phi_step;
-    sa.clockwise = cw;
-    sa.nB = nB;
-    sa.w = a.w;
-    sa.dpart = P<double>(s->dpart);
-    sa.mpart = P<int64_t>(s->mpart);
-    sa.nblk = nblk;
-
-    std::vector<double> sig((size_t)nB * es::kMaxT * 2);
-    std::vector<double> ltsum(nB);
-    std::vector<int64_t> mh((size_t)nB * es::kMetricSlots);
-    double t_temp = 0;
-
+    Shape sh;
+    std::vector<int> nblocks;
+    double t_life = 0, t_temp = 0;
+    std::vector<double> ltsum;
+    std::vector<int64_t> mh;
     const int nchunks = nT == 0 ? 1 : (nT + es::kMaxT - 1) / es::kMaxT;
-    for (int c = 0; c < nchunks; c++) {
-        const int t0 = c * es::kMaxT;
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int t0 = ch * es::kMaxT;
         const int nt = nT == 0 ? 0 : std::min(es::kMaxT, nT - t0);
-        sa.nT = nt;
-        sa.with_metrics = (c == 0) ? 1 : 0;
+        es::SampleDev& sd = pl.samples[0];
+        sd.nT = nt;
         for (int t = 0; t < es::kMaxT; t++) {
-            sa.tau[t] = t < nt ? taus[t0 + t] : 1.0;
-            sa.max_lt[t] = 15.0 * sa.tau[t];  // Simulation.h:91
+            sd.tau[t] = t < nt ? taus[t0 + t] : 1.0;
+            sd.max_lt[t] = 15.0 * sd.tau[t];  // Simulation.h:91
         }
-        std::vector<double> fac((size_t)nB * std::max(nt, 1), 0.0);
+        pl.factor.assign((size_t)nB * es::kMaxT, 0.0);
         for (int b = 0; b < nB; b++)
-            for (int t = 0; t < nt; t++) fac[(size_t)b * nt + t] = integrand_factor(s, w[b], taus[t0 + t]);
-        ES_HIP(s, hipMemcpyAsync(s->factor.p, fac.data(), fac.size() * sizeof(double), hipMemcpyHostToDevice,
-                                 s->stream));
-        ES_HIP(s, hipEventRecord(s->ev[2], s->stream));
-        ES_HIP(s, es::launch_sigma(sa, s->stream));
-        ES_HIP(s, es::launch_finalize(P<double>(s->dpart), P<int64_t>(s->mpart), nB, nblk, nt, sa.with_metrics,
-                                      P<double>(s->factor), P<double>(s->sig), P<double>(s->ltsum), P<int64_t>(s->mout),
-                                      s->stream));
-        ES_HIP(s, hipEventRecord(s->ev[3], s->stream));
-        ES_HIP(s, hipMemcpyAsync(sig.data(), s->sig.p, (size_t)nB * std::max(nt, 1) * 2 * sizeof(double),
-                                 hipMemcpyDeviceToHost, s->stream));
-        if (c == 0) {
-            ES_HIP(s, hipMemcpyAsync(ltsum.data(), s->ltsum.p, nB * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-            ES_HIP(s, hipMemcpyAsync(mh.data(), s->mout.p, mh.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                     s->stream));
+            for (int t = 0; t < nt; t++)
+                pl.factor[(size_t)b * es::kMaxT + t] =
+                    integrand_factor(ppr, sv->ps.phi_integrand_factor, sv->st.particle_speed, w[b], taus[t0 + t]);
+        plan_offsets(pl, c.seg_items);
+        if ((rc = stages_prepare(c, sv->s, pl))) return rc;
+        // Later temperature chunks reuse the item lifetimes of a one-segment
+        // plan; with several segments only the last one's are still resident.
+        const bool trace = (ch == 0) || pl.segs.size() > 1;
+        const bool first = (ch == 0);
+        if (trace && (rc = trace_blocks(c, pl.clockwise, sh, pl, nblocks))) return rc;
+        if ((rc = stages_enqueue(c, sv->s, pl, sh, trace, first ? 1 : 0,
+                                 (lifetimes_host && first) ? P<double>(sv->s.lifetimes) : nullptr, nblocks)))
+            return rc;
+        StageOut so;
+        if ((rc = stages_collect(c, sv->s, pl, trace, first, so))) return rc;
+        if (first) {
+            t_life = so.t_trace;
+            ltsum = so.ltsum;
+            mh = so.m;
             if (lifetimes_host)
-                ES_HIP(s, hipMemcpyAsync(lifetimes_host, s->lifetimes.p, (size_t)nB * P_ * sizeof(double),
-                                         hipMemcpyDeviceToHost, s->stream));
+                ES_HIP(c, hipMemcpy(lifetimes_host, sv->s.lifetimes.p, (size_t)nB * P_ * sizeof(double),
+                                    hipMemcpyDeviceToHost));
         }
-        ES_HIP(s, hipStreamSynchronize(s->stream));
-        float ms2 = 0;
-        ES_HIP(s, hipEventElapsedTime(&ms2, s->ev[2], s->ev[3]));
-        t_temp += ms2 * 1e-3;
+        t_temp += so.t_temp;
         for (int b = 0; b < nB; b++)
             for (int t = 0; t < nt; t++) {
-                out[(size_t)b * nT + t0 + t].xx = sig[((size_t)b * nt + t) * 2];
-                out[(size_t)b * nT + t0 + t].xy = sig[((size_t)b * nt + t) * 2 + 1];
+                out[(size_t)b * nT + t0 + t].xx = so.sig[((size_t)b * es::kMaxT + t) * 2];
+                out[(size_t)b * nT + t0 + t].xy = so.sig[((size_t)b * es::kMaxT + t) * 2 + 1];
             }
     }
-    float ms1 = 0;
-    ES_HIP(s, hipEventElapsedTime(&ms1, s->ev[0], s->ev[1]));
-    const double t_life = ms1 * 1e-3;
     if (metrics) {
-        const int64_t inside = s->n_inside_positions * ppp;
-        for (int b = 0; b < nB; b++) {
-            const int64_t* m = &mh[(size_t)b * es::kMetricSlots];
-            es_metrics& o = metrics[b];
-            o.cells_passed = m[0];
-            o.impurities_tested = m[1];
-            o.particles_escaped = m[2];
-            o.particles_at_bound = m[3];
-            o.trace_cap_hits = m[4];
-            o.max_impurities_tested = m[5];
-            o.max_cells_passed = m[6];
-            o.particles_inside_impurity = inside;
-            o.real_particles = P_ - inside;
-            o.avg_particle_lifetime = P_ ? ltsum[b] / (double)P_ : 0.0;
-            o.time_elapsed_lifetimes = t_life / nB;  // one launch for all fields: its share
-            o.time_elapsed_temperatures = t_temp / nB;
-            o.traced_particles = m[7];
-            o.edge_hits = m[8];
-            o.cross_calls = m[9];
-            o.sigma_evals = m[10];
-            o.cross_exact = m[11];
-        }
+        const int64_t inside = sv->f.n_inside * ppp;
+        for (int b = 0; b < nB; b++)
+            fill_metrics(metrics[b], &mh[(size_t)b * es::kMetricSlots], ltsum[b], P_, inside, t_life / nB, t_temp / nB);
     }
     return ES_OK;
 }
@@ -449,7 +796,7 @@ int compute(es_solver* s, const double* B, int nB, const double* T, int nT, es_s
 
 extern "C" {
 
-const char* es_version(void) { return "es-mi355x 0.1 (gfx950, FP64)"; }
+const char* es_version(void) { return "es-mi355x 0.2 (gfx950, FP64, batched)"; }
 
 int es_solver_create(const es_solver_desc* desc, es_solver** out)
 {
@@ -475,28 +822,12 @@ int es_solver_create(const es_solver_desc* desc, es_solver** out)
         g_create_error = "unsupported precision mode";
         return ES_ERR_INVALID_ARG;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        (void)hipGetLastError();
-        g_create_error = "no HIP device";
-        return ES_ERR_NO_DEVICE;
-    }
-    if (desc->device < 0 || desc->device >= ndev) {
-        g_create_error = "device ordinal out of range";
-        return ES_ERR_INVALID_ARG;
-    }
     es_solver* s = new es_solver();
     s->desc = *desc;
-    s->device = desc->device;
-    if (const char* cap = std::getenv("ES_MAX_DEVICE_BYTES")) s->byte_cap = (size_t)std::strtoull(cap, nullptr, 10);
-    hipError_t e = hipSetDevice(s->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&s->ev[i]);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
-    if (e != hipSuccess) {
-        g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
+    const int rc = ctx_init(s->c, desc->device, g_create_error);
+    if (rc != ES_OK) {
         es_solver_destroy(s);
-        return ES_ERR_HIP;
+        return rc;
     }
     *out = s;
     return ES_OK;
@@ -505,93 +836,66 @@ int es_solver_create(const es_solver_desc* desc, es_solver** out)
 void es_solver_destroy(es_solver* s)
 {
     if (!s) return;
-    (void)hipSetDevice(s->device);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    DevBuf* bufs[] = {&s->in_x, &s->in_y, &s->ixy, &s->cell_end, &s->b_targets, &s->b_ntarg, &s->b_counts,
-                      &s->b_fill, &s->b_begin, &s->b_bsum, &s->b_order, &s->flags, &s->bcount, &s->free_pos,
-                      &s->small_i32, &s->w, &s->factor, &s->dpart, &s->mpart, &s->sig, &s->ltsum, &s->mout,
-                      &s->lifetimes, &s->lt_item, &s->m_item, &s->x_item, &s->f_item, &s->queue};
-    for (DevBuf* b : bufs) release(*b);
-    for (hipEvent_t& ev : s->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    (void)hipSetDevice(s->c.device);
+    if (s->c.stream) (void)hipStreamSynchronize(s->c.stream);
+    field_release(s->c, s->f);
+    stages_release(s->c, s->s);
+    ctx_destroy(s->c);
     delete s;
 }
 
-const char* es_last_error(const es_solver* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+const char* es_last_error(const es_solver* s) { return s ? s->c.err.c_str() : g_create_error.c_str(); }
 
 int es_grid_set_impurities(es_solver* s, const double* x, const double* y, int64_t n, const es_grid_desc* desc,
                            es_grid_info* out_info)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    if (!desc || n < 0 || (n > 0 && (!x || !y))) return fail(s, ES_ERR_INVALID_ARG, "bad impurity arguments");
-    if (n > 10000000) return fail(s, ES_ERR_INVALID_ARG, "more than 1e7 impurities (Grid.cpp:36)");
-    ES_HIP(s, hipSetDevice(s->device));
-    int rc;
-    if ((rc = ensure(s, s->in_x, n * sizeof(double), "impurity input x"))) return rc;
-    if ((rc = ensure(s, s->in_y, n * sizeof(double), "impurity input y"))) return rc;
-    if (n > 0) {
-        ES_HIP(s, hipMemcpyAsync(s->in_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
-        ES_HIP(s, hipMemcpyAsync(s->in_y.p, y, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    }
-    return bin_uploaded(s, n, desc, out_info);
+    ES_HIP(s->c, hipSetDevice(s->c.device));
+    es_grid_info info{};
+    const int rc = field_upload(s->c, s->f, x, y, n, desc, &info);
+    if (rc) return rc;
+    // GridInformation of sample 0's grid when the solver was created without one
+    // (SimulationRunner.cpp:75-84 builds the Simulation from the first grid).
+    if (!(s->desc.grid_info.cell_size > 0)) s->desc.grid_info = info;
+    if (out_info) *out_info = info;
+    return ES_OK;
 }
 
 int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_desc* desc, es_grid_info* out_info)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    if (!desc) return fail(s, ES_ERR_INVALID_ARG, "null grid description");
-    ES_HIP(s, hipSetDevice(s->device));
-    const double lo = -desc->region_extends, hi = desc->region_size + desc->region_extends;
-    int64_t n = count;
-    if (n <= 0) {  // Grid.cpp:33-39
-        const double area_length = hi - lo;
-        const double proposal = area_length * area_length * desc->impurity_density;
-        if (proposal > 1e7) return fail(s, ES_ERR_INVALID_ARG, "impurity count proposal > 1e7 (Grid.cpp:36)");
-        if (!(proposal >= 1)) return fail(s, ES_ERR_INVALID_ARG, "no impurities (Grid.cpp:37)");
-        n = std::max<int64_t>(1, (int64_t)std::ceil(proposal));
-    }
-    if (n > 10000000) return fail(s, ES_ERR_INVALID_ARG, "more than 1e7 impurities (Grid.cpp:36)");
-    int rc;
-    if ((rc = ensure(s, s->in_x, n * sizeof(double), "impurity input x"))) return rc;
-    if ((rc = ensure(s, s->in_y, n * sizeof(double), "impurity input y"))) return rc;
-    ES_HIP(s, es::launch_generate(esr_key(seed), n, lo, hi, P<double>(s->in_x), P<double>(s->in_y), s->stream));
-    return bin_uploaded(s, n, desc, out_info);
+    ES_HIP(s->c, hipSetDevice(s->c.device));
+    es_grid_info info{};
+    const int rc = field_generate(s->c, s->f, seed, count, desc, &info);
+    if (rc) return rc;
+    if (!(s->desc.grid_info.cell_size > 0)) s->desc.grid_info = info;
+    if (out_info) *out_info = info;
+    return ES_OK;
 }
 
 int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    if (!s->has_grid) return fail(s, ES_ERR_NO_GRID, "no grid set");
-    ES_HIP(s, hipSetDevice(s->device));
-    const int64_t total = s->g.total, ncell = (int64_t)s->g.cpr * s->g.cpr;
-    if (ordered_xy && total > 0) {
-        ES_HIP(s, hipMemcpyAsync(ordered_xy, s->g.ixy, total * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
-        ES_HIP(s, hipStreamSynchronize(s->stream));
-    }
-    if (imp_index) {
-        ES_HIP(s, hipMemcpyAsync(imp_index, s->g.cell_end, ncell * sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
-        ES_HIP(s, hipStreamSynchronize(s->stream));
-    }
+    Ctx& c = s->c;
+    if (!s->f.has_grid) return fail(c, ES_ERR_NO_GRID, "no grid set");
+    ES_HIP(c, hipSetDevice(c.device));
+    const int64_t total = s->f.g.total, ncell = (int64_t)s->f.g.cpr * s->f.g.cpr;
+    if (ordered_xy && total > 0)
+        ES_HIP(c, hipMemcpyAsync(ordered_xy, s->f.g.ixy, total * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+    if (imp_index)
+        ES_HIP(c, hipMemcpyAsync(imp_index, s->f.g.cell_end, ncell * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    ES_HIP(c, hipStreamSynchronize(c.stream));
     return ES_OK;
 }
 
 int es_solver_init_sample(es_solver* s, const es_settings* settings, int32_t coherent)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    if (!settings) return fail(s, ES_ERR_INVALID_ARG, "null settings");
-    if (!(settings->particle_speed > 0)) return fail(s, ES_ERR_INVALID_ARG, "particle_speed must be > 0");
-    if (!(settings->alpha >= 0) || !(settings->alpha <= kPI / 4 + 1e-12))
-        return fail(s, ES_ERR_INVALID_ARG, "alpha must lie in [0, pi/4]");
+    const int rc = check_settings(s->c, settings);
+    if (rc) return rc;
     s->st = *settings;
     s->coherent = coherent ? 1 : 0;
-    // Simulation::InitSample (Simulation.h:111-126)
-    const int ppq = s->desc.particles_per_quadrant;
-    const double base_area = settings->alpha * 2.0;
-    s->integrand_angle_area = !coherent ? base_area : (kHALF_PI - base_area);
-    s->phi_integrand_factor = s->integrand_angle_area / ((ppq - 1) * 3.0);
-    s->phi_start = !coherent ? -settings->alpha : settings->alpha;
-    s->phi_step = s->integrand_angle_area / (ppq - 1);
+    s->ps = pass_settings(s->desc.particles_per_quadrant, settings->alpha, s->coherent);  // Simulation.h:111-126
     s->has_sample = true;
     return ES_OK;
 }
@@ -600,18 +904,357 @@ int es_solver_compute_sigmas(es_solver* s, double B, const double* T, int32_t nT
                              double* lifetimes)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    hipError_t e = hipSetDevice(s->device);
-    if (e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
-    return compute(s, &B, 1, T, nT, out, metrics, lifetimes);
+    ES_HIP(s->c, hipSetDevice(s->c.device));
+    return solver_compute(s, &B, 1, T, nT, out, metrics, lifetimes);
 }
 
 int es_solver_compute_sigmas_sweep(es_solver* s, const double* B, int32_t nB, const double* T, int32_t nT,
                                    es_sigma* out, es_metrics* metrics)
 {
     if (!s) return ES_ERR_INVALID_ARG;
-    hipError_t e = hipSetDevice(s->device);
-    if (e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
-    return compute(s, B, nB, T, nT, out, metrics, nullptr);
+    ES_HIP(s->c, hipSetDevice(s->c.device));
+    return solver_compute(s, B, nB, T, nT, out, metrics, nullptr);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+
+namespace {
+
+int batch_plan(es_batch* bt)
+{
+    Ctx& c = bt->c;
+    Plan& pl = bt->plan;
+    pl = Plan();
+    if (bt->entries.empty()) return fail(c, ES_ERR_INVALID_ARG, "empty batch");
+    pl.clockwise = bt->entries[0].st.is_clockwise ? 1 : 0;
+    int out = 0;
+    for (size_t e = 0; e < bt->entries.size(); e++) {
+        const es_batch::Entry& en = bt->entries[e];
+        if ((en.st.is_clockwise ? 1 : 0) != pl.clockwise)
+            return fail(c, ES_ERR_INVALID_ARG, "a batch holds one rotation sense (is_clockwise)");
+        Field& f = *bt->fields[en.field];
+        es::SampleDev sd = sample_dev(f, en.ppr, en.ppq, en.dist, en.offx, en.offy, en.ps, en.st, en.coherent);
+        sd.nT = (int)en.T.size();
+        for (int t = 0; t < es::kMaxT; t++) {
+            sd.tau[t] = t < sd.nT ? en.taus[t] : 1.0;
+            sd.max_lt[t] = 15.0 * sd.tau[t];
+        }
+        pl.samples.push_back(sd);
+        for (size_t b = 0; b < en.B.size(); b++) {
+            es::PairDev p{};
+            p.sample = (int)e;
+            p.out = out++;
+            p.w = kE * en.B[b] / kM;
+            p.nitems = (long long)f.nfree * 4 * en.ppq;
+            pl.pairs.push_back(p);
+        }
+    }
+    pl.nout = out;
+    pl.factor.assign((size_t)out * es::kMaxT, 0.0);
+    for (const es::PairDev& p : pl.pairs) {
+        const es_batch::Entry& en = bt->entries[p.sample];
+        for (size_t t = 0; t < en.T.size(); t++)
+            pl.factor[(size_t)p.out * es::kMaxT + t] =
+                integrand_factor(en.ppr, en.ps.phi_integrand_factor, en.st.particle_speed, p.w, en.taus[t]);
+    }
+    plan_offsets(pl, c.seg_items);
+    bt->plan_ready = true;
+    return ES_OK;
+}
+
+void batch_release_graph(es_batch* bt)
+{
+    if (bt->s.exec) (void)hipGraphExecDestroy(bt->s.exec);
+    if (bt->s.graph) (void)hipGraphDestroy(bt->s.graph);
+    bt->s.exec = nullptr;
+    bt->s.graph = nullptr;
+    bt->graph_ready = false;
+}
+
+// One run with a given shape; fills `so` (sigma, sums, metrics, event times) and
+// the wall time between events around the whole stage sequence.
+int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
+{
+    Ctx& c = bt->c;
+    int rc;
+    if (!bt->plan_ready) {
+        batch_release_graph(bt);
+        if ((rc = batch_plan(bt))) return rc;
+        if ((rc = stages_prepare(c, bt->s, bt->plan))) return rc;
+    }
+    std::vector<int> nblocks;
+    if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks))) return rc;
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    ES_HIP(c, hipEventCreate(&t0));
+    ES_HIP(c, hipEventCreate(&t1));
+    rc = ES_OK;
+    hipError_t e = hipEventRecord(t0, c.stream);
+    if (e != hipSuccess) rc = hip_fail(c, e, "hipEventRecord");
+    if (!rc && bt->capture) {
+        if (bt->graph_ready && bt->graph_blocks != nblocks) batch_release_graph(bt);
+        if (!bt->graph_ready) {
+            e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) rc = hip_fail(c, e, "hipStreamBeginCapture");
+            if (!rc) {
+                rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, false);
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(c.stream, &g);
+                if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
+                if (!rc) {
+                    bt->s.graph = g;
+                    e = hipGraphInstantiate(&bt->s.exec, g, nullptr, nullptr, 0);
+                    if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphInstantiate");
+                    bt->graph_ready = (rc == ES_OK);
+                    bt->graph_blocks = nblocks;
+                } else if (g) {
+                    (void)hipGraphDestroy(g);
+                }
+            }
+        }
+        if (!rc) {
+            e = hipGraphLaunch(bt->s.exec, c.stream);
+            if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphLaunch");
+        }
+    } else if (!rc) {
+        rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks);
+    }
+    if (!rc) {
+        e = hipEventRecord(t1, c.stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipEventRecord");
+    }
+    if (!rc) rc = stages_collect(c, bt->s, bt->plan, true, true, so, !bt->capture);
+    float ms = 0;
+    if (!rc) {
+        e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipEventElapsedTime");
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (wall) *wall = ms * 1e-3;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int es_batch_create(int32_t device, es_batch** out)
+{
+    if (!out) {
+        g_create_error = "null argument";
+        return ES_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    es_batch* b = new es_batch();
+    const int rc = ctx_init(b->c, device, g_create_error);
+    if (rc != ES_OK) {
+        es_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return ES_OK;
+}
+
+void es_batch_destroy(es_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->c.device);
+    if (b->c.stream) (void)hipStreamSynchronize(b->c.stream);
+    batch_release_graph(b);
+    for (Field* f : b->fields) {
+        field_release(b->c, *f);
+        delete f;
+    }
+    stages_release(b->c, b->s);
+    ctx_destroy(b->c);
+    delete b;
+}
+
+const char* es_batch_last_error(const es_batch* b) { return b ? b->c.err.c_str() : g_create_error.c_str(); }
+
+int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    Ctx& c = bt->c;
+    ES_HIP(c, hipSetDevice(c.device));
+    if (!smp) return fail(c, ES_ERR_INVALID_ARG, "null sample");
+    if (smp->positions_per_row < 2 || smp->particles_per_quadrant < 2)
+        return fail(c, ES_ERR_INVALID_ARG, "positions_per_row and particles_per_quadrant must be >= 2");
+    if (smp->n_fields <= 0 || !smp->fields) return fail(c, ES_ERR_INVALID_ARG, "need at least one field");
+    if (smp->n_temperatures < 0 || smp->n_temperatures > es::kMaxT || (smp->n_temperatures > 0 && !smp->temperatures))
+        return fail(c, ES_ERR_INVALID_ARG, "a batch sample takes 0 to 8 temperatures");
+    for (int b = 0; b < smp->n_fields; b++)
+        if (!(smp->fields[b] > 0) || !std::isfinite(smp->fields[b]))
+            return fail(c, ES_ERR_INVALID_ARG, "magnetic field must be > 0");
+    int rc = check_settings(c, &smp->settings);
+    if (rc) return rc;
+    if (!(smp->grid.region_size > 0)) return fail(c, ES_ERR_INVALID_ARG, "region_size must be > 0");
+    es_batch::Entry en;
+    en.ppr = smp->positions_per_row;
+    en.ppq = smp->particles_per_quadrant;
+    if (smp->share_field_of >= 0) {
+        if (smp->share_field_of >= (int)bt->entries.size())
+            return fail(c, ES_ERR_INVALID_ARG, "share_field_of names no earlier sample");
+        en.field = bt->entries[smp->share_field_of].field;
+    } else {
+        Field* f = new Field();
+        es_grid_info info{};
+        rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, &info);
+        if (rc) {
+            field_release(c, *f);
+            delete f;
+            return rc;
+        }
+        bt->fields.push_back(f);
+        en.field = (int)bt->fields.size() - 1;
+    }
+    Field& f = *bt->fields[en.field];
+    const double cell = smp->grid_cell_size > 0 ? smp->grid_cell_size : f.g.cell_size;
+    en.dist = smp->grid.region_size / (double)(en.ppr - 1);  // Simulation.h:142-143
+    en.offx = cell * 0.01;
+    en.offy = cell * 0.005;
+    for (const es_batch::Entry& other : bt->entries)
+        if (other.field == en.field && (other.ppr != en.ppr || other.dist != en.dist || other.offx != en.offx))
+            return fail(c, ES_ERR_INVALID_ARG, "samples sharing a field must share its positions");
+    en.st = smp->settings;
+    en.coherent = smp->coherent ? 1 : 0;
+    en.ps = pass_settings(en.ppq, en.st.alpha, en.coherent);
+    en.B.assign(smp->fields, smp->fields + smp->n_fields);
+    en.T.assign(smp->temperatures, smp->temperatures + smp->n_temperatures);
+    if ((rc = taus_for(c, en.st, en.coherent, en.T.data(), (int)en.T.size(), en.taus))) return rc;
+    if ((rc = field_positions(c, f, en.ppr, en.dist, en.offx, en.offy))) return rc;
+    bt->entries.push_back(en);
+    bt->plan_ready = false;
+    if (index) *index = (int32_t)bt->entries.size() - 1;
+    return ES_OK;
+}
+
+int es_batch_set_shape(es_batch* bt, const es_batch_shape* shape)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    if (!shape || (shape->waves_per_eu != 2 && shape->waves_per_eu != 3) || shape->chunk < 1 || shape->chunk > 4096 ||
+        shape->grid_mult < 1 || shape->grid_mult > 8)
+        return fail(bt->c, ES_ERR_INVALID_ARG, "shape: waves_per_eu 2 or 3, chunk 1..4096, grid_mult 1..8");
+    bt->shape.waves_per_eu = shape->waves_per_eu;
+    bt->shape.chunk = shape->chunk;
+    bt->shape.grid_mult = shape->grid_mult;
+    batch_release_graph(bt);
+    return ES_OK;
+}
+
+int es_batch_get_shape(es_batch* bt, es_batch_shape* shape)
+{
+    if (!bt || !shape) return ES_ERR_INVALID_ARG;
+    shape->waves_per_eu = bt->shape.waves_per_eu;
+    shape->chunk = bt->shape.chunk;
+    shape->grid_mult = bt->shape.grid_mult;
+    return ES_OK;
+}
+
+int es_batch_set_capture(es_batch* bt, int32_t enable)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    bt->capture = enable != 0;
+    batch_release_graph(bt);
+    return ES_OK;
+}
+
+int es_batch_run(es_batch* bt, es_sigma* out, es_metrics* metrics, double* seconds)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    Ctx& c = bt->c;
+    ES_HIP(c, hipSetDevice(c.device));
+    StageOut so;
+    double wall = 0;
+    const int rc = batch_run_once(bt, bt->shape, so, &wall);
+    if (rc) return rc;
+    if (seconds) *seconds = wall;
+    const Plan& pl = bt->plan;
+    size_t o = 0;
+    for (const es::PairDev& p : pl.pairs) {
+        const es_batch::Entry& en = bt->entries[p.sample];
+        const int nT = (int)en.T.size();
+        for (int t = 0; t < nT && out; t++) {
+            out[o + t].xx = so.sig[((size_t)p.out * es::kMaxT + t) * 2];
+            out[o + t].xy = so.sig[((size_t)p.out * es::kMaxT + t) * 2 + 1];
+        }
+        o += nT;
+        if (metrics) {
+            const Field& f = *bt->fields[en.field];
+            const int64_t P_ = (int64_t)en.ppr * en.ppr * 4 * en.ppq;
+            fill_metrics(metrics[p.out], &so.m[(size_t)p.out * es::kMetricSlots], so.ltsum[p.out], P_,
+                         f.n_inside * 4 * en.ppq, so.t_trace / pl.nout, so.t_temp / pl.nout);
+        }
+    }
+    return ES_OK;
+}
+
+int es_batch_counts(es_batch* bt, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    if (!bt->plan_ready) {
+        batch_release_graph(bt);
+        const int rc = batch_plan(bt);
+        if (rc) return rc;
+        const int rc2 = stages_prepare(bt->c, bt->s, bt->plan);
+        if (rc2) return rc2;
+    }
+    if (n_samples) *n_samples = (int32_t)bt->entries.size();
+    if (n_pairs) *n_pairs = (int32_t)bt->plan.pairs.size();
+    int32_t nout = 0;
+    for (const es_batch::Entry& en : bt->entries) nout += (int32_t)(en.B.size() * en.T.size());
+    if (n_outputs) *n_outputs = nout;
+    if (traced_items) *traced_items = bt->plan.total_items;
+    return ES_OK;
+}
+
+int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    Ctx& c = bt->c;
+    ES_HIP(c, hipSetDevice(c.device));
+    const bool cap = bt->capture;
+    bt->capture = false;  // the candidates are timed on direct launches
+    Shape best_sh = bt->shape;
+    double best_t = 1e300;
+    const int waves[] = {2, 3};
+    const int chunks[] = {32, 64, 128, 256, 512};
+    const int mults[] = {1, 2};
+    StageOut so;
+    double t = 0;
+    int rc = batch_run_once(bt, bt->shape, so, &t);  // warm-up (plan, allocation, code load)
+    for (int w : waves)
+        for (int ch : chunks)
+            for (int m : mults) {
+                if (rc) break;
+                Shape sh;
+                sh.waves_per_eu = w;
+                sh.chunk = ch;
+                sh.grid_mult = m;
+                double tt = 1e300;
+                for (int rep = 0; rep < 2 && !rc; rep++) {
+                    rc = batch_run_once(bt, sh, so, &t);
+                    tt = std::min(tt, so.t_trace);
+                }
+                if (!rc && tt < best_t) {
+                    best_t = tt;
+                    best_sh = sh;
+                }
+            }
+    bt->capture = cap;
+    if (rc) return rc;
+    bt->shape = best_sh;
+    batch_release_graph(bt);
+    if (best) {
+        best->waves_per_eu = best_sh.waves_per_eu;
+        best->chunk = best_sh.chunk;
+        best->grid_mult = best_sh.grid_mult;
+    }
+    if (best_seconds) *best_seconds = best_t;
+    return ES_OK;
 }
 
 }  // extern "C"

@@ -1,4 +1,8 @@
 // es_kernels.h — launchers of the gfx950 kernels (es_kernels.hip).
+//
+// The lifetime and temperature stages work on a table of "pairs": one pair is
+// one (sample, pass, field).  A single ComputeSigmas call is one sample with
+// one pair per field; a batch is many samples of uneven size in one launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,7 +11,7 @@
 namespace es {
 
 constexpr int kBlock = 256;        // threads per block (4 wave64)
-constexpr int kMaxT = 8;           // temperatures per sigma launch
+constexpr int kMaxT = 8;           // temperatures per sample
 // int64 metric partials per block: 0 cells, 1 tested, 2 escaped, 3 at_bound,
 // 4 cap hits, 5 max tested (max), 6 max cells (max), 7 traced particles,
 // 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time solves.
@@ -22,51 +26,62 @@ struct GridDev {
     int cpr = 0;
 };
 
-// Lifetime stage: one work item per (field, free position, phi).
-struct TraceArgs {
+// One sample and pass: its field, positions and particle settings.
+struct SampleDev {
     const double2* ixy;
     const int32_t* cell_end;
-    double g_start, g_size, g_L, g_radius;
-    int g_cpr;
-    const int32_t* free_pos;       // position ids j*ppr + i outside every impurity
+    double start, size, L, radius;   // ImpuritySettings (settings.h:36-45)
+    int cpr;
+    const int32_t* free_pos;         // position ids j*ppr + i outside every impurity
     int nfree;
     int ppr, ppq;
-    double dist, offx, offy;
+    double dist, offx, offy;         // distance_between_positions, small_offset
     double phi_start, phi_step, alpha, speed;
     int coherent;
-    int clockwise;
-    int nB;
-    const double* w;               // [nB] angular speed E*B/M
-    unsigned long long* queue;     // work counter, 0 at launch
-    int chunk;                     // items taken from the queue at a time per wave
-    // outputs per item g = b * nitems + k
-    double* lt_item;               // raw lifetime
-    int4* m_item;                  // cells, tested, edge hits, cross calls
-    int32_t* x_item;               // exact cross-time solves
-    uint8_t* f_item;               // ES_F_* flags
-    double* lifetimes;             // [nB][P] reference layout, or null
+    int nT;
+    double tau[kMaxT];
+    double max_lt[kMaxT];            // 15 * tau (Simulation.h:91)
 };
 
-// Temperature stage per field: truncation, GetSigma, Simpson weights, sums.
+// One (sample, pass, field).
+struct PairDev {
+    int sample;
+    int out;                         // this pair's row in the outputs
+    double w;                        // angular speed E*B/M
+    long long item_off;              // first work item (ascending over pairs)
+    long long nitems;                // nfree * 4 * ppq
+    int blk_off;                     // first temperature-stage block (ascending)
+    int nblk;
+};
+
+struct TraceArgs {
+    const SampleDev* samples;
+    const PairDev* pairs;
+    int npairs;
+    long long total_items;
+    unsigned long long* queue;       // work counter, reset by the launcher
+    int chunk;                       // items a wave takes from the queue at a time
+    double* lt_item;                 // raw lifetime per item
+    int4* m_item;                    // cells, tested, edge hits, cross calls
+    int32_t* x_item;                 // exact cross-time solves
+    uint8_t* f_item;                 // ES_F_* flags
+    double* lifetimes;               // [pair][P] reference layout, or null
+    long long P;                     // ppr^2 * 4 * ppq when lifetimes is set
+};
+
 struct SigmaArgs {
+    const SampleDev* samples;
+    const PairDev* pairs;
+    int npairs;
+    int total_blocks;
+    int clockwise;
     const double* lt_item;
     const int4* m_item;
     const int32_t* x_item;
     const uint8_t* f_item;
-    const int32_t* free_pos;
-    int nfree;
-    int ppr, ppq;
-    double phi_start, phi_step;
-    int clockwise;
-    int nB;
-    const double* w;
-    int nT;
-    double tau[kMaxT];
-    double max_lt[kMaxT];          // 15 * tau
-    int with_metrics;              // reduce the per-item work counters too
-    double* dpart;                 // [nB][nblk][2*nT+1][2] double-double partials
-    int64_t* mpart;                // [nB][nblk][kMetricSlots]
-    int nblk;
+    int with_metrics;
+    double* dpart;                   // [block][2*NT+1][2] double-double partials
+    int64_t* mpart;                  // [block][kMetricSlots]
 };
 
 hipError_t launch_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y, hipStream_t st);
@@ -92,14 +107,15 @@ hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev
 hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
                             int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st);
 
-hipError_t launch_trace(const TraceArgs& a, int nblocks, hipStream_t st);
-// Resident blocks per CU of the trace kernel (for sizing its persistent grid).
-hipError_t trace_blocks_per_cu(int clockwise, int* blocks);
-hipError_t launch_sigma(const SigmaArgs& a, hipStream_t st);
+// Trace kernel variants (register budget 2 or 3 waves per SIMD).
+hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int* blocks);
+// Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
+hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks, hipStream_t st);
+hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st);
 
-// Sum the per-block partials: out_sig [nB][nT][2], out_lt_sum [nB], out_m [nB][kMetricSlots].
-hipError_t launch_finalize(const double* dpart, const int64_t* mpart, int nB, int nblk, int nT, int with_metrics,
-                           const double* factor /* [nB][nT] device */, double* out_sig, double* out_lt_sum,
-                           int64_t* out_m, hipStream_t st);
+// One block per pair: out_sig [out][kMaxT][2], out_lt_sum [out], out_m [out][kMetricSlots].
+hipError_t launch_finalize(const SampleDev* samples, const PairDev* pairs, int npairs, int NT, int with_metrics,
+                           const double* dpart, const int64_t* mpart, const double* factor /* [out][kMaxT] */,
+                           double* out_sig, double* out_lt_sum, int64_t* out_m, hipStream_t st);
 
 }  // namespace es

@@ -12,7 +12,7 @@
 //   k_positions     start cell + "inside an impurity" test per position
 //                   (lifetime.h:84-99), hoisted out of the phi and B loops
 //   k_compact       ordered list of free positions (deterministic)
-//   k_trace<CW>     CreateParticle + TraceOrbit (lifetime.h:36-148) per work
+//   k_trace_w*<CW>  CreateParticle + TraceOrbit (lifetime.h:36-148) per work
 //                   item; a lane whose trace ends takes the next item of its
 //                   wave's queue chunk, so long and short traces share waves;
 //                   per-item raw lifetime + work counters
@@ -374,37 +374,43 @@ __global__ __launch_bounds__(256) void k_compact(const uint8_t* flags, int64_t n
 
 // --------------------------------------------------------------------- trace
 
-#ifndef ES_TRACE_WAVES_PER_EU
-#define ES_TRACE_WAVES_PER_EU 3
-#endif
+// Pair owning work item g (pairs are sorted by item_off).
+__device__ __forceinline__ int pair_of_item(const PairDev* pairs, int npairs, long long g)
+{
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].item_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Pair owning temperature-stage block k (pairs are sorted by blk_off).
+__device__ __forceinline__ int pair_of_block(const PairDev* pairs, int npairs, int k)
+{
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].blk_off <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 template <bool CW>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ES_TRACE_WAVES_PER_EU)))
-void k_trace(TraceArgs a)
+__device__ __forceinline__ void trace_body(const TraceArgs& a)
 {
-    const int ppq = a.ppq;
-    const int ppp = 4 * ppq;
-    const int64_t nitems = (int64_t)a.nfree * ppp;
-    const int64_t total = nitems * a.nB;
-    const int64_t P = (int64_t)a.ppr * a.ppr * ppp;
-
-    DevGrid g;
-    g.ixy = a.ixy;
-    g.cell_end = a.cell_end;
-    g.start = a.g_start;
-    g.size = a.g_size;
-    g.L = a.g_L;
-    g.radius = a.g_radius;
-    g.cpr = a.g_cpr;
-    const int cap = 4 * g.cpr * g.cpr + 16;
-
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const long long total = a.total_items;
 
-    int64_t q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
+    long long q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
     bool exhausted = false;
     bool active = false;
-    int64_t item = 0, out_idx = 0;
+    long long item = 0, out_idx = -1;
+    int cap = 0;
+    DevGrid g;
     Orb o;
     double phi0 = 0.0, bound_time = 0.0;
     TraceState st;
@@ -417,39 +423,50 @@ void k_trace(TraceArgs a)
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
                 base = __shfl(base, 0, 64);
-                if ((int64_t)base >= total) {
+                if ((long long)base >= total) {
                     exhausted = true;
                     break;
                 }
-                q_next = (int64_t)base;
+                q_next = (long long)base;
                 q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
             }
-            const int64_t left = q_end - q_next;
+            const long long left = q_end - q_next;
             const int n = __popcll(need);
-            const int take = (int)((int64_t)n < left ? (int64_t)n : left);
+            const int take = (int)((long long)n < left ? (long long)n : left);
             if (!active) {
                 const int rank = __popcll(need & below);
                 if (rank < take) {
                     item = q_next + rank;
-                    const int b = (int)(item / nitems);
-                    const int64_t k = item - (int64_t)b * nitems;
-                    const int64_t slot = k / ppp;
+                    const int pi = pair_of_item(a.pairs, a.npairs, item);
+                    const PairDev& pr = a.pairs[pi];
+                    const SampleDev& sd = a.samples[pr.sample];
+                    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
+                    const long long k = item - pr.item_off;
+                    const long long slot = k / ppp;
                     const int v = (int)(k - slot * ppp);
                     const int q = v / ppq, p = v - (v / ppq) * ppq;
-                    const int pidx = a.free_pos[slot];
-                    const int j = pidx / a.ppr, i = pidx - (pidx / a.ppr) * a.ppr;
-                    out_idx = (int64_t)b * P + (int64_t)pidx * ppp + v;
-                    const double px = (double)i * a.dist + a.offx;  // SimulationCPU.cpp:114
-                    const double py = (double)j * a.dist + a.offy;
-                    const double w = a.w[b];
+                    const int pidx = sd.free_pos[slot];
+                    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+                    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+                    g.ixy = sd.ixy;
+                    g.cell_end = sd.cell_end;
+                    g.start = sd.start;
+                    g.size = sd.size;
+                    g.L = sd.L;
+                    g.radius = sd.radius;
+                    g.cpr = sd.cpr;
+                    cap = 4 * sd.cpr * sd.cpr + 16;
+                    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
+                    const double py = (double)j * sd.dist + sd.offy;
+                    const double w = pr.w;
                     // CreateParticle (lifetime.h:36-58)
-                    const double phi = a.phi_start + q * ES_HALF_PI + p * a.phi_step;
+                    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
                     double sphi, cphi;
                     esm_sincos(phi, &sphi, &cphi);
-                    const double vx = cphi * a.speed, vy = sphi * a.speed;
-                    const double R = a.speed / w;
-                    const double sx = vy * R / a.speed;
-                    const double sy = -vx * R / a.speed;
+                    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
+                    const double R = sd.speed / w;
+                    const double sx = vy * R / sd.speed;
+                    const double sy = -vx * R / sd.speed;
                     const double radd = R + g.radius, rmin = R - g.radius;
                     o.cx = CW ? (px - sx) : (px + sx);
                     o.cy = CW ? (py - sy) : (py + sy);
@@ -466,9 +483,9 @@ void k_trace(TraceArgs a)
                     o.r = (float)g.radius;
                     phi0 = phi;
                     bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
-                    if (!a.coherent) {
-                        const double remaining = d_smod(phi + a.alpha, ES_HALF_PI);
-                        const double dphi = CW ? remaining : (2.0 * a.alpha - remaining);
+                    if (!sd.coherent) {
+                        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
+                        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
                         bound_time = dphi / w;
                     }
                     const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
@@ -494,11 +511,24 @@ void k_trace(TraceArgs a)
                 a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
                 a.x_item[item] = st.cross_exact;
                 a.f_item[item] = (uint8_t)flags;
-                if (a.lifetimes) a.lifetimes[out_idx] = lt;
+                if (out_idx >= 0) a.lifetimes[out_idx] = lt;
                 active = false;
             }
         }
     }
+}
+
+// One kernel per rotation sense and register budget (the tuner picks one).
+template <bool CW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_w3(TraceArgs a)
+{
+    trace_body<CW>(a);
+}
+
+template <bool CW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_w2(TraceArgs a)
+{
+    trace_body<CW>(a);
 }
 
 // --------------------------------------------------------------------- sigma
@@ -507,12 +537,15 @@ template <int NT>
 __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 {
     constexpr int NQ = 2 * NT + 1;
-    const int b = blockIdx.y;
-    const int ppq = a.ppq;
+    const int pi = pair_of_block(a.pairs, a.npairs, blockIdx.x);
+    const PairDev& pr = a.pairs[pi];
+    const SampleDev& sd = a.samples[pr.sample];
+    const int ppq = sd.ppq;
     const int ppp = 4 * ppq;
-    const int64_t nitems = (int64_t)a.nfree * ppp;
-    const double w = a.w[b];
+    const long long nitems = pr.nitems;
+    const double w = pr.w;
     const double ws = a.clockwise ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
+    const int nT = sd.nT;
 
     DD acc[NQ];
 #pragma unroll
@@ -520,15 +553,15 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
     int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0, m_edge = 0, m_cross = 0, m_sigma = 0;
     int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0;
 
-    const int64_t stride = (int64_t)a.nblk * kBlock;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nitems; k += stride) {
-        const int64_t gi = (int64_t)b * nitems + k;
+    const long long stride = (long long)pr.nblk * kBlock;
+    for (long long k = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x; k < nitems; k += stride) {
+        const long long gi = pr.item_off + k;
         const double lt_raw = a.lt_item[gi];
-        const int64_t slot = k / ppp;
+        const long long slot = k / ppp;
         const int v = (int)(k - slot * ppp);
         const int q = v / ppq, p = v - (v / ppq) * ppq;
-        const int pidx = a.free_pos[slot];
-        const int j = pidx / a.ppr, i = pidx - (pidx / a.ppr) * a.ppr;
+        const int pidx = sd.free_pos[slot];
+        const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
         if (a.with_metrics) {
             const int4 m = a.m_item[gi];
             const int f = a.f_item[gi];
@@ -548,17 +581,17 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 
         // ApplySigma + IntegrateResult terms (SimulationCPU.cpp:178-222): a zero
         // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
-        const double phi = a.phi_start + q * ES_HALF_PI + p * a.phi_step;
+        const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
         double sphi, cphi;
         esm_sincos(phi, &sphi, &cphi);
-        const double wgt = simpson_weight(j, a.ppr) * simpson_weight(i, a.ppr) * simpson_weight(p, ppq);
+        const double wgt = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr) * simpson_weight(p, ppq);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            if (t < a.nT) {
-                const double ml = a.max_lt[t];
+            if (t < nT) {
+                const double ml = sd.max_lt[t];
                 const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
                 if (lt != 0.0) {
-                    const double tau = a.tau[t];
+                    const double tau = sd.tau[t];
                     const double z = esm_exp(-lt / tau);
                     double sa, ca;
                     esm_sincos(phi + ws * lt, &sa, &ca);
@@ -594,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
         if (lane == 0) sh_m[s][wave] = r;
     }
     __syncthreads();
-    const int64_t base = (int64_t)b * a.nblk + blockIdx.x;
+    const long long base = blockIdx.x;
     if (threadIdx.x < NQ) {
         const int q = threadIdx.x;
         DD r;
@@ -606,15 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
             o.c = sh_c[q][wv];
             r = dd_add(r, o);
         }
-        // Partials are stored with the launch's stride 2 * nT + 1.
-        const int nq = 2 * a.nT + 1;
-        int dst = -1;
-        if (q < 2 * a.nT) dst = q;
-        else if (q == 2 * NT) dst = 2 * a.nT;
-        if (dst >= 0) {
-            a.dpart[(base * nq + dst) * 2] = r.s;
-            a.dpart[(base * nq + dst) * 2 + 1] = r.c;
-        }
+        a.dpart[(base * NQ + q) * 2] = r.s;
+        a.dpart[(base * NQ + q) * 2 + 1] = r.c;
     } else if (threadIdx.x >= 64 && threadIdx.x < 64 + kMetricSlots) {
         const int s = threadIdx.x - 64;
         int64_t r = sh_m[s][0];
@@ -630,23 +656,29 @@ __global__ void k_reset(unsigned long long* q) { *q = 0ull; }
 
 // ------------------------------------------------------------------ finalize
 
-__global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int64_t* mpart, int nblk, int nT,
-                                                  int with_metrics, const double* factor, double* out_sig,
-                                                  double* out_lt_sum, int64_t* out_m)
+__global__ __launch_bounds__(256) void k_finalize(const SampleDev* samples, const PairDev* pairs, int NT,
+                                                  int with_metrics, const double* dpart, const int64_t* mpart,
+                                                  const double* factor, double* out_sig, double* out_lt_sum,
+                                                  int64_t* out_m)
 {
-    const int b = blockIdx.x;
-    const int nq = 2 * nT + 1;
+    const PairDev& pr = pairs[blockIdx.x];
+    const int nT = samples[pr.sample].nT;
+    const int NQ = 2 * NT + 1;
+    const long long o = pr.out;
     __shared__ double sh_s[4], sh_c[4];
     __shared__ int64_t sh_m[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = 0; q < nq; q++) {
+    for (int q = 0; q < NQ; q++) {
+        const bool used = (q < 2 * nT) || (q == 2 * NT);
+        if (!used) continue;  // uniform across the block
         DD acc;
         acc.s = acc.c = 0.0;
-        for (int blk = threadIdx.x; blk < nblk; blk += 256) {
-            DD o;
-            o.s = dpart[(((int64_t)b * nblk + blk) * nq + q) * 2];
-            o.c = dpart[(((int64_t)b * nblk + blk) * nq + q) * 2 + 1];
-            acc = dd_add(acc, o);
+        for (int blk = threadIdx.x; blk < pr.nblk; blk += 256) {
+            const long long b = (long long)pr.blk_off + blk;
+            DD v;
+            v.s = dpart[(b * NQ + q) * 2];
+            v.c = dpart[(b * NQ + q) * 2 + 1];
+            acc = dd_add(acc, v);
         }
         acc = dd_wave_reduce(acc);
         if (lane == 0) {
@@ -659,17 +691,17 @@ __global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int
             r.s = sh_s[0];
             r.c = sh_c[0];
             for (int wv = 1; wv < 4; wv++) {
-                DD o;
-                o.s = sh_s[wv];
-                o.c = sh_c[wv];
-                r = dd_add(r, o);
+                DD v;
+                v.s = sh_s[wv];
+                v.c = sh_c[wv];
+                r = dd_add(r, v);
             }
             const double total = r.s + r.c;
-            if (q < 2 * nT) {
+            if (q < 2 * NT) {
                 const int t = q >> 1;
-                out_sig[((int64_t)b * nT + t) * 2 + (q & 1)] = total * factor[(int64_t)b * nT + t];
+                out_sig[(o * kMaxT + t) * 2 + (q & 1)] = total * factor[o * kMaxT + t];
             } else {
-                out_lt_sum[b] = total;
+                out_lt_sum[o] = total;
             }
         }
         __syncthreads();
@@ -678,9 +710,9 @@ __global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int
     for (int s = 0; s < kMetricSlots; s++) {
         const bool is_max = (s == 5 || s == 6);
         int64_t acc = 0;
-        for (int blk = threadIdx.x; blk < nblk; blk += 256) {
-            const int64_t o = mpart[((int64_t)b * nblk + blk) * kMetricSlots + s];
-            acc = is_max ? (o > acc ? o : acc) : acc + o;
+        for (int blk = threadIdx.x; blk < pr.nblk; blk += 256) {
+            const int64_t v = mpart[((long long)pr.blk_off + blk) * kMetricSlots + s];
+            acc = is_max ? (v > acc ? v : acc) : acc + v;
         }
         acc = is_max ? i64_wave_max(acc) : i64_wave_sum(acc);
         if (lane == 0) sh_m[wave] = acc;
@@ -688,7 +720,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double* dpart, const int
         if (threadIdx.x == 0) {
             int64_t r = sh_m[0];
             for (int wv = 1; wv < 4; wv++) r = is_max ? (sh_m[wv] > r ? sh_m[wv] : r) : r + sh_m[wv];
-            out_m[(int64_t)b * kMetricSlots + s] = r;
+            out_m[o * kMetricSlots + s] = r;
         }
         __syncthreads();
     }
@@ -763,47 +795,60 @@ hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx,
     return hipGetLastError();
 }
 
-hipError_t launch_trace(const TraceArgs& a, int nblocks, hipStream_t st)
+hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int* blocks)
 {
-    if (a.nB <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (a.clockwise)
-        hipLaunchKernelGGL((k_trace<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_trace<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
-    return hipGetLastError();
+    if (waves_per_eu == 2)
+        return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w2<true>, kBlock, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w2<false>, kBlock, 0);
+    return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w3<true>, kBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w3<false>, kBlock, 0);
 }
 
-hipError_t trace_blocks_per_cu(int clockwise, int* blocks)
+hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks, hipStream_t st)
 {
-    if (clockwise)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace<true>, kBlock, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace<false>, kBlock, 0);
+    if (a.npairs <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.total_items <= 0) return e;
+    if (waves_per_eu == 2) {
+        if (clockwise)
+            hipLaunchKernelGGL((k_trace_w2<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_trace_w2<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
+    } else {
+        if (clockwise)
+            hipLaunchKernelGGL((k_trace_w3<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_trace_w3<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
+    }
+    return hipGetLastError();
 }
 
 template <int NT>
 static hipError_t launch_sigma_t(const SigmaArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_sigma<NT>), dim3(a.nblk, a.nB), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_sigma<NT>), dim3(a.total_blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_sigma(const SigmaArgs& a, hipStream_t st)
+hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st)
 {
-    if (a.nT < 0 || a.nT > kMaxT || a.nB <= 0 || a.nblk <= 0) return hipErrorInvalidValue;
-    if (a.nT <= 1) return launch_sigma_t<1>(a, st);
-    if (a.nT <= 2) return launch_sigma_t<2>(a, st);
-    if (a.nT <= 4) return launch_sigma_t<4>(a, st);
-    return launch_sigma_t<8>(a, st);
+    if (a.npairs <= 0 || a.total_blocks <= 0) return hipErrorInvalidValue;
+    switch (NT) {
+    case 1: return launch_sigma_t<1>(a, st);
+    case 2: return launch_sigma_t<2>(a, st);
+    case 4: return launch_sigma_t<4>(a, st);
+    case 8: return launch_sigma_t<8>(a, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
-hipError_t launch_finalize(const double* dpart, const int64_t* mpart, int nB, int nblk, int nT, int with_metrics,
-                           const double* factor, double* out_sig, double* out_lt_sum, int64_t* out_m, hipStream_t st)
+hipError_t launch_finalize(const SampleDev* samples, const PairDev* pairs, int npairs, int NT, int with_metrics,
+                           const double* dpart, const int64_t* mpart, const double* factor, double* out_sig,
+                           double* out_lt_sum, int64_t* out_m, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(nB), dim3(256), 0, st, dpart, mpart, nblk, nT, with_metrics, factor, out_sig,
-                       out_lt_sum, out_m);
+    hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(256), 0, st, samples, pairs, NT, with_metrics, dpart, mpart,
+                       factor, out_sig, out_lt_sum, out_m);
     return hipGetLastError();
 }
 

@@ -37,7 +37,52 @@ PRODUCTION_FP64 = dict(
     tau=1e-12, particle_speed=7e5, clockwise=1, seeds=[(1 + 100) * (i + 1) for i in range(8)],
 )
 
-WORKLOADS = {w["name"]: w for w in (SMALL_FP64, PRODUCTION_FP64)}
+# Batched uneven workload: 64 samples on a 2.0e-6 m side, density log-uniform
+# over 1e14-1.5e16 m^-2 (400-60,000 disks, count = ceil(side^2 * density),
+# Grid.cpp:35-39), radius uniform over 0.5e-8-2e-8 m, positions per row drawn
+# from {32, 64, 128, 256, 512}, 65 Simpson points per window, cells from the
+# reference rule with a target population of 4, 40 fields log-spaced over
+# 1-40 T and one temperature uniform over 0-150 K (floored at 1e-3 K) per
+# sample.  One incoherent pass per sample: with alpha = pi/4 the coherent
+# window is empty (Simulation.h:115-117) and its sigma is exactly zero.
+BATCHED_UNEVEN = dict(
+    name="batched_uneven_64samples_400-60000imp_32-512pos_65phi_40B",
+    side=2.0e-6, n_samples=64, density=(1e14, 1.5e16), radius=(0.5e-8, 2e-8),
+    positions_per_row=(32, 64, 128, 256, 512), particles_per_quadrant=65, target_cell_population=4,
+    alpha=math.pi / 4, fields=list(np.logspace(0.0, math.log10(40.0), 40)), temperature=(0.0, 150.0),
+    tau=1e-12, particle_speed=7e5, clockwise=1, draw_seed=2024, base_seed=3,
+)
+
+WORKLOADS = {w["name"]: w for w in (SMALL_FP64, PRODUCTION_FP64, BATCHED_UNEVEN)}
+
+
+def batched_samples(w=BATCHED_UNEVEN, n_samples=None, max_ppr=None):
+    """Per-sample keyword arguments of :meth:`elasticscattering_amd.Batch.add`.
+
+    The draws come from numpy's PCG64 seeded with ``draw_seed``; the field
+    seeds follow SimulationRunner.cpp:93 ((seed + 100) * (i + 1)).
+    ``max_ppr`` caps the drawn position grid (for smaller test batches).
+    """
+    rng = np.random.default_rng(w["draw_seed"])
+    n = w["n_samples"] if n_samples is None else n_samples
+    rs, ext = region(w)
+    lo_d, hi_d = (math.log(v) for v in w["density"])
+    out = []
+    for i in range(n):
+        density = math.exp(rng.uniform(lo_d, hi_d))
+        radius = rng.uniform(*w["radius"])
+        ppr = int(rng.choice(w["positions_per_row"]))
+        T = max(rng.uniform(*w["temperature"]), 1e-3)
+        if max_ppr is not None:
+            ppr = min(ppr, max_ppr)
+        out.append(dict(
+            seed=(w["base_seed"] + 100) * (i + 1), region_size=rs, region_extends=ext, impurity_radius=radius,
+            density=density, target_cell_population=w["target_cell_population"], positions_per_row=ppr,
+            particles_per_quadrant=w["particles_per_quadrant"], fields=list(w["fields"]), temperatures=[T],
+            tau=w["tau"], alpha=w["alpha"], is_clockwise=w["clockwise"], particle_speed=w["particle_speed"],
+            coherent=False,
+        ))
+    return out
 
 
 def region(w):

@@ -158,6 +158,61 @@ int es_solver_compute_sigmas(es_solver* s, double B, const double* T, int32_t nT
 int es_solver_compute_sigmas_sweep(es_solver* s, const double* B, int32_t nB, const double* T, int32_t nT,
                                    es_sigma* out, es_metrics* metrics);
 
+/* ---------------------------------------------------------------- batches
+ *
+ * Many samples of uneven size (own field, cell grid, positions, fields B and
+ * temperatures) traced in one launch of each stage.  No reference counterpart
+ * as one call: it is the sample loop of SimulationRunner::RunSimulation
+ * (SimulationRunner.cpp:88-98) and the B loop of RunSample (:129-145) run as
+ * one piece of work on the GPU.  Each sample entry is one pass (coherent or
+ * incoherent) of one field; two passes of one field share it through
+ * share_field_of.  Every entry of one batch has the same is_clockwise.
+ */
+typedef struct es_batch es_batch;
+
+typedef struct es_batch_sample {
+    uint64_t seed;                    /* field drawn with es_rng.h (es_grid_generate) */
+    int64_t impurity_count;           /* <= 0: ceil(side^2 * density) (Grid.cpp:35-39) */
+    es_grid_desc grid;
+    double grid_cell_size;            /* GridInformation.cell_size for the position offset; 0: this field's */
+    int32_t positions_per_row;
+    int32_t particles_per_quadrant;
+    es_settings settings;
+    int32_t coherent;
+    int32_t share_field_of;           /* >= 0: use the field of that earlier entry instead of drawing one */
+    const double* fields;             /* n_fields values of B (copied) */
+    int32_t n_fields;
+    const double* temperatures;       /* 0..8 temperatures (copied) */
+    int32_t n_temperatures;
+} es_batch_sample;
+
+/* Launch shape of the trace kernel (what es_batch_tune chooses). */
+typedef struct es_batch_shape {
+    int32_t waves_per_eu;             /* register budget: 2 or 3 waves per SIMD */
+    int32_t chunk;                    /* work items a wave takes from the queue at a time */
+    int32_t grid_mult;                /* resident blocks per CU times this = grid size */
+} es_batch_shape;
+
+int es_batch_create(int32_t device, es_batch** out);
+void es_batch_destroy(es_batch* b);
+const char* es_batch_last_error(const es_batch* b);
+/* Draws and bins the entry's field on the GPU (unless shared) and runs its start-cell test. */
+int es_batch_add_sample(es_batch* b, const es_batch_sample* sample, int32_t* index);
+int es_batch_set_shape(es_batch* b, const es_batch_shape* shape);
+int es_batch_get_shape(es_batch* b, es_batch_shape* shape);
+/* enable != 0: record the stage launches once in a hipGraph and replay it. */
+int es_batch_set_capture(es_batch* b, int32_t enable);
+/*
+ * Runs every entry: out holds, entry by entry, n_fields * n_temperatures
+ * sigmas (field-major); metrics (or NULL) one es_metrics per (entry, field);
+ * seconds (or NULL) the device time of the whole stage sequence.
+ */
+int es_batch_run(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
+/* Sizes of the batch: entries, (entry, field) pairs, sigma outputs, traced work items. */
+int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items);
+/* Times every trace launch shape on this batch and keeps the fastest. */
+int es_batch_tune(es_batch* b, es_batch_shape* best, double* best_trace_seconds);
+
 /* Build/version string of the library. */
 const char* es_version(void);
 

@@ -1,0 +1,249 @@
+"""Batched path: many uneven samples in one launch per stage (needs an MI355X).
+
+Bars: every entry of a batch gives bit-identical sigmas and equal metrics to
+the same sample run alone through es_solver (each pair keeps its own work
+items and its own fixed reduction order, whatever else shares the launch);
+entries are also checked against the CPU oracle at the bars of the one-sample
+parity tests; graph replays, launch segments and every tuner shape give the
+eager result bit for bit.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+METRIC_KEYS = ("cells_passed", "impurities_tested", "particles_inside_impurity", "particles_escaped",
+               "particles_at_bound", "max_impurities_tested", "max_cells_passed", "trace_cap_hits",
+               "real_particles", "avg_particle_lifetime", "traced_particles", "edge_hits", "cross_calls",
+               "sigma_evals", "cross_exact")
+
+RS, EXT = 1e-6, 5e-7
+
+
+def _entries():
+    """Uneven entries: sizes, grids, fields, temperatures, passes, a shared field."""
+    common = dict(region_size=RS, region_extends=EXT, is_clockwise=1, particle_speed=7e5)
+    return [
+        dict(seed=101, count=2000, impurity_radius=1.1e-8, cells_per_row=32, positions_per_row=64,
+             particles_per_quadrant=17, fields=[20.0, 5.0, 1.3], temperatures=[0.0], tau=1e-12,
+             alpha=math.pi / 4, coherent=False, incoherent_tau=1e-12, **common),
+        dict(seed=7, count=600, impurity_radius=2e-8, target_cell_population=4, positions_per_row=24,
+             particles_per_quadrant=9, fields=[1.0, 40.0], temperatures=[2.0, 40.0, 100.0], tau=1e-12,
+             alpha=math.pi / 4, coherent=False, **common),
+        dict(seed=0, share_field_of=1, impurity_radius=2e-8, target_cell_population=4, positions_per_row=24,
+             particles_per_quadrant=9, fields=[3.0, 11.0], temperatures=[2.0, 150.0], tau=1e-12, alpha=0.3,
+             coherent=True, **common),
+        dict(seed=13, count=3000, impurity_radius=5e-9, target_cell_population=15, positions_per_row=40,
+             particles_per_quadrant=5, fields=[3.0], temperatures=[], tau=1e-12, alpha=0.3, coherent=False,
+             incoherent_tau=2e-13, **common),
+        dict(seed=17, count=50, impurity_radius=1e-8, target_cell_population=2, positions_per_row=16,
+             particles_per_quadrant=3, fields=[0.7, 2.0, 9.0, 30.0],
+             temperatures=[1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 150.0], tau=1e-12, alpha=math.pi / 4,
+             coherent=False, **common),
+    ]
+
+
+def _add(batch, e):
+    kw = dict(e)
+    return batch.add(**kw)
+
+
+def _solo(es, e, entries):
+    """The entry run alone through es_solver: (sigma [nB, nT, 2], metrics per field)."""
+    src = entries[e["share_field_of"]] if e.get("share_field_of", -1) >= 0 else e
+    with es.Simulation(e["positions_per_row"], e["particles_per_quadrant"]) as sim:
+        sim.generate_impurities(src["seed"], RS, EXT, src["impurity_radius"], count=src.get("count", 0),
+                                target_cell_population=src.get("target_cell_population", 15),
+                                cells_per_row=src.get("cells_per_row", 0))
+        sim.init_sample(e["tau"], e["alpha"], e["is_clockwise"], e["particle_speed"], e["coherent"],
+                        e.get("incoherent_tau", 0.0))
+        return sim.compute_sweep(e["fields"], e["temperatures"])
+
+
+def _same_bits(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def _same_results(r0, r1):
+    sig0, m0, _ = r0
+    sig1, m1, _ = r1
+    assert len(sig0) == len(sig1)
+    for e, (a, b) in enumerate(zip(sig0, sig1)):
+        assert _same_bits(a, b), f"entry {e}: {a} vs {b}"
+    if m0 is not None and m1 is not None:
+        for e, (ma, mb) in enumerate(zip(m0, m1)):
+            for fa, fb in zip(ma, mb):
+                for k in METRIC_KEYS:
+                    assert fa[k] == fb[k], (e, k, fa[k], fb[k])
+
+
+@pytest.fixture(scope="module")
+def es():
+    import elasticscattering_amd as es_mod
+
+    es_mod.load_library()
+    return es_mod
+
+
+@pytest.fixture(scope="module")
+def batch_eager(es):
+    entries = _entries()
+    announce("batch of 5 uneven entries: eager run")
+    b = es.Batch()
+    for e in entries:
+        _add(b, e)
+    res = b.run()
+    yield entries, b, res
+    b.close()
+
+
+def test_batch_counts(batch_eager):
+    entries, b, res = batch_eager
+    c = b.counts()
+    assert c["samples"] == len(entries)
+    assert c["pairs"] == sum(len(e["fields"]) for e in entries)
+    assert c["outputs"] == sum(len(e["fields"]) * len(e["temperatures"]) for e in entries)
+    traced = sum(m["traced_particles"] for per in res[1] for m in per)
+    assert c["traced_items"] == traced
+
+
+def test_batch_entries_equal_solo_runs(es, batch_eager):
+    entries, b, (sig, mets, secs) = batch_eager
+    assert secs > 0
+    for i, e in enumerate(entries):
+        announce(f"batch entry {i} vs es_solver alone")
+        sig_s, ms_s = _solo(es, e, entries)
+        assert _same_bits(sig[i], sig_s), f"entry {i}: {sig[i]} vs {sig_s}"
+        for bi, (mb, msolo) in enumerate(zip(mets[i], ms_s)):
+            for k in METRIC_KEYS:
+                assert mb[k] == msolo[k], (i, bi, k, mb[k], msolo[k])
+
+
+@pytest.mark.parametrize("entry", [1, 2, 4])
+def test_batch_entries_against_oracle(es, oracle_esm, batch_eager, entry):
+    entries, b, (sig, mets, _) = batch_eager
+    e = entries[entry]
+    src = entries[e["share_field_of"]] if e.get("share_field_of", -1) >= 0 else e
+    announce(f"batch entry {entry} vs oracle")
+    og = O.OracleGrid.generate(oracle_esm, src["seed"], RS, EXT, 0.0, src["impurity_radius"],
+                               src.get("target_cell_population", 15), src.get("cells_per_row", 0),
+                               n=src.get("count", 0))
+    prm = O.sim_params(e["positions_per_row"], e["particles_per_quadrant"], RS, EXT, og.cell_size, e["tau"],
+                       e["alpha"], e["is_clockwise"], e["particle_speed"], e.get("incoherent_tau", 0.0), nthreads=8)
+    for bi, B in enumerate(e["fields"]):
+        so, mo, _ = O.compute_sigmas(oracle_esm, og, prm, int(e["coherent"]), B, e["temperatures"])
+        m = mets[entry][bi]
+        assert m["cells_passed"] == mo.pm.cells_passed
+        assert m["impurities_tested"] == mo.pm.impurities_tested
+        assert m["particles_escaped"] == mo.pm.particles_escaped
+        assert m["real_particles"] == mo.real_particles
+        for t, o in enumerate(so):
+            gxx, gxy = sig[entry][bi, t]
+            atol = 1e-12 * abs(o.xx_comp)
+            assert abs(gxx - o.xx_comp) <= 1e-12 * abs(o.xx_comp) + 1e-300
+            assert abs(gxy - o.xy_comp) <= 1e-12 * abs(o.xy_comp) + atol
+            assert abs(gxx - o.xx_naive) <= 1e-8 * abs(o.xx_naive) + 1e-300
+            assert abs(gxy - o.xy_naive) <= 1e-8 * abs(o.xy_naive) + atol
+
+
+def test_graph_replays_equal_eager(es, batch_eager):
+    entries, b, eager = batch_eager
+    announce("hipGraph capture + 3 replays with eager work between")
+    b.set_capture(True)
+    try:
+        other = es.Simulation(24, 5)
+        other.generate_impurities(99, RS, EXT, 1e-8, count=500, target_cell_population=4)
+        other.init_sample(1e-12, 0.3, 1, 7e5, False)
+        for rep in range(3):
+            r = b.run()
+            _same_results(eager, r)
+            other.compute_sigmas(2.0 + rep, [10.0])  # eager work on another stream between replays
+            np.sort(np.random.default_rng(rep).random(100000))  # and host work
+        other.close()
+    finally:
+        b.set_capture(False)
+    _same_results(eager, b.run())
+
+
+def test_segments_equal_one_launch(es, batch_eager):
+    entries, b, eager = batch_eager
+    announce("same batch cut into launch segments of <= 20000 items")
+    old = os.environ.get("ES_SEGMENT_ITEMS")
+    os.environ["ES_SEGMENT_ITEMS"] = "20000"
+    try:
+        b2 = es.Batch()
+    finally:
+        if old is None:
+            del os.environ["ES_SEGMENT_ITEMS"]
+        else:
+            os.environ["ES_SEGMENT_ITEMS"] = old
+    with b2:
+        for e in entries:
+            _add(b2, e)
+        _same_results(eager, b2.run())
+        b2.set_capture(True)
+        _same_results(eager, b2.run())
+        _same_results(eager, b2.run())
+
+
+def test_tuner_shapes_give_same_results(es, batch_eager):
+    entries, b, eager = batch_eager
+    announce("tuner over launch shapes")
+    best, t = b.tune()
+    assert best["waves_per_eu"] in (2, 3) and best["chunk"] in (32, 64, 128, 256, 512)
+    assert best["grid_mult"] in (1, 2) and t > 0
+    assert b.shape == best
+    _same_results(eager, b.run())
+    for shape in ((2, 32, 2), (3, 512, 1)):
+        b.set_shape(*shape)
+        _same_results(eager, b.run())
+    b.set_shape(3, 128, 1)
+
+
+def test_batch_errors(es):
+    announce("batch argument errors")
+    with es.Batch() as b:
+        with pytest.raises(es.EsError):
+            b.run()  # empty batch
+        e = _entries()[1]
+        _add(b, e)
+        bad = dict(e, is_clockwise=0)
+        _add(b, bad)
+        with pytest.raises(es.EsError) as ei:
+            b.run()  # two rotation senses
+        assert "rotation" in str(ei.value)
+    with es.Batch() as b:
+        with pytest.raises(es.EsError):
+            _add(b, dict(_entries()[1], temperatures=list(range(1, 10))))  # more than 8 temperatures
+        with pytest.raises(es.EsError):
+            _add(b, dict(_entries()[1], share_field_of=3))  # no such entry
+        with pytest.raises(es.EsError):
+            b.set_shape(4, 128, 1)
+
+
+def test_batched_uneven_workload_subset(es):
+    """Six samples of the 64-sample uneven workload (position grids capped at
+    64 per row) against their solo runs."""
+    from elasticscattering_amd.workloads import batched_samples
+
+    samples = batched_samples(n_samples=6, max_ppr=64)
+    announce("uneven workload subset: 6 samples, ppr <= 64")
+    with es.Batch() as b:
+        for s in samples:
+            b.add(**s)
+        sig, mets, _ = b.run()
+    for i, s in enumerate(samples):
+        with es.Simulation(s["positions_per_row"], s["particles_per_quadrant"]) as sim:
+            sim.generate_impurities(s["seed"], s["region_size"], s["region_extends"], s["impurity_radius"],
+                                    density=s["density"], target_cell_population=s["target_cell_population"])
+            sim.init_sample(s["tau"], s["alpha"], s["is_clockwise"], s["particle_speed"], s["coherent"])
+            ref, _ = sim.compute_sweep(s["fields"], s["temperatures"])
+        assert _same_bits(sig[i], ref), i
