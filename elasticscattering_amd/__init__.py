@@ -170,6 +170,19 @@ _SIGNATURES = {
                                   C.POINTER(C.c_int64)]),
     "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
     "es_version": (C.c_char_p, []),
+    # include/es_surrogate.h
+    "es_mlp_create": (C.c_int, [C.c_void_p, C.POINTER(_P)]),
+    "es_mlp_destroy": (None, [_P]),
+    "es_mlp_last_error": (C.c_char_p, [_P]),
+    "es_mlp_members": (C.c_int, [_P]),
+    "es_mlp_get_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "es_mlp_set_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "es_mlp_set_data": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64]),
+    "es_mlp_gradients": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "es_mlp_train": (C.c_int, [_P, C.c_void_p, C.POINTER(C.c_float)]),
+    "es_mlp_predict": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "es_mlp_last_train_time": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     # include/es_runner.h
     "es_config_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
     "es_run": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),

@@ -21,7 +21,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libes_hip.so")
 RUNNER = os.path.join(PKG, "es_run")
 
-HIP_SOURCES = ["es_kernels.hip", "es_abi.hip"]
+HIP_SOURCES = ["es_kernels.hip", "es_abi.hip", "es_mlp.hip"]
 HOST_SOURCES = ["host/es_config.cpp", "host/es_runner.cpp"]
 RUNNER_MAIN = "host/es_run_main.cpp"
 

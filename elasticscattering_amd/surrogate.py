@@ -1,0 +1,427 @@
+"""The learned sigma(B) surrogate and the surrogate-guided B sweep.
+
+* :class:`MLP` — ctypes handle of the ensemble MLP of include/es_surrogate.h
+  (6 -> 128 -> 128 -> 2, tanh; BF16 MFMA forward/backward, Adam).
+* :class:`Surrogate` — the MLP over physical inputs: features (log B, T,
+  log density, radius, tau, alpha) scaled to [-1, 1] over fixed ranges,
+  standardised targets.  Targets are asinh(sigma / S0)
+  per component: log-like over the decades sigma spans, finite where a
+  material conducts nothing (every start position inside an impurity gives
+  sigma = 0 exactly, at every B).
+* :func:`solver_runs` — sigma = coherent + incoherent pass
+  (SimulationRunner.cpp:161-173) from the batched GPU solver, for material
+  draws; the surrogate's training data and the sweep's solver calls.
+* :func:`adaptive_sweep` — a B curve per material from few solver points:
+  bisection of the gaps between solver points, each new point a check of the
+  interpolant, random solver checks before a curve is accepted; the
+  surrogate, when given, resolves gaps where it predicts the interpolant is
+  already right.  With ``surrogate=None`` it is the method without the
+  learned part.
+
+Error of a curve point: max(|d sigma_xx|, |d sigma_xy|) / |sigma|, with
+|sigma| = sqrt(sigma_xx^2 + sigma_xy^2) of the FP64 solver (the relative
+error of the conductivity tensor; sigma_xy alone can sit near zero, where
+the solver's own jitter exceeds any relative bound on it).
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import ES_OK, Batch, EsError, load_library
+
+NPARAM = 20752
+N_IN = 6
+
+# Feature ranges: B 0.1-60 T, T 0-150 K, density 1e14-1.5e16 m^-2,
+# radius 0.5e-8-2e-8 m, tau 1e-13-1e-12 s, alpha 0.2-pi/4.
+RANGES = dict(B=(0.1, 60.0), T=(0.0, 150.0), density=(1e14, 1.5e16), radius=(0.5e-8, 2e-8),
+              tau=(1e-13, 1e-12), alpha=(0.2, math.pi / 4))
+
+
+class MlpDesc(C.Structure):
+    _fields_ = [("members", C.c_int32), ("device", C.c_int32), ("seed", C.c_uint64)]
+
+
+class MlpTrainDesc(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("out_weight", C.c_float * 2),
+                ("shuffle_seed", C.c_uint64)]
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """Ensemble MLP on the GPU (include/es_surrogate.h)."""
+
+    def __init__(self, members=4, seed=1, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        d = MlpDesc(int(members), int(device), int(seed))
+        rc = self.lib.es_mlp_create(C.byref(d), C.byref(h))
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_mlp_last_error(None).decode())
+        self.h = h
+        self.members = int(members)
+
+    def _check(self, rc):
+        if rc != ES_OK:
+            raise EsError(rc, self.lib.es_mlp_last_error(self.h).decode())
+
+    @property
+    def params(self):
+        p = np.empty((self.members, NPARAM), dtype=np.float32)
+        self._check(self.lib.es_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).reshape(self.members, NPARAM)
+        self._check(self.lib.es_mlp_set_params(self.h, _fp(p)))
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert x.shape[1] == N_IN and t.shape == (x.shape[0], 2)
+        self._check(self.lib.es_mlp_set_data(self.h, _fp(x), _fp(t), x.shape[0]))
+
+    def gradients(self, rows, out_weight=(1.0, 1.0)):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        ow = np.ascontiguousarray(out_weight, dtype=np.float32)
+        g = np.empty((self.members, NPARAM), dtype=np.float32)
+        loss = np.empty(self.members, dtype=np.float32)
+        self._check(self.lib.es_mlp_gradients(self.h, rows.ctypes.data_as(C.POINTER(C.c_int32)), len(rows), _fp(ow),
+                                              _fp(g), _fp(loss)))
+        return g, loss
+
+    def train(self, epochs, batch=256, lr=2e-3, lr_final=1e-5, out_weight=(1.0, 1.0), shuffle_seed=7,
+              beta1=0.9, beta2=0.999, eps=1e-8):
+        d = MlpTrainDesc(int(epochs), int(batch), lr, lr_final, beta1, beta2, eps, (C.c_float * 2)(*out_weight),
+                         int(shuffle_seed))
+        hist = np.empty((int(epochs), self.members), dtype=np.float32)
+        self._check(self.lib.es_mlp_train(self.h, C.byref(d), _fp(hist)))
+        return hist
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((self.members, x.shape[0], 2), dtype=np.float32)
+        self._check(self.lib.es_mlp_predict(self.h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def last_train_time(self):
+        s = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.es_mlp_last_train_time(self.h, C.byref(s), C.byref(n)))
+        return s.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.es_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- features
+
+def features(B, T, density, radius, tau, alpha):
+    """Inputs scaled to [-1, 1]: (log B, T, log density, radius, tau, alpha)."""
+    B, T, density, radius, tau, alpha = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64)
+                                                               for v in (B, T, density, radius, tau, alpha)))
+
+    def lin(v, lo, hi):
+        return 2.0 * (v - lo) / (hi - lo) - 1.0
+
+    def log(v, lo, hi):
+        return lin(np.log(v), math.log(lo), math.log(hi))
+
+    R = RANGES
+    return np.stack([log(B, *R["B"]), lin(T, *R["T"]), log(density, *R["density"]), lin(radius, *R["radius"]),
+                     lin(tau, *R["tau"]), lin(alpha, *R["alpha"])], axis=-1)
+
+
+S0 = 1e-7  # target scale: asinh is logarithmic well above it, linear below
+
+
+def targets(sig):
+    """asinh(sigma / S0) of sigma [..., 2] (both components)."""
+    return np.arcsinh(np.asarray(sig, dtype=np.float64) / S0)
+
+
+def from_targets(y):
+    return S0 * np.sinh(np.asarray(y, dtype=np.float64))
+
+
+# ------------------------------------------------------------- solver runs
+
+def draw_materials(n, seed, alpha=None):
+    """n materials over RANGES (density and tau log-uniform, T floored at 1 K)."""
+    rng = np.random.default_rng(seed)
+    R = RANGES
+    mats = []
+    for i in range(n):
+        mats.append(dict(
+            density=math.exp(rng.uniform(math.log(R["density"][0]), math.log(R["density"][1]))),
+            radius=rng.uniform(*R["radius"]),
+            tau=math.exp(rng.uniform(math.log(R["tau"][0]), math.log(R["tau"][1]))),
+            T=max(rng.uniform(*R["T"]), 1.0),
+            alpha=rng.uniform(*R["alpha"]) if alpha is None else float(alpha),
+            seed=int(seed) * 100003 + i + 1,
+        ))
+    return mats
+
+
+def solver_runs(materials, fields, temperatures=None, positions_per_row=128, particles_per_quadrant=65,
+                side=2.0e-6, clockwise=1, speed=7e5, device=0, with_free=False):
+    """sigma [n_mat][n_B][n_T][2] = coherent + incoherent pass, all materials in one batch.
+
+    ``fields[i]`` / ``temperatures[i]`` are material i's B values and
+    temperatures (default: its own T).  Returns (sigma list, device seconds,
+    traced work items[, start positions outside every impurity per material]).
+    """
+    rs, ext = side / 2.0, side / 4.0
+    with Batch(device) as b:
+        for i, m in enumerate(materials):
+            Ts = [m["T"]] if temperatures is None else list(temperatures[i])
+            kw = dict(seed=m["seed"], region_size=rs, region_extends=ext, impurity_radius=m["radius"],
+                      density=m["density"], target_cell_population=4, positions_per_row=positions_per_row,
+                      particles_per_quadrant=particles_per_quadrant, fields=list(fields[i]), temperatures=Ts,
+                      tau=m["tau"], alpha=m["alpha"], is_clockwise=clockwise, particle_speed=speed)
+            j = b.add(coherent=True, **kw)
+            b.add(coherent=False, share_field_of=j, **kw)
+        items = b.counts()["traced_items"]
+        sig, mets, secs = b.run(metrics=with_free)
+    out = [sig[2 * i] + sig[2 * i + 1] for i in range(len(materials))]
+    if with_free:
+        free = [mets[2 * i + 1][0]["real_particles"] // (4 * particles_per_quadrant) for i in range(len(materials))]
+        return out, secs, items, free
+    return out, secs, items
+
+
+def training_set(n_materials, fields_per_material, temps_per_material, seed, **kw):
+    """(raw inputs [N][6] = B, T, density, radius, tau, alpha; sigma [N][2]; device s; items)."""
+    rng = np.random.default_rng(seed + 1)
+    mats = draw_materials(n_materials, seed)
+    lo, hi = (math.log(v) for v in RANGES["B"])
+    fields = [np.exp(rng.uniform(lo, hi, fields_per_material)) for _ in mats]
+    temps = [np.maximum(rng.uniform(*RANGES["T"], temps_per_material), 1.0) for _ in mats]
+    sig, secs, items = solver_runs(mats, fields, temps, **kw)
+    X, Y = [], []
+    for m, Bs, Ts, s in zip(mats, fields, temps, sig):
+        for bi, B in enumerate(Bs):
+            for ti, T in enumerate(Ts):
+                X.append((B, T, m["density"], m["radius"], m["tau"], m["alpha"]))
+                Y.append(s[bi, ti])
+    return np.array(X), np.array(Y), secs, items
+
+
+# --------------------------------------------------------------- surrogate
+
+class Surrogate:
+    """Ensemble MLP over physical inputs; predicts sigma and its spread."""
+
+    def __init__(self, members=4, seed=1, device=0):
+        self.mlp = MLP(members, seed, device)
+        self.mean = np.zeros(2)
+        self.std = np.ones(2)
+
+    def fit(self, X, sig, epochs=300, batch=256, lr=2e-3, lr_final=1e-5, shuffle_seed=7):
+        f = features(*X.T)
+        y = targets(sig)
+        self.mean = y.mean(axis=0)
+        self.std = y.std(axis=0) + 1e-12
+        self.mlp.set_data(f, (y - self.mean) / self.std)
+        return self.mlp.train(epochs, batch, lr, lr_final, shuffle_seed=shuffle_seed)
+
+    def predict_targets(self, X):
+        """Per-member targets [members][n][2] (asinh(sigma / S0))."""
+        y = self.mlp.predict(features(*np.asarray(X).T)).astype(np.float64)
+        return y * self.std + self.mean
+
+    def predict(self, X):
+        """(mean sigma [n][2], spread of the targets [n][2]) over the ensemble."""
+        y = self.predict_targets(X)
+        return from_targets(y.mean(axis=0)), y.std(axis=0)
+
+    def close(self):
+        self.mlp.close()
+
+
+def tensor_error(approx, exact):
+    """max(|d xx|, |d xy|) / |sigma| per point (0 where both are exactly 0)."""
+    approx = np.asarray(approx, dtype=np.float64)
+    exact = np.asarray(exact, dtype=np.float64)
+    norm = np.hypot(exact[..., 0], exact[..., 1])
+    d = np.max(np.abs(approx - exact), axis=-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e = d / norm
+    return np.where(d == 0.0, 0.0, np.where(norm > 0, e, np.inf))
+
+
+# ---------------------------------------------------------- adaptive sweep
+
+def _curve_from(u, idx, vals):
+    """PCHIP through the solved points in (log B, asinh(sigma / S0))."""
+    from scipy.interpolate import PchipInterpolator
+
+    y = targets(vals)
+    if len(idx) == 1:
+        return np.repeat(vals[:1], len(u), axis=0)
+    if len(idx) == 2:
+        est = np.stack([np.interp(u, u[idx], y[:, c]) for c in range(2)], axis=-1)
+    else:
+        est = PchipInterpolator(u[idx], y, axis=0)(u)
+    return from_targets(est)
+
+
+class _Curve:
+    def __init__(self, mat, u, prior):
+        self.mat = mat
+        self.u = u
+        self.prior = prior      # sigma [n][2] of the surrogate, or None
+        self.solved = {}        # index -> sigma [2]
+        self.active = set()     # gaps (a, b) of consecutive solved indices still to resolve
+        self.phase = "init"     # init -> refine -> check -> done
+        self.pending = []       # [(index, gap or None)] asked of the solver this round
+        self.checks = []        # (index, error) of the acceptance checks
+        self.accepted = False
+        self.skipped = 0        # gaps the surrogate resolved without a solver point
+
+    def estimate(self):
+        idx = np.array(sorted(self.solved))
+        vals = np.array([self.solved[i] for i in idx])
+        est = _curve_from(self.u, idx, vals)
+        est[idx] = vals
+        return est
+
+
+def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.25, kappa=0.05, n_checks=3,
+                   max_rounds=40, seed=11, **solver_kw):
+    """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
+
+    Each curve starts from n_initial solver points spread evenly over the
+    grid and is interpolated with PCHIP in (log B, asinh(sigma / S0)).  Every
+    gap between consecutive solver points is resolved by a solver run at its
+    midpoint, which is also a check: the gap is done when the interpolant
+    missed that point by at most safety * tol, otherwise both halves go on.
+    With a surrogate, a gap is done without a solver run when the surrogate's
+    own curve, interpolated through the same points, stays within kappa * tol
+    of itself over the gap (the surrogate predicts where interpolation
+    fails).  A curve is accepted once n_checks solver runs at randomly drawn
+    unsolved points all lie within tol of it; a failed check reopens the gaps
+    beside it.  Each round runs the solver once, in one batch, for every
+    point any unfinished curve asks for.  A material whose start positions
+    all lie inside impurities has sigma = 0 at every B (that test does not
+    depend on B): its curve is zero after the first round.
+    """
+    rng = np.random.default_rng(seed)
+    B = np.asarray(B, dtype=np.float64)
+    u = np.log(B)
+    n = len(B)
+    curves = []
+    for m in materials:
+        prior = None
+        if surrogate is not None:
+            X = np.stack([B, np.full(n, m["T"]), np.full(n, m["density"]), np.full(n, m["radius"]),
+                          np.full(n, m["tau"]), np.full(n, m["alpha"])], axis=-1)
+            prior = from_targets(surrogate.predict_targets(X).mean(axis=0))
+        curves.append(_Curve(m, u, prior))
+    init = [int(i) for i in np.unique(np.round(np.linspace(0, n - 1, max(2, n_initial))).astype(int))]
+    solver_s = 0.0
+    items = 0
+    rounds = 0
+    t0 = time.time()
+    while rounds < max_rounds:
+        # What every unfinished curve asks of the solver this round.
+        ask = []
+        for ci, c in enumerate(curves):
+            c.pending = []
+            if c.phase == "done":
+                continue
+            if c.phase == "init":
+                c.pending = [(i, None) for i in init]
+            elif c.phase == "refine":
+                idx = np.array(sorted(c.solved))
+                apri = None
+                if c.prior is not None:
+                    apri = tensor_error(_curve_from(u, idx, c.prior[idx]), c.prior)
+                for (a, b) in sorted(c.active):
+                    if apri is not None and apri[a:b + 1].max() <= kappa * tol:
+                        c.skipped += 1
+                        continue
+                    c.pending.append(((a + b) // 2, (a, b)))
+                c.active = set()
+                if not c.pending:
+                    c.phase = "check"
+            if c.phase == "check":
+                free = np.array([i for i in range(n) if i not in c.solved])
+                if free.size == 0:
+                    c.phase, c.accepted = "done", True
+                    continue
+                pts = rng.choice(free, size=min(n_checks, free.size), replace=False)
+                c.pending = [(int(i), None) for i in sorted(pts)]
+            if c.pending:
+                ask.append(ci)
+        if not ask:
+            break
+        rounds += 1
+        sig, secs, it, free_pos = solver_runs([curves[ci].mat for ci in ask],
+                                              [B[[i for i, _ in curves[ci].pending]] for ci in ask],
+                                              with_free=True, **solver_kw)
+        solver_s += secs
+        items += it
+        for ci, s, nf in zip(ask, sig, free_pos):
+            c = curves[ci]
+            vals = s[:, 0, :]
+            if c.phase == "init":
+                for (i, _), v in zip(c.pending, vals):
+                    c.solved[i] = v
+                if nf == 0:
+                    c.phase, c.accepted = "done", True
+                    continue
+                s_idx = sorted(c.solved)
+                c.active = {(a, b) for a, b in zip(s_idx[:-1], s_idx[1:]) if b - a > 1}
+                c.phase = "refine"
+                continue
+            est = c.estimate()
+            errs = tensor_error(est[[i for i, _ in c.pending]], vals)
+            for (i, _), v in zip(c.pending, vals):
+                c.solved[i] = v
+            if c.phase == "refine":
+                for (i, gap), e in zip(c.pending, errs):
+                    if e > tol * safety:
+                        a, b = gap
+                        c.active |= {g for g in ((a, i), (i, b)) if g[1] - g[0] > 1}
+                if not c.active:
+                    c.phase = "check"
+            else:  # check
+                c.checks += [(i, float(e)) for (i, _), e in zip(c.pending, errs)]
+                if np.all(errs <= tol):
+                    c.phase, c.accepted = "done", True
+                else:
+                    s_idx = sorted(c.solved)
+                    for (i, _), e in zip(c.pending, errs):
+                        if e > tol:
+                            k = s_idx.index(i)
+                            for g in ((s_idx[k - 1], i) if k > 0 else None,
+                                      (i, s_idx[k + 1]) if k + 1 < len(s_idx) else None):
+                                if g and g[1] - g[0] > 1:
+                                    c.active.add(g)
+                    c.phase = "refine" if c.active else "check"
+    est = np.stack([c.estimate() for c in curves])
+    report = dict(
+        rounds=rounds, curves=len(curves), accepted=int(sum(c.accepted for c in curves)),
+        solver_points=int(sum(len(c.solved) for c in curves)),
+        points_per_curve=float(np.mean([len(c.solved) for c in curves])),
+        gaps_skipped=int(sum(c.skipped for c in curves)),
+        max_check_error=float(max((e for c in curves for _, e in c.checks), default=0.0)),
+        solver_seconds=solver_s, traced_items=int(items), wall_seconds=time.time() - t0,
+    )
+    return est, report

@@ -16,7 +16,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def _declared_functions():
     names = []
-    for h in ("es_abi.h", "es_runner.h"):
+    for h in ("es_abi.h", "es_runner.h", "es_surrogate.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(es_[a-z_0-9]+)\s*\(", text, flags=re.M)

@@ -1,0 +1,110 @@
+"""Host logic of the adaptive B sweep (elasticscattering_amd.surrogate), with
+the solver replaced by closed-form Drude curves so no GPU is needed:
+sigma_xx = s0 / (1 + (mu B)^2), sigma_xy = s0 mu B / (1 + (mu B)^2)."""
+import math
+
+import numpy as np
+import pytest
+
+from elasticscattering_amd import surrogate as S
+
+B = np.logspace(-1, math.log10(60), 400)
+
+
+def _drude(m, Bs):
+    s0, mu = m["s0"], m["mu"]
+    d = 1 + (mu * Bs) ** 2
+    return np.stack([s0 / d, s0 * mu * Bs / d], axis=-1)
+
+
+def _mats(n=12, seed=0):
+    rng = np.random.default_rng(seed)
+    return [dict(s0=math.exp(rng.uniform(-3, 0)), mu=math.exp(rng.uniform(-3, 0.5)), density=1e15, radius=1e-8,
+                 tau=3e-13, T=50.0, alpha=0.5, seed=i, zero=(i == 3)) for i in range(n)]
+
+
+@pytest.fixture
+def fake_solver(monkeypatch):
+    calls = []
+
+    def run(materials, fields, temperatures=None, with_free=False, **kw):
+        out = []
+        for m, f in zip(materials, fields):
+            v = np.zeros((len(f), 2)) if m["zero"] else _drude(m, np.asarray(f))
+            out.append(v[:, None, :])
+        calls.append(sum(len(f) for f in fields))
+        free = [0 if m["zero"] else 1000 for m in materials]
+        return (out, 0.0, 0, free) if with_free else (out, 0.0, 0)
+
+    monkeypatch.setattr(S, "solver_runs", run)
+    return calls
+
+
+def test_sweep_accepts_within_tolerance(fake_solver):
+    mats = _mats()
+    est, rep = S.adaptive_sweep(mats, B, tol=1e-3)
+    assert rep["accepted"] == len(mats)
+    for m, e in zip(mats, est):
+        exact = np.zeros((len(B), 2)) if m["zero"] else _drude(m, B)
+        assert S.tensor_error(e, exact).max() <= 1e-3
+    assert rep["points_per_curve"] <= 80  # at least 5x fewer solver runs than the 400-point grid
+    assert rep["solver_points"] == sum(fake_solver)  # every solver point is counted once
+    # one solver batch per round
+    assert len(fake_solver) == rep["rounds"]
+
+
+def test_zero_material_stops_after_first_round(fake_solver):
+    mats = _mats()
+    est, rep = S.adaptive_sweep(mats, B)
+    assert not est[3].any()
+
+
+class _ExactSurrogate:
+    """A surrogate that knows the curves exactly (asinh targets per member)."""
+
+    def __init__(self, mats):
+        self.mats = mats
+
+    def predict_targets(self, X):
+        m = [m for m in self.mats if m["tau"] == X[0, 4]][0]
+        return S.targets(_drude(m, X[:, 0]))[None]
+
+
+def test_exact_surrogate_saves_solver_points(fake_solver):
+    mats = [dict(m, tau=1e-13 * (i + 1)) for i, m in enumerate(_mats())]
+    _, plain = S.adaptive_sweep(mats, B)
+    est, guided = S.adaptive_sweep(mats, B, surrogate=_ExactSurrogate(mats))
+    assert guided["gaps_skipped"] > 0
+    assert guided["solver_points"] < plain["solver_points"]
+    assert guided["accepted"] == len(mats)
+    for m, e in zip(mats, est):
+        exact = np.zeros((len(B), 2)) if m["zero"] else _drude(m, B)
+        assert S.tensor_error(e, exact).max() <= 1e-3
+
+
+def test_failed_check_reopens_gaps(fake_solver, monkeypatch):
+    """A narrow bump the initial points miss is found by a random check or
+    a midpoint and then resolved."""
+    mats = _mats(4, seed=2)
+    base = S.solver_runs
+
+    def bumpy(materials, fields, temperatures=None, with_free=False, **kw):
+        r = base(materials, fields, temperatures, with_free, **kw)
+        for m, f, s in zip(materials, fields, r[0]):
+            s[:, 0, 0] *= 1 + 0.02 * np.exp(-((np.log(np.asarray(f)) - 1.0) / 0.3) ** 2)
+        return r
+
+    monkeypatch.setattr(S, "solver_runs", bumpy)
+    est, rep = S.adaptive_sweep(mats, B, n_checks=8)
+    assert rep["accepted"] == len(mats)
+
+
+def test_tensor_error_conventions():
+    assert S.tensor_error(np.zeros(2), np.zeros(2)) == 0.0
+    assert S.tensor_error(np.array([1e-9, 0]), np.zeros(2)) == np.inf
+    assert S.tensor_error(np.array([1.001, 0.5]), np.array([1.0, 0.5])) == pytest.approx(0.001 / math.hypot(1, 0.5))
+
+
+def test_targets_round_trip():
+    sig = np.array([[0.5, 1e-3], [0.0, 0.0], [1e-9, -1e-12]])
+    np.testing.assert_allclose(S.from_targets(S.targets(sig)), sig, rtol=1e-12, atol=1e-30)
