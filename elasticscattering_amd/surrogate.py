@@ -285,7 +285,7 @@ class _Curve:
     def __init__(self, mat, u, prior):
         self.mat = mat
         self.u = u
-        self.prior = prior      # sigma [n][2] of the surrogate, or None
+        self.prior = prior      # sigma [members][n][2] of the surrogate, or None
         self.solved = {}        # index -> sigma [2]
         self.active = set()     # gaps (a, b) of consecutive solved indices still to resolve
         self.phase = "init"     # init -> refine -> check -> done
@@ -302,7 +302,7 @@ class _Curve:
         return est
 
 
-def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.25, kappa=0.05, n_checks=3,
+def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.25, kappa=0.2, n_checks=3,
                    max_rounds=40, seed=11, **solver_kw):
     """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
 
@@ -311,10 +311,11 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     gap between consecutive solver points is resolved by a solver run at its
     midpoint, which is also a check: the gap is done when the interpolant
     missed that point by at most safety * tol, otherwise both halves go on.
-    With a surrogate, a gap is done without a solver run when the surrogate's
-    own curve, interpolated through the same points, stays within kappa * tol
-    of itself over the gap (the surrogate predicts where interpolation
-    fails).  A curve is accepted once n_checks solver runs at randomly drawn
+    With a surrogate, a gap is done without a solver run when the curve of
+    every ensemble member, interpolated through the same points, stays within
+    kappa * tol of that member over the gap: the surrogate predicts where
+    interpolation fails, and members that disagree on the curve's shape (its
+    uncertainty) keep the solver run.  A curve is accepted once n_checks solver runs at randomly drawn
     unsolved points all lie within tol of it; a failed check reopens the gaps
     beside it.  Each round runs the solver once, in one batch, for every
     point any unfinished curve asks for.  A material whose start positions
@@ -331,7 +332,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
         if surrogate is not None:
             X = np.stack([B, np.full(n, m["T"]), np.full(n, m["density"]), np.full(n, m["radius"]),
                           np.full(n, m["tau"]), np.full(n, m["alpha"])], axis=-1)
-            prior = from_targets(surrogate.predict_targets(X).mean(axis=0))
+            prior = from_targets(surrogate.predict_targets(X))
         curves.append(_Curve(m, u, prior))
     init = [int(i) for i in np.unique(np.round(np.linspace(0, n - 1, max(2, n_initial))).astype(int))]
     solver_s = 0.0
@@ -351,7 +352,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
                 idx = np.array(sorted(c.solved))
                 apri = None
                 if c.prior is not None:
-                    apri = tensor_error(_curve_from(u, idx, c.prior[idx]), c.prior)
+                    apri = np.max([tensor_error(_curve_from(u, idx, p[idx]), p) for p in c.prior], axis=0)
                 for (a, b) in sorted(c.active):
                     if apri is not None and apri[a:b + 1].max() <= kappa * tol:
                         c.skipped += 1
