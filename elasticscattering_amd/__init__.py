@@ -38,6 +38,8 @@ ES_ERR_OOM = -5
 ES_ERR_OUT_OF_GRID = -6
 ES_ERR_NO_DEVICE = -7
 ES_PRECISION_FP64 = 0
+ES_PRECISION_MIXED = 1
+_PRECISIONS = {"fp64": ES_PRECISION_FP64, "mixed": ES_PRECISION_MIXED}
 
 
 class EsError(RuntimeError):
@@ -102,6 +104,7 @@ class Metrics(C.Structure):
         ("cross_calls", C.c_int64),
         ("sigma_evals", C.c_int64),
         ("cross_exact", C.c_int64),
+        ("fp64_recomputed", C.c_int64),
     ]
 
     def as_dict(self):
@@ -165,6 +168,7 @@ _SIGNATURES = {
     "es_batch_set_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_get_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
+    "es_batch_set_precision": (C.c_int, [_P, C.c_int32]),
     "es_batch_run": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_batch_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int64)]),
@@ -246,7 +250,7 @@ class Simulation:
     first grid set on the solver, as SimulationRunner.cpp:75-84 does.
     """
 
-    def __init__(self, positions_per_row, particles_per_quadrant, grid_info=None, device=0):
+    def __init__(self, positions_per_row, particles_per_quadrant, grid_info=None, device=0, precision="fp64"):
         self.lib = load_library()
         self.ppr = int(positions_per_row)
         self.ppq = int(particles_per_quadrant)
@@ -256,7 +260,7 @@ class Simulation:
             gi = grid_info
         else:
             gi = GridInfo(**grid_info)
-        desc = SolverDesc(self.ppr, self.ppq, gi, ES_PRECISION_FP64, int(device))
+        desc = SolverDesc(self.ppr, self.ppq, gi, _PRECISIONS.get(precision, precision), int(device))
         h = _P()
         rc = self.lib.es_solver_create(C.byref(desc), C.byref(h))
         if rc != ES_OK:
@@ -413,6 +417,10 @@ class Batch:
 
     def set_capture(self, enable):
         self._check(self.lib.es_batch_set_capture(self.h, int(bool(enable))))
+
+    def set_precision(self, precision):
+        """"fp64" (default) or "mixed" (FP32 trace with FP64 recompute of undecidable cases)."""
+        self._check(self.lib.es_batch_set_precision(self.h, _PRECISIONS.get(precision, precision)))
 
     def tune(self):
         """Times every trace launch shape on this batch; keeps and returns the fastest (shape, seconds)."""

@@ -56,6 +56,7 @@ struct Ctx {
     size_t bytes_held = 0;
     int cus = 0;
     int trace_bpc[2][4] = {{0}};  // resident trace blocks per CU [clockwise][waves_per_eu]
+    int trace32_bpc[2] = {0, 0};  // resident FP32 trace blocks per CU [clockwise]
     long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
 };
@@ -384,6 +385,7 @@ struct Shape {
 // Device buffers and the optional captured graph of the stage kernels.
 struct Stages {
     DevBuf samples, pairs, factor, dpart, mpart, sig, ltsum, mout, lt_item, m_item, x_item, f_item, queue, lifetimes;
+    DevBuf list, list_count;  // mixed precision: items the FP32 trace hands to the FP64 one
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -394,8 +396,8 @@ void stages_release(Ctx& c, Stages& s)
     if (s.graph) (void)hipGraphDestroy(s.graph);
     s.exec = nullptr;
     s.graph = nullptr;
-    DevBuf* bufs[] = {&s.samples, &s.pairs, &s.factor, &s.dpart,  &s.mpart,  &s.sig,    &s.ltsum,
-                      &s.mout,    &s.lt_item, &s.m_item, &s.x_item, &s.f_item, &s.queue, &s.lifetimes};
+    DevBuf* bufs[] = {&s.samples, &s.pairs,  &s.factor, &s.dpart,  &s.mpart, &s.sig,       &s.ltsum, &s.mout,
+                      &s.lt_item, &s.m_item, &s.x_item, &s.f_item, &s.queue, &s.lifetimes, &s.list,  &s.list_count};
     for (DevBuf* b : bufs) release(c, *b);
 }
 
@@ -416,6 +418,7 @@ struct Plan {
     std::vector<Segment> segs;
     int nout = 0;
     int clockwise = 0;
+    int precision = ES_PRECISION_FP64;
     int NT = 1;
     long long total_items = 0;   // over all segments
     long long max_items = 0;     // largest segment
@@ -486,6 +489,10 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     if ((rc = ensure(c, s.x_item, items * sizeof(int32_t), "item exact solves"))) return rc;
     if ((rc = ensure(c, s.f_item, items, "item flags"))) return rc;
     if ((rc = ensure(c, s.queue, sizeof(unsigned long long), "work queue"))) return rc;
+    if (pl.precision == ES_PRECISION_MIXED) {
+        if ((rc = ensure(c, s.list, items * sizeof(int32_t), "recompute list"))) return rc;
+        if ((rc = ensure(c, s.list_count, sizeof(int32_t), "recompute count"))) return rc;
+    }
     if ((rc = ctx_events(c, 4 * pl.segs.size()))) return rc;
     ES_HIP(c, hipMemcpyAsync(s.samples.p, pl.samples.data(), pl.samples.size() * sizeof(es::SampleDev),
                              hipMemcpyHostToDevice, c.stream));
@@ -496,19 +503,31 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
 }
 
 // Trace grid size of every segment: enough resident blocks to fill the chip
-// (times grid_mult), never more blocks than the segment has items for.
-int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<int>& nblocks)
+// (times grid_mult), never more blocks than the segment has items for.  With
+// mixed precision, nblocks holds the FP32 trace's grid and nblocks64 the FP64
+// recompute's (the full resident grid: its item count is known on the device only).
+int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<int>& nblocks,
+                 int* nblocks64 = nullptr)
 {
     const int w = (sh.waves_per_eu == 2) ? 2 : 3;
     if (c.trace_bpc[cw][w] <= 0) {
         ES_HIP(c, es::trace_blocks_per_cu(cw, w, &c.trace_bpc[cw][w]));
         if (c.trace_bpc[cw][w] <= 0) c.trace_bpc[cw][w] = 1;
     }
+    int bpc = c.trace_bpc[cw][w];
+    if (pl.precision == ES_PRECISION_MIXED) {
+        if (c.trace32_bpc[cw] <= 0) {
+            ES_HIP(c, es::trace32_blocks_per_cu(cw, &c.trace32_bpc[cw]));
+            if (c.trace32_bpc[cw] <= 0) c.trace32_bpc[cw] = 1;
+        }
+        if (nblocks64) *nblocks64 = c.trace_bpc[cw][w] * c.cus;
+        bpc = c.trace32_bpc[cw];
+    }
     nblocks.clear();
     for (const Segment& g : pl.segs) {
         const long long want = (g.items + es::kBlock - 1) / es::kBlock;
         nblocks.push_back((int)std::max<long long>(
-            1, std::min<long long>(want, (long long)c.trace_bpc[cw][w] * c.cus * std::max(1, sh.grid_mult))));
+            1, std::min<long long>(want, (long long)bpc * c.cus * std::max(1, sh.grid_mult))));
     }
     return ES_OK;
 }
@@ -517,7 +536,7 @@ int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<in
 // Enqueues the stage kernels of every segment (no host copies or memsets: the
 // sequence can be captured in a graph).  Events (4 per segment) only when not capturing.
 int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trace, int with_metrics,
-                   double* lifetimes_dev, const std::vector<int>& nblocks, bool events = true)
+                   double* lifetimes_dev, const std::vector<int>& nblocks, bool events = true, int nblocks64 = 0)
 {
     for (size_t si = 0; si < pl.segs.size(); si++) {
         const Segment& g = pl.segs[si];
@@ -538,7 +557,13 @@ int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trac
             a.lifetimes = lifetimes_dev;
             a.P = pl.P;
             if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si], c.stream));
-            ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, nblocks[si], c.stream));
+            if (pl.precision == ES_PRECISION_MIXED) {
+                a.list = P<int32_t>(s.list);
+                a.list_count = P<int32_t>(s.list_count);
+                ES_HIP(c, es::launch_trace_mixed(a, pl.clockwise, sh.waves_per_eu, nblocks[si], nblocks64, c.stream));
+            } else {
+                ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, nblocks[si], c.stream));
+            }
             if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 1], c.stream));
         }
         es::SigmaArgs sa{};
@@ -617,6 +642,7 @@ void fill_metrics(es_metrics& o, const int64_t* m, double ltsum, int64_t P, int6
     o.cross_calls = m[9];
     o.sigma_evals = m[10];
     o.cross_exact = m[11];
+    o.fp64_recomputed = m[12];
 }
 
 // One sample / pass as a SampleDev (tau and nT are filled by the caller).
@@ -685,6 +711,7 @@ struct es_batch {
     std::vector<Entry> entries;
     Stages s;
     Shape shape;
+    int precision = ES_PRECISION_FP64;
     bool capture = false;
     bool graph_ready = false;
     std::vector<int> graph_blocks;
@@ -720,6 +747,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
     Plan pl;
     pl.samples.push_back(sample_dev(sv->f, ppr, ppq, dist, offx, offy, sv->ps, sv->st, sv->coherent));
     pl.clockwise = sv->st.is_clockwise ? 1 : 0;
+    pl.precision = sv->desc.precision;
     pl.nout = nB;
     std::vector<double> w(nB);
     for (int b = 0; b < nB; b++) {
@@ -739,6 +767,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
 
     Shape sh;
     std::vector<int> nblocks;
+    int nblocks64 = 0;
     double t_life = 0, t_temp = 0;
     std::vector<double> ltsum;
     std::vector<int64_t> mh;
@@ -763,9 +792,10 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
         // plan; with several segments only the last one's are still resident.
         const bool trace = (ch == 0) || pl.segs.size() > 1;
         const bool first = (ch == 0);
-        if (trace && (rc = trace_blocks(c, pl.clockwise, sh, pl, nblocks))) return rc;
+        if (trace && (rc = trace_blocks(c, pl.clockwise, sh, pl, nblocks, &nblocks64))) return rc;
         if ((rc = stages_enqueue(c, sv->s, pl, sh, trace, first ? 1 : 0,
-                                 (lifetimes_host && first) ? P<double>(sv->s.lifetimes) : nullptr, nblocks)))
+                                 (lifetimes_host && first) ? P<double>(sv->s.lifetimes) : nullptr, nblocks, true,
+                                 nblocks64)))
             return rc;
         StageOut so;
         if ((rc = stages_collect(c, sv->s, pl, trace, first, so))) return rc;
@@ -796,7 +826,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
 
 extern "C" {
 
-const char* es_version(void) { return "es-mi355x 0.2 (gfx950, FP64, batched)"; }
+const char* es_version(void) { return "es-mi355x 0.3 (gfx950, FP64 + mixed FP32/FP64, batched, MLP surrogate)"; }
 
 int es_solver_create(const es_solver_desc* desc, es_solver** out)
 {
@@ -818,7 +848,7 @@ int es_solver_create(const es_solver_desc* desc, es_solver** out)
         g_create_error = "grid_info.region_size and cell_size must be >= 0 (0: taken from the first grid)";
         return ES_ERR_INVALID_ARG;
     }
-    if (desc->precision != ES_PRECISION_FP64) {
+    if (desc->precision != ES_PRECISION_FP64 && desc->precision != ES_PRECISION_MIXED) {
         g_create_error = "unsupported precision mode";
         return ES_ERR_INVALID_ARG;
     }
@@ -929,6 +959,7 @@ int batch_plan(es_batch* bt)
     pl = Plan();
     if (bt->entries.empty()) return fail(c, ES_ERR_INVALID_ARG, "empty batch");
     pl.clockwise = bt->entries[0].st.is_clockwise ? 1 : 0;
+    pl.precision = bt->precision;
     int out = 0;
     for (size_t e = 0; e < bt->entries.size(); e++) {
         const es_batch::Entry& en = bt->entries[e];
@@ -985,7 +1016,8 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
         if ((rc = stages_prepare(c, bt->s, bt->plan))) return rc;
     }
     std::vector<int> nblocks;
-    if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks))) return rc;
+    int nblocks64 = 0;
+    if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks, &nblocks64))) return rc;
     ES_HIP(c, hipStreamSynchronize(c.stream));
     hipEvent_t t0 = nullptr, t1 = nullptr;
     ES_HIP(c, hipEventCreate(&t0));
@@ -999,7 +1031,7 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
             e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) rc = hip_fail(c, e, "hipStreamBeginCapture");
             if (!rc) {
-                rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, false);
+                rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, false, nblocks64);
                 hipGraph_t g = nullptr;
                 e = hipStreamEndCapture(c.stream, &g);
                 if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
@@ -1019,7 +1051,7 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
             if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphLaunch");
         }
     } else if (!rc) {
-        rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks);
+        rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, true, nblocks64);
     }
     if (!rc) {
         e = hipEventRecord(t1, c.stream);
@@ -1159,6 +1191,19 @@ int es_batch_set_capture(es_batch* bt, int32_t enable)
     if (!bt) return ES_ERR_INVALID_ARG;
     bt->capture = enable != 0;
     batch_release_graph(bt);
+    return ES_OK;
+}
+
+int es_batch_set_precision(es_batch* bt, int32_t precision)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    if (precision != ES_PRECISION_FP64 && precision != ES_PRECISION_MIXED)
+        return fail(bt->c, ES_ERR_INVALID_ARG, "unsupported precision mode");
+    if (precision != bt->precision) {
+        bt->precision = precision;
+        bt->plan_ready = false;
+        batch_release_graph(bt);
+    }
     return ES_OK;
 }
 

@@ -14,8 +14,9 @@ constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kMaxT = 8;           // temperatures per sample
 // int64 metric partials per block: 0 cells, 1 tested, 2 escaped, 3 at_bound,
 // 4 cap hits, 5 max tested (max), 6 max cells (max), 7 traced particles,
-// 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time solves.
-constexpr int kMetricSlots = 12;
+// 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time
+// solves, 12 items traced again in FP64 (mixed precision).
+constexpr int kMetricSlots = 13;
 
 // Binned field on the device (cell-ordered (x, y) pairs; reference index layout).
 struct GridDev {
@@ -67,6 +68,12 @@ struct TraceArgs {
     uint8_t* f_item;                 // ES_F_* flags
     double* lifetimes;               // [pair][P] reference layout, or null
     long long P;                     // ppr^2 * 4 * ppq when lifetimes is set
+    // Mixed precision: the FP32 trace appends the items it cannot decide to
+    // `list` (count in *list_count); the FP64 trace with list_mode = 1 takes
+    // its work items from that list.
+    int32_t* list;
+    int32_t* list_count;
+    int list_mode;
 };
 
 struct SigmaArgs {
@@ -111,6 +118,10 @@ hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx,
 hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int* blocks);
 // Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
 hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks, hipStream_t st);
+// Mixed precision: FP32 trace of every item, then FP64 trace of the flagged ones.
+hipError_t trace32_blocks_per_cu(int clockwise, int* blocks);
+hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks32, int nblocks64,
+                              hipStream_t st);
 hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st);
 
 // One block per pair: out_sig [out][kMaxT][2], out_lt_sum [out], out_m [out][kMetricSlots].

@@ -25,6 +25,7 @@
 #include "es_kernels.h"
 #include "es_rng.h"
 #include "es_trace.h"
+#include "es_trace32.h"
 
 namespace es {
 
@@ -403,7 +404,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const long long total = a.total_items;
+    const long long total = a.list_mode ? (long long)*a.list_count : a.total_items;
 
     long long q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
     bool exhausted = false;
@@ -436,7 +437,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
             if (!active) {
                 const int rank = __popcll(need & below);
                 if (rank < take) {
-                    item = q_next + rank;
+                    item = a.list_mode ? (long long)a.list[q_next + rank] : q_next + rank;
                     const int pi = pair_of_item(a.pairs, a.npairs, item);
                     const PairDev& pr = a.pairs[pi];
                     const SampleDev& sd = a.samples[pr.sample];
@@ -510,6 +511,146 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
                 a.lt_item[item] = lt;
                 a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
                 a.x_item[item] = st.cross_exact;
+                a.f_item[item] = (uint8_t)(flags | (a.list_mode ? ES_F_RECOMPUTE : 0));
+                if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+                active = false;
+            }
+        }
+    }
+}
+
+// The FP32 trace (es_trace32.h) over the same work items: same lane refill,
+// same start data (position, phi, orbit centre, bound time in FP64); items it
+// cannot decide are appended to a.list for the FP64 trace.
+template <bool CW>
+__device__ __forceinline__ void trace32_body(const TraceArgs& a)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const long long total = a.total_items;
+
+    long long q_next = 0, q_end = 0;
+    bool exhausted = false;
+    bool active = false;
+    long long item = 0, out_idx = -1;
+    int cap = 0;
+    DevGrid g;
+    Orb32 o;
+    double w = 1.0, bound_time = 0.0;
+    Trace32 st;
+
+    while (true) {
+        unsigned long long need = __ballot(!active);
+        while (need != 0ull && !exhausted) {
+            if (q_next >= q_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = __shfl(base, 0, 64);
+                if ((long long)base >= total) {
+                    exhausted = true;
+                    break;
+                }
+                q_next = (long long)base;
+                q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
+            }
+            const long long left = q_end - q_next;
+            const int n = __popcll(need);
+            const int take = (int)((long long)n < left ? (long long)n : left);
+            if (!active) {
+                const int rank = __popcll(need & below);
+                if (rank < take) {
+                    item = q_next + rank;
+                    const int pi = pair_of_item(a.pairs, a.npairs, item);
+                    const PairDev& pr = a.pairs[pi];
+                    const SampleDev& sd = a.samples[pr.sample];
+                    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
+                    const long long k = item - pr.item_off;
+                    const long long slot = k / ppp;
+                    const int v = (int)(k - slot * ppp);
+                    const int q = v / ppq, p = v - (v / ppq) * ppq;
+                    const int pidx = sd.free_pos[slot];
+                    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+                    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+                    g.ixy = sd.ixy;
+                    g.cell_end = sd.cell_end;
+                    g.start = sd.start;
+                    g.size = sd.size;
+                    g.L = sd.L;
+                    g.radius = sd.radius;
+                    g.cpr = sd.cpr;
+                    cap = 4 * sd.cpr * sd.cpr + 16;
+                    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
+                    const double py = (double)j * sd.dist + sd.offy;
+                    w = pr.w;
+                    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;  // lifetime.h:36-58
+                    double sphi, cphi;
+                    esm_sincos(phi, &sphi, &cphi);
+                    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
+                    const double R = sd.speed / w;
+                    const double sx = vy * R / sd.speed;
+                    const double sy = -vx * R / sd.speed;
+                    const double ocx = CW ? (px - sx) : (px + sx);
+                    const double ocy = CW ? (py - sy) : (py + sy);
+                    o.sx = px;
+                    o.sy = py;
+                    o.inv_r = 1.0 / R;
+                    o.nxd = (ocx - px) * o.inv_r;
+                    o.nyd = (ocy - py) * o.inv_r;
+                    o.nx = (float)o.nxd;
+                    o.ny = (float)o.nyd;
+                    o.r2 = g.radius * g.radius;
+                    // The reference's travel angle (GetCrossAngle, details.h:111-116)
+                    // runs along the orbit against (cos phi, sin phi), in both senses.
+                    o.txd = -cphi;
+                    o.tyd = -sphi;
+                    o.tx = (float)o.txd;
+                    o.ty = (float)o.tyd;
+                    o.rr = (float)(g.radius * o.inv_r);
+                    o.rr2 = o.rr * o.rr;
+                    o.ccw = !CW;
+                    // A negative raw angle changes the reference's ranges (es_trace32.h).
+                    o.quirk = phi < 0.0;
+                    const float hm = tanf((float)(-0.5 * phi));
+                    o.km = t32_key(Root32{hm, false});
+                    o.km2 = t32_key(Root32{-hm, false});
+                    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
+                    if (!sd.coherent) {
+                        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
+                        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
+                        bound_time = dphi / w;
+                    }
+                    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
+                    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
+                    t32_begin(st, c0x, c0y);
+                    active = true;
+                }
+            }
+            q_next += take;
+            need = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+
+        if (active) {
+            int flags = 0;
+            double theta = 0.0;
+            if (t32_step(o, g, cap, st, flags, theta)) {
+                double lt = 0.0;
+                if (flags == 0) {
+                    lt = theta / w;
+                    if (bound_time < ES_INF && fabs(lt - bound_time) <= 1e-5 * bound_time) {
+                        flags = ES_F_RECOMPUTE;  // FP32 cannot tell which side of the bound
+                    } else if (lt > bound_time) {  // lifetime.h:142-146
+                        lt = bound_time;
+                        flags = ES_F_AT_BOUND;
+                    }
+                }
+                if (flags & ES_F_RECOMPUTE) {
+                    const int idx = atomicAdd(a.list_count, 1);
+                    a.list[idx] = (int32_t)item;
+                }
+                a.lt_item[item] = lt;
+                a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
+                a.x_item[item] = 0;
                 a.f_item[item] = (uint8_t)flags;
                 if (out_idx >= 0) a.lifetimes[out_idx] = lt;
                 active = false;
@@ -529,6 +670,12 @@ template <bool CW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_w2(TraceArgs a)
 {
     trace_body<CW>(a);
+}
+
+template <bool CW>
+__global__ __launch_bounds__(kBlock) void k_trace32(TraceArgs a)
+{
+    trace32_body<CW>(a);
 }
 
 // --------------------------------------------------------------------- sigma
@@ -551,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q].s = acc[q].c = 0.0;
     int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0, m_edge = 0, m_cross = 0, m_sigma = 0;
-    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0;
+    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0, m_recomp = 0;
 
     const long long stride = (long long)pr.nblk * kBlock;
     for (long long k = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x; k < nitems; k += stride) {
@@ -575,6 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
             m_escaped += (f & ES_F_ESCAPED) ? 1 : 0;
             m_at_bound += (f & ES_F_AT_BOUND) ? 1 : 0;
             m_cap += (f & ES_F_CAP) ? 1 : 0;
+            m_recomp += (f & ES_F_RECOMPUTE) ? 1 : 0;
             m_items += 1;
         }
         dd_add_d(acc[2 * NT], lt_raw);
@@ -620,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
         }
     }
     int64_t mv[kMetricSlots] = {m_cells, m_tested, m_escaped, m_at_bound, m_cap, m_max_tested,
-                                m_max_cells, m_items, m_edge, m_cross, m_sigma, m_exact};
+                                m_max_cells, m_items, m_edge, m_cross, m_sigma, m_exact, m_recomp};
 #pragma unroll
     for (int s = 0; s < kMetricSlots; s++) {
         const int64_t r = (s == 5 || s == 6) ? i64_wave_max(mv[s]) : i64_wave_sum(mv[s]);
@@ -653,6 +801,12 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 }
 
 __global__ void k_reset(unsigned long long* q) { *q = 0ull; }
+
+__global__ void k_reset2(unsigned long long* q, int32_t* count)
+{
+    *q = 0ull;
+    *count = 0;
+}
 
 // ------------------------------------------------------------------ finalize
 
@@ -809,7 +963,7 @@ hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int
     if (a.npairs <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || a.total_items <= 0) return e;
+    if (e != hipSuccess || (!a.list_mode && a.total_items <= 0)) return e;
     if (waves_per_eu == 2) {
         if (clockwise)
             hipLaunchKernelGGL((k_trace_w2<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
@@ -822,6 +976,35 @@ hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int
             hipLaunchKernelGGL((k_trace_w3<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
     }
     return hipGetLastError();
+}
+
+hipError_t trace32_blocks_per_cu(int clockwise, int* blocks)
+{
+    return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace32<true>, kBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace32<false>, kBlock, 0);
+}
+
+hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks32, int nblocks64,
+                              hipStream_t st)
+{
+    if (a.npairs <= 0 || nblocks32 <= 0 || nblocks64 <= 0 || a.chunk <= 0 || !a.list || !a.list_count)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset2, dim3(1), dim3(1), 0, st, a.queue, a.list_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.total_items <= 0) return e;
+    TraceArgs a32 = a;
+    a32.list_mode = 0;
+    if (clockwise)
+        hipLaunchKernelGGL((k_trace32<true>), dim3(nblocks32), dim3(kBlock), 0, st, a32);
+    else
+        hipLaunchKernelGGL((k_trace32<false>), dim3(nblocks32), dim3(kBlock), 0, st, a32);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // The flagged items, in FP64: the count is read on the device, so no host
+    // round trip separates the two traces.
+    TraceArgs a64 = a;
+    a64.list_mode = 1;
+    a64.chunk = 16;
+    return launch_trace(a64, clockwise, waves_per_eu, nblocks64, st);
 }
 
 template <int NT>

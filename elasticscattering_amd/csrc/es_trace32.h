@@ -1,0 +1,385 @@
+// es_trace32.h — one cell of a particle's trace in FP32, with flags for the
+// cases FP32 cannot decide (the mixed-precision path; es_trace.h is FP64).
+//
+// Same trace as TraceOrbit (lifetime.h:78-148), in other coordinates.  Every
+// length is taken relative to the particle's start point S and divided by the
+// orbit radius R; the orbit is
+//     X(theta) = n (1 - cos theta) + t sin theta,
+// n the unit vector from S to the orbit centre, theta >= 0 the angle of the
+// reference's travel (GetCrossAngle, details.h:111-116), t its direction at
+// S — which is -(cos phi, sin phi) in both rotation senses: the reference's
+// time runs along the orbit against the velocity it starts with.  With
+// w = tan(theta / 2) a crossing with a disk or a cell-edge line is a
+// quadratic A w^2 + 2 Bh w + C = 0, solved in the stable form
+// s = -Bh - sign(Bh) sqrt(disc), roots s / A and C / s.  A root is kept as w
+// when |w| <= 1, else as v = 1 / w, so nothing overflows; the key
+//     k = w / (1 + w)        (0 <= w <= 1)      k = 1 / (1 + v)      (0 <= v <= 1)
+//     k = 2 + w / (1 - w)    (-1 <= w < 0)      k = 2 + 1 / (v - 1)  (-1 <= v < 0)
+// orders travel angles over one revolution (theta in [0, 2 pi) <-> k in [0, 2))
+// with no trigonometric call; only the final hit is turned into an angle.
+//
+// A disk is screened in FP32; for the disks that may cross, the quantities
+// that would cancel are formed from FP64 differences: q = (|P - S|^2 - r^2) / R^2
+// (small when the start is near the disk), g = |P - C|^2 / R^2 - 1 and
+// A = 4 (1 - b) + q; the discriminant is written as
+// (g - (-2 rr + rr^2)) ((2 rr + rr^2) - g), rr = r / R — the product of the two
+// CirclesCross margins (details.h:141-151), zero only at tangency.
+//
+// The reference's valid angle range starts the first cell at the raw particle
+// angle phi (lifetime.h:101-106), which is negative for the incoherent
+// window's first quadrant, and ends a cell without a successor at that raw
+// angle too; AngleInRange (details.h:119-136) and the "next arc holds the
+// start angle" test of GetNextCell (cell_grid.h:177-180) then accept other
+// arcs than the one travelled.  With m = -phi > 0 these are, in travel angle:
+//   first cell, exit at theta_e:  counter-clockwise: no next cell when
+//     theta_e < m, else disks count on [m, theta_e); clockwise: (0, theta_e)
+//     when theta_e < 2 pi - m, else none;
+//   later cell entered at theta_i, no next cell:  counter-clockwise:
+//     (theta_i, 2 pi) when theta_i >= m, else none; clockwise: (theta_i, 2 pi - m)
+//     when theta_i < 2 pi - m, else (0, 2 pi - m) and (theta_i, 2 pi);
+//   first cell without a next cell: the whole revolution.
+// They are reproduced here, with m and 2 pi - m as keys.
+//
+// Cases FP32 cannot decide raise ES_F_RECOMPUTE and end the trace: the item is
+// traced again in FP64.  They are: a disk within FP32 error of tangency to the
+// orbit, an edge line within error of tangency, an exit within error of a
+// cell corner (the corner snap of cell_grid.h:162-175) or of the entry point
+// (DifferentPoint, :74-77), a start point on a cell edge, a crossing within
+// error of its cell's range ends, an exit or a hit within error of a full
+// revolution or of the angles m, 2 pi - m above, a hit within the FP64
+// path's angle floor of the start, and a lifetime within 1e-5 of the bound
+// time (lifetime.h:142-146).
+#pragma once
+
+#include "es_trace.h"
+
+#define ES_F_RECOMPUTE 8
+
+// One orbit in the start point's frame.
+struct Orb32 {
+    double sx, sy;          // start point (FP64)
+    double inv_r;           // 1 / R
+    double nxd, nyd;        // start -> centre, unit (FP64, for the crossing disks)
+    double txd, tyd;        // direction of travel at the start (FP64)
+    double r2;              // r^2
+    float nx, ny, tx, ty;   // the same in FP32
+    float rr, rr2;          // r / R, (r / R)^2
+    float km, km2;          // keys of theta = m and 2 pi - m (quirk only)
+    bool ccw;               // counter-clockwise orbit
+    bool quirk;             // raw start angle phi < 0 (m = -phi)
+};
+
+// A root of the w-quadratic: z = w (inv == false, |z| <= 1) or z = 1 / w.
+struct Root32 {
+    float z;
+    bool inv;
+};
+
+__device__ __forceinline__ float t32_key(Root32 r)
+{
+    if (!r.inv) return (r.z >= 0.0f) ? r.z / (1.0f + r.z) : 2.0f + r.z / (1.0f - r.z);
+    return (r.z > 0.0f) ? 1.0f / (1.0f + r.z) : 2.0f + 1.0f / (r.z - 1.0f);
+}
+
+// Travel angle of a root: 2 atan(w) (+2 pi when w < 0), or pi - 2 atan(v).
+__device__ __forceinline__ double t32_theta(Root32 r)
+{
+    const double a = 2.0 * (double)atanf(r.z);
+    if (r.inv) return ES_PI - a;
+    return (r.z >= 0.0f) ? a : ES_PI2 + a;
+}
+
+// Coordinate along (n_c, t_c) of the orbit point at root r:
+// (2 n w^2 + 2 t w) / (1 + w^2)  or  (2 n + 2 t v) / (1 + v^2).
+__device__ __forceinline__ float t32_coord(Root32 r, float n_c, float t_c)
+{
+    const float z2 = r.z * r.z;
+    const float num = r.inv ? (2.0f * n_c + 2.0f * t_c * r.z) : (2.0f * n_c * z2 + 2.0f * t_c * r.z);
+    return num / (1.0f + z2);
+}
+
+// Roots of A w^2 + 2 Bh w + C = 0 given disc = Bh^2 - A C >= 0.  Returns
+// false when s = 0 (a double root at w = 0 or a degenerate quadratic).
+__device__ __forceinline__ bool t32_roots(float A, float Bh, float C, float disc, Root32& r1, Root32& r2)
+{
+    const float h = sqrtf(disc > 0.0f ? disc : 0.0f);
+    const float s = (Bh >= 0.0f) ? (-Bh - h) : (-Bh + h);
+    if (s == 0.0f) return false;
+    // root 1 = s / A, root 2 = C / s
+    if (fabsf(s) <= fabsf(A)) {
+        r1.z = s / A;
+        r1.inv = false;
+    } else {
+        r1.z = A / s;
+        r1.inv = true;
+    }
+    if (fabsf(C) <= fabsf(s)) {
+        r2.z = C / s;
+        r2.inv = false;
+    } else {
+        r2.z = s / C;
+        r2.inv = true;
+    }
+    return true;
+}
+
+struct Trace32 {
+    int cx, cy;             // current cell
+    int in_edge;            // edge entered through (0 top, 1 bottom, 2 right, 3 left), -1 at the start
+    float k_in;             // key of the entry point
+    float best_k;           // key of the first hit so far (2 when none)
+    Root32 best;            // its root
+    int cells, tested, edge_hits, cross_calls;
+};
+
+__device__ __forceinline__ void t32_begin(Trace32& st, int c0x, int c0y)
+{
+    st.cx = c0x;
+    st.cy = c0y;
+    st.in_edge = -1;
+    st.k_in = 0.0f;
+    st.best_k = 2.0f;
+    st.best.z = 0.0f;
+    st.best.inv = false;
+    st.cells = st.tested = st.edge_hits = st.cross_calls = 0;
+}
+
+// Error bands of the flags: relative, a few hundred FP32 ulps; absolute, in
+// key units (half a small travel angle), the FP64 path's own floor — its
+// angles come from asin(y / R), off by up to 1e-8 rad near y = +-R
+// (details.h:93-107), so it may decide either way within that distance.
+#define T32_REL 2e-5f
+#define T32_ABS 1e-8f
+
+__device__ __forceinline__ bool t32_near(float a, float b, float band) { return fabsf(a - b) <= band; }
+
+// One cell.  Returns true when the trace has ended; *flags then holds
+// ES_F_ESCAPED / ES_F_CAP / ES_F_RECOMPUTE, and *theta the travel angle of
+// the hit (0 without one).
+__device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int cap, Trace32& st, int& flags,
+                                         double& theta)
+{
+    const double L = g.L;
+    const double x0 = g.start + (double)st.cx * L, y0 = g.start + (double)st.cy * L;
+    // cell edges in the start frame, in units of R
+    const float xl = (float)((x0 - o.sx) * o.inv_r), xh = (float)((x0 + L - o.sx) * o.inv_r);
+    const float yl = (float)((y0 - o.sy) * o.inv_r), yh = (float)((y0 + L - o.sy) * o.inv_r);
+    const float k_in = st.k_in;
+    const bool first = st.in_edge < 0;
+    const float band_in = T32_REL * k_in + T32_ABS;
+
+    // ---- exit of the cell: smallest key after the entry over the 4 edges
+    float k_out = 2.0f;
+    int e_out = -1;
+    float k_second = 2.0f;   // best key of another edge, for the corner test
+    float k_flag = 3.0f;     // smallest key of an event FP32 cannot decide
+    bool bad = false;
+#pragma unroll 1
+    for (int e = 0; e < 4; e++) {
+        const bool vert = (e >= 2);                  // right / left edges are x = const
+        const float c = (e == 0) ? yh : (e == 1) ? yl : (e == 2) ? xh : xl;
+        const float n_c = vert ? o.nx : o.ny, t_c = vert ? o.tx : o.ty;     // across the edge
+        const float n_o = vert ? o.ny : o.nx, t_o = vert ? o.ty : o.tx;     // along the edge
+        const float lo = vert ? yl : xl, hi = vert ? yh : xh;
+        const float cband = T32_REL * (fabsf(lo) + fabsf(hi));
+        // n_c (1 - cos) + t_c sin = c  ->  (2 n_c - c) w^2 + 2 t_c w - c = 0
+        const float A = 2.0f * n_c - c;
+        const float disc = t_c * t_c + c * (2.0f * n_c - c);
+        const float dscale = t_c * t_c + fabsf(2.0f * n_c * c) + c * c;
+        if (disc <= T32_REL * dscale) {
+            if (disc > -T32_REL * dscale) {
+                // near tangency: undecidable if the touching point lies on the edge
+                Root32 tp;
+                tp.inv = fabsf(t_c) > fabsf(A);
+                tp.z = tp.inv ? (A / -t_c) : (-t_c / A);
+                const float a = t32_coord(tp, n_o, t_o);
+                const float kt = t32_key(tp);
+                if (a > lo - cband && a < hi + cband && kt > k_in - band_in) k_flag = fminf(k_flag, kt);
+            }
+            continue;
+        }
+        Root32 rt[2];
+        if (!t32_roots(A, t_c, -c, disc, rt[0], rt[1])) {
+            bad = true;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float a = t32_coord(rt[k], n_o, t_o);
+            if (!(a > lo - cband && a < hi + cband)) continue;
+            const float kk = t32_key(rt[k]);
+            if (kk <= k_in - band_in) continue;  // behind the entry: after a full revolution
+            if (a < lo + cband || a > hi - cband) {  // within error of a corner
+                k_flag = fminf(k_flag, kk);
+                continue;
+            }
+            st.edge_hits += 1;
+            if (fabsf(kk - k_in) <= band_in) {
+                if (e != st.in_edge) k_flag = fminf(k_flag, kk);  // another edge at the entry point
+                continue;                                         // else: the entry point itself
+            }
+            if (kk < k_out) {
+                if (e != e_out) k_second = k_out;
+                k_out = kk;
+                e_out = e;
+            } else if (e != e_out && kk < k_second) {
+                k_second = kk;
+            }
+        }
+    }
+    const float band_out = T32_REL * k_out + T32_ABS;
+    if (e_out >= 0 && k_second - k_out <= band_out) bad = true;  // two edges at one point
+    if (k_flag <= k_out + band_out) bad = true;                 // an undecidable event before the exit
+    if (e_out >= 0 && k_out >= 2.0f - 1e-5f) bad = true;        // exit near a full revolution
+    const int ncx = st.cx + ((e_out == 2) ? 1 : (e_out == 3) ? -1 : 0);
+    const int ncy = st.cy + ((e_out == 0) ? 1 : (e_out == 1) ? -1 : 0);
+    bool avail = e_out >= 0 && ncx >= 0 && ncx < g.cpr && ncy >= 0 && ncy < g.cpr;
+
+    // ---- the reference's accepted travel angles in this cell: (a1, b1) and (a2, b2)
+    float a1 = k_in, b1 = avail ? k_out : 2.0f, a2 = 2.0f, b2 = 2.0f;
+    if (o.quirk) {
+        const float bm = T32_REL * o.km + T32_ABS, bm2 = T32_REL * o.km2;
+        if (first) {
+            if (avail && o.ccw) {
+                if (t32_near(k_out, o.km, bm)) bad = true;
+                if (k_out < o.km) avail = false;  // the exit arc holds the raw start angle
+            }
+            if (!avail) {
+                a1 = 0.0f;  // raw angle at both ends: the whole revolution
+                b1 = 2.0f;
+            } else if (o.ccw) {
+                a1 = o.km;
+            } else {
+                if (t32_near(k_out, o.km2, bm2)) bad = true;
+                if (k_out >= o.km2) a1 = b1 = 2.0f;
+            }
+        } else if (!avail) {
+            if (o.ccw) {
+                if (t32_near(k_in, o.km, bm)) bad = true;
+                if (k_in < o.km) a1 = b1 = 2.0f;
+            } else {
+                if (t32_near(k_in, o.km2, bm2)) bad = true;
+                if (k_in < o.km2) {
+                    b1 = o.km2;
+                } else {
+                    a2 = k_in;
+                    a1 = 0.0f;
+                    b1 = o.km2;
+                }
+            }
+        }
+    }
+    if (bad) {
+        flags = ES_F_RECOMPUTE;
+        theta = 0.0;
+        return true;
+    }
+
+    // ---- impurities of the cell: crossings with keys in the accepted ranges
+    const int ci = st.cy * g.cpr + st.cx;
+    const int s = (ci > 0) ? g.cell_end[ci - 1] : 0;
+    const int en = g.cell_end[ci];
+    st.cells += 1;
+    st.tested += (en - s);
+    const float glo = -2.0f * o.rr + o.rr2, ghi = 2.0f * o.rr + o.rr2;
+    const float b_hi = fmaxf(b1, b2) + band_out;
+    float best_k = st.best_k;
+    Root32 best = st.best;
+    for (int i = s; i < en; i += 4) {
+        double2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = (i + u < en) ? g.ixy[i + u] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i + u >= en) continue;
+            const double dx = p[u].x - o.sx, dy = p[u].y - o.sy;
+            // FP32 screen of CirclesCross: |P - C|^2 / R^2 - 1 = |p|^2 - 2 p.n
+            {
+                const float px = (float)(dx * o.inv_r), py = (float)(dy * o.inv_r);
+                const float b0 = px * o.nx + py * o.ny;
+                const float pp0 = px * px + py * py;
+                const float g0 = pp0 - 2.0f * b0;
+                const float band0 = 1e-5f * (pp0 + 2.0f * fabsf(b0)) + 1e-12f;
+                if (g0 < glo - band0 || g0 > ghi + band0) continue;
+            }
+            // candidates: the quantities that cancel, from FP64 differences
+            const double ppn = (dx * dx + dy * dy) * (o.inv_r * o.inv_r);
+            const double bn = (dx * o.nxd + dy * o.nyd) * o.inv_r;
+            const float a = (float)((dx * o.txd + dy * o.tyd) * o.inv_r);  // along the travel
+            const double qd = (dx * dx + dy * dy - o.r2) * (o.inv_r * o.inv_r);
+            const float q = (float)qd;
+            const float gg = (float)(ppn - 2.0 * bn);       // |P - C|^2 / R^2 - 1
+            const float A = (float)((4.0 - 4.0 * bn) + qd);  // 4 (1 - b) + q (small near theta = pi)
+            const float gband = T32_REL * (fabsf(gg) + 2.0f * o.rr) + 1e-14f * (float)(ppn + 2.0 * fabs(bn));
+            if (fabsf(gg - glo) <= gband || fabsf(gg - ghi) <= gband) {
+                // within error of tangency: undecidable if the touching point is in range
+                Root32 tp;
+                tp.inv = fabsf(2.0f * a) > fabsf(A);
+                tp.z = tp.inv ? (A / (2.0f * a)) : (2.0f * a / A);
+                const float kt = t32_key(tp);
+                if (kt > k_in - band_in && kt < b_hi) bad = true;
+                continue;
+            }
+            if (!(gg > glo && gg < ghi)) continue;
+            st.cross_calls += 1;
+            if (!(q > 0.0f)) {  // the start lies outside every disk (k_positions)
+                bad = true;
+                continue;
+            }
+            // |X - P|^2 = rr^2  ->  A w^2 - 4 a w + q = 0, disc = (2a)^2 - A q
+            const float disc = (gg - glo) * (ghi - gg);
+            Root32 rt[2];
+            if (!t32_roots(A, -2.0f * a, q, disc, rt[0], rt[1])) {
+                bad = true;
+                continue;
+            }
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const float kk = t32_key(rt[k]);
+                const bool in = (kk > a1 && kk < b1) || (kk > a2 && kk < b2);
+                // A crossing within error of a range end (a cell boundary) may
+                // belong to either cell, and FP32 keys can tie there.
+                const float rb = 2e-6f * kk + T32_ABS;
+                const bool at_end = (a1 > 0.0f && t32_near(kk, a1, rb)) || (b1 < 2.0f && t32_near(kk, b1, rb)) ||
+                                    (a2 > 0.0f && a2 < 2.0f && t32_near(kk, a2, rb)) ||
+                                    (b2 < 2.0f && t32_near(kk, b2, rb));
+                if (at_end && kk < best_k + rb) bad = true;
+                if (in && kk < best_k) {
+                    best_k = kk;
+                    best = rt[k];
+                }
+            }
+        }
+    }
+    st.best_k = best_k;
+    st.best = best;
+    if (!bad && o.quirk && best_k < 2.0f &&
+        (t32_near(best_k, o.km, T32_REL * o.km + T32_ABS) || t32_near(best_k, o.km2, T32_REL * o.km2)))
+        bad = true;  // a hit at the quirk angles
+    if (!bad && best_k < 2.0f * T32_ABS) bad = true;  // a hit within the FP64 floor of the start
+    if (bad) {
+        flags = ES_F_RECOMPUTE;
+        theta = 0.0;
+        return true;
+    }
+    if (best_k < 2.0f) {
+        flags = (best_k >= 2.0f - 1e-5f) ? ES_F_RECOMPUTE : 0;
+        theta = t32_theta(best);
+        return true;
+    }
+    if (!avail) {
+        flags = ES_F_ESCAPED;
+        theta = 0.0;
+        return true;
+    }
+    if (st.cells >= cap) {
+        flags = ES_F_CAP;
+        theta = 0.0;
+        return true;
+    }
+    st.cx = ncx;
+    st.cy = ncy;
+    st.in_edge = (e_out == 0) ? 1 : (e_out == 1) ? 0 : (e_out == 2) ? 3 : 2;  // the shared edge, seen from the new cell
+    st.k_in = k_out;
+    return false;
+}

@@ -33,6 +33,8 @@ extern "C" {
 #define ES_ERR_NO_DEVICE (-7)      /* no HIP device */
 
 #define ES_PRECISION_FP64 0        /* FP64 throughout; lifetimes bit-identical to the CPU restatement */
+#define ES_PRECISION_MIXED 1       /* FP32 trace, the cases FP32 cannot decide traced again in FP64;
+                                      start data, Simpson accumulation and sigma sums stay FP64 */
 
 typedef struct es_solver es_solver;
 
@@ -93,6 +95,7 @@ typedef struct es_metrics {
     int64_t sigma_evals;              /* GetSigma evaluations with a non-zero truncated lifetime */
     int64_t cross_exact;              /* GetFirstCrossTime calls evaluated exactly; the others were
                                          screened out by a bound that proves they cannot lower the minimum */
+    int64_t fp64_recomputed;          /* ES_PRECISION_MIXED: particles traced again in FP64 */
 } es_metrics;
 
 /* Simulation(positions_per_row, particles_per_quadrant, GridInformation) (Simulation.h:131-146). */
@@ -101,7 +104,7 @@ typedef struct es_solver_desc {
     int32_t particles_per_quadrant;
     es_grid_info grid_info;           /* region_size, region_extends, cell_size of sample 0's grid;
                                          cell_size 0: taken from the first grid set on the solver */
-    int32_t precision;                /* ES_PRECISION_FP64 */
+    int32_t precision;                /* ES_PRECISION_FP64 or ES_PRECISION_MIXED */
     int32_t device;                   /* HIP device ordinal */
 } es_solver_desc;
 
@@ -202,6 +205,8 @@ int es_batch_set_shape(es_batch* b, const es_batch_shape* shape);
 int es_batch_get_shape(es_batch* b, es_batch_shape* shape);
 /* enable != 0: record the stage launches once in a hipGraph and replay it. */
 int es_batch_set_capture(es_batch* b, int32_t enable);
+/* ES_PRECISION_FP64 (default) or ES_PRECISION_MIXED for every entry. */
+int es_batch_set_precision(es_batch* b, int32_t precision);
 /*
  * Runs every entry: out holds, entry by entry, n_fields * n_temperatures
  * sigmas (field-major); metrics (or NULL) one es_metrics per (entry, field);

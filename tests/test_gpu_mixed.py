@@ -1,0 +1,103 @@
+"""Mixed precision (FP32 trace, undecidable cases traced again in FP64) against
+the FP64 path, which is bit-exact to the CPU oracle (needs an MI355X).
+
+Stated errors against FP64, checked here on every case: each lifetime t within
+1e-5 t + 3e-8 / omega — the FP32 error of the travel angle plus the FP64
+path's own floor (its angles come from asin(y / R), which near y = +-R is
+off by up to 1e-8 rad, test_oracle_kats.py); a missed or extra hit would be
+an O(1) error.  sigma_xx and sigma_xy within 1e-6 of |sigma|; the work
+counters within 0.1 %.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LT_REL = 1e-5
+ANGLE_FLOOR = 3e-8  # rad
+SIGMA_REL = 1e-6
+E_CHARGE, M_EFF = 1.60217646e-19, 5.0 * 9.10938188e-31  # constants.h:23-26
+
+
+@pytest.fixture(scope="module")
+def es():
+    import elasticscattering_amd as es_mod
+
+    es_mod.load_library()
+    return es_mod
+
+
+def _compare(es, setup, ppr, ppq, tau, alpha, cw, speed, coherent, fields, temps, incoherent_tau=0.0):
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with es.Simulation(ppr, ppq, precision=prec) as sim:
+            setup(sim)
+            sim.init_sample(tau, alpha, cw, speed, coherent, incoherent_tau)
+            res[prec] = [sim.compute_sigmas(B, temps, want_lifetimes=True) for B in fields]
+    worst = 0.0
+    stats = []
+    for B, (s64, m64, l64), (s32, m32, l32) in zip(fields, res["fp64"], res["mixed"]):
+        d = np.abs(l32 - l64)
+        rel = d / np.maximum(np.abs(l64), 1e-300)
+        omega = E_CHARGE * B / M_EFF
+        bad = np.flatnonzero(d > LT_REL * np.abs(l64) + ANGLE_FLOOR / omega)
+        nz = l64 != 0
+        stats.append((B, m32["fp64_recomputed"], m32["traced_particles"], float(rel[nz].max()) if nz.any() else 0.0,
+                      bad.size))
+        assert bad.size == 0, (B, bad[:5], l64[bad[:5]], l32[bad[:5]])
+        for (x64, y64), (x32, y32) in zip(s64, s32):
+            norm = math.hypot(x64, y64)
+            e = max(abs(x32 - x64), abs(y32 - y64)) / norm if norm > 0 else abs(x32) + abs(y32)
+            worst = max(worst, e)
+            assert e <= SIGMA_REL, (B, x64, x32, y64, y32)
+        for k in ("cells_passed", "impurities_tested", "cross_calls", "particles_escaped"):
+            assert abs(m32[k] - m64[k]) <= 1e-3 * max(m64[k], 1000), (k, m32[k], m64[k])
+        assert m32["particles_inside_impurity"] == m64["particles_inside_impurity"]
+        assert m64["fp64_recomputed"] == 0
+    print("B, recomputed, traced, max lifetime rel err, bad:", stats, "worst sigma err", worst)
+    return worst
+
+
+@pytest.mark.parametrize("cw", [1, 0])
+@pytest.mark.parametrize("coherent", [0, 1])
+def test_mixed_file_field(es, cw, coherent):
+    announce(f"mixed vs fp64: test_impurities.dat cw={cw} coherent={coherent}")
+    x, y = O.read_points(os.path.join(GOLDEN, "test_impurities.dat"))
+    _compare(es, lambda s: s.set_impurities(x, y, 1e-7, 1e-7, 2e-9, 10), 31, 7, 1.0, 0.3, cw, 1.67834e5, coherent,
+             [0.01, 12.008, 60.0], [0.01, 0.1, 1, 10, 100])
+
+
+def test_mixed_small_workload(es):
+    from elasticscattering_amd.workloads import SMALL_FP64 as w, region
+
+    rs, ext = region(w)
+    announce("mixed vs fp64: small workload")
+    _compare(es, lambda s: s.generate_impurities(101, rs, ext, w["impurity_radius"], count=w["impurity_count"],
+                                                 cells_per_row=w["cells_per_row"]),
+             w["positions_per_row"], w["particles_per_quadrant"], w["tau"], w["alpha"], 1, w["particle_speed"], 0,
+             [1.0, 20.0, 60.0], [0.0], w["incoherent_tau"])
+
+
+def test_mixed_sparse_long_traces(es):
+    announce("mixed vs fp64: sparse field, long traces")
+    _compare(es, lambda s: s.generate_impurities(5, 1e-6, 5e-7, 2e-8, count=300, target_cell_population=3),
+             24, 9, 1e-12, 0.3, 1, 7e5, 0, [0.1, 1.0, 5.0, 40.0], [2.0, 40.0, 100.0])
+
+
+def test_mixed_production_field(es):
+    announce("mixed vs fp64: 60000 impurities, 128x128 cells, 64x64 positions")
+    _compare(es, lambda s: s.generate_impurities(101, 1e-6, 5e-7, 1.1e-8, count=60000, cells_per_row=128),
+             64, 17, 1e-12, math.pi / 4, 1, 7e5, 0, [1.0, 6.3, 40.0], [2.0, 40.0, 100.0])
+
+
+def test_mixed_low_density_many_cells(es):
+    announce("mixed vs fp64: 400 impurities on 2e-6 m, 128x128 cells")
+    _compare(es, lambda s: s.generate_impurities(7, 1e-6, 5e-7, 1e-8, count=400, cells_per_row=128),
+             48, 9, 1e-12, 0.5, 0, 7e5, 1, [0.1, 2.0, 30.0], [10.0])

@@ -478,3 +478,25 @@ def test_oracle_builds_agree_end_to_end():
     assert np.mean(rel > 1e-9) < 1e-3
     for s1, s2 in zip(out[0][0], out[1][0]):
         assert abs(s1.xx_comp - s2.xx_comp) <= 1e-8 * abs(s1.xx_comp)
+
+
+def test_get_angle_error_floor_near_top_of_orbit(oracle_esm):
+    """GetAngle (details.h:93-107) takes asin(y / R): near y = +-R its angle
+    carries an absolute error up to ~sqrt(eps) ~ 1e-8 rad, elsewhere ~1 ulp.
+    Travel angles of the FP64 path inherit this floor; it bounds how closely
+    another formulation (the mixed-precision trace) can agree with it."""
+    R = 2e-5
+    near, far = 0.0, 0.0
+    for k in range(1, 2000):
+        for base, is_near in ((math.pi / 2, True), (0.3, False)):
+            ang = base + (k - 1000) * 1e-9
+            x, y = R * math.cos(ang), R * math.sin(ang)
+            a = oracle_esm.get_angle(x, y, 0.0, 0.0, R, 1)
+            exact = (math.atan2(y, x) + math.pi / 2) % (2 * math.pi)
+            d = abs((a - exact + math.pi) % (2 * math.pi) - math.pi)
+            if is_near:
+                near = max(near, d)
+            else:
+                far = max(far, d)
+    assert 1e-9 < near <= 1.5e-8
+    assert far <= 1e-15
