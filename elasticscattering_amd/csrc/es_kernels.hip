@@ -673,7 +673,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 }
 
 template <bool CW>
-__global__ __launch_bounds__(kBlock) void k_trace32(TraceArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_trace32(TraceArgs a)
 {
     trace32_body<CW>(a);
 }
