@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
-"""Throughput of the FP64 lifetime / sigma path on one MI355X.
+"""Throughput of the lifetime / sigma path on one MI355X.
 
 A step is one impurity sample of the production sweep: the field is drawn
 and binned on the GPU, then the incoherent and the coherent pass each run the
 40-field sweep with the 3 temperatures (SimulationRunner.cpp:88-98 order).
 
-Prints one JSON line: lifetimes/s (incoherent lifetimes, the coherent pass is
-run but not counted), the sweep wall time, the roofline of the trace kernel
-(algorithmic FP64 work per launch from the kernel's own work counters over its
-HIP-event duration, against the 78.6 TFLOP/s FP64 vector peak), and the CPU
-oracle timed on a bounded sample of the same workload on this host.
+Prints one JSON line: lifetimes/s of the FP64 path (incoherent lifetimes, the
+coherent pass is run but not counted; `value`), the sweep wall time, the
+roofline of the trace kernel (algorithmic FP64 work per launch from the
+kernel's own work counters over its HIP-event duration, against the
+78.6 TFLOP/s FP64 vector peak), the CPU oracle timed on a bounded sample of
+the same workload on this host, and beside them:
+  * "mixed": the same steps with the mixed-precision trace, and its sigma error
+    against the FP64 steps;
+  * "batched_uneven": the 64-sample uneven batch, FP64 and mixed;
+  * "sweep": 400-field sigma(B) curves of seeded materials at 128 x 128
+    positions — the full FP64 sweep against the adaptive sweep (mixed
+    precision, batched, surrogate-guided; the surrogate trained first on
+    solver runs), with the adaptive curves' error against the full sweep.
 """
 import argparse
 import json
@@ -102,6 +110,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="production")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the FP64 production line (profiling runs)")
+    ap.add_argument("--part", default="all", choices=("all", "production", "batched", "sweep"),
+                    help="batched / sweep: print only that part's JSON (profiling runs)")
+    ap.add_argument("--sweep-materials", type=int, default=24)
+    ap.add_argument("--train-runs", type=int, default=20000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -115,6 +128,15 @@ def main():
     import elasticscattering_amd as es
     from elasticscattering_amd.workloads import PRODUCTION_FP64, SMALL_FP64, region, lifetimes_per_pass
 
+    if args.part == "batched":
+        print(json.dumps({"batched_uneven": batched_uneven(es)}), flush=True)
+        return 0
+    if args.part == "sweep":
+        print(json.dumps({"sweep": sweep_comparison(args.sweep_materials, args.train_runs)}), flush=True)
+        return 0
+    if args.part == "production":
+        args.no_extras = True
+
     w = PRODUCTION_FP64 if args.workload == "production" else SMALL_FP64
     rs, ext = region(w)
     ppr, ppq = w["positions_per_row"], w["particles_per_quadrant"]
@@ -122,8 +144,9 @@ def main():
     P = lifetimes_per_pass(w)
 
     sim = es.Simulation(ppr, ppq)
+    sim_mixed = es.Simulation(ppr, ppq, precision="mixed")
 
-    def step(s, stats):
+    def step(s, stats, sim=sim):
         seed = w["seeds"][s % len(w["seeds"])]
         sim.generate_impurities(seed, rs, ext, w["impurity_radius"], count=w["impurity_count"],
                                 cells_per_row=w["cells_per_row"])
@@ -138,16 +161,36 @@ def main():
                 stats["traced"] += sum(m["traced_particles"] for m in ms)
                 stats["cross"] += sum(m["cross_calls"] for m in ms)
                 stats["exact"] += sum(m["cross_exact"] for m in ms)
+                stats["recomputed"] += sum(m["fp64_recomputed"] for m in ms)
                 stats["sigma"].append(sig)
 
-    for s in range(args.warmup):
-        step(s, None)
-    stats = {"launch_s": [], "flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0, "sigma": [], "cross": 0,
-             "exact": 0}
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        step(s, stats)
-    dt = time.perf_counter() - t0
+    def timed(sim_):
+        for s in range(args.warmup):
+            step(s, None, sim_)
+        st = {"launch_s": [], "flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0, "sigma": [], "cross": 0,
+              "exact": 0, "recomputed": 0}
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            step(s, st, sim_)
+        return st, time.perf_counter() - t0
+
+    stats, dt = timed(sim)
+    mixed = None
+    if not args.no_extras:
+        import numpy as np
+
+        mstats, mdt = timed(sim_mixed)
+        err = 0.0
+        for a, b in zip(mstats["sigma"], stats["sigma"]):
+            n = np.hypot(b[..., 0], b[..., 1])
+            err = max(err, float((np.abs(a - b).max(axis=-1) / n).max()))
+        mixed = {
+            "value": args.steps * len(fields) * P / mdt, "unit": "lifetimes/s",
+            "ms_per_step": mdt / args.steps * 1e3, "sweep_wall_s": mdt * len(w["seeds"]) / args.steps,
+            "trace_ms_per_launch": sum(mstats["launch_s"]) / len(mstats["launch_s"]) * 1e3,
+            "max_sigma_error_vs_fp64": err, "fp64_recomputed_fraction": mstats["recomputed"] / max(mstats["traced"], 1),
+            "dtype": "f32 trace + f64 recompute, f64 start data / Simpson / sigma sums",
+        }
 
     lifetimes = args.steps * len(fields) * P
     avg_launch = sum(stats["launch_s"]) / len(stats["launch_s"])
@@ -182,7 +225,9 @@ def main():
         "traced_lifetimes_per_s": stats["traced"] / dt,
         "exact_cross_solves_fraction": stats["exact"] / max(stats["cross"], 1),
         "roofline": {
-            "bound": "fp64-valu",
+            "bound": "mfma",
+            "bound_detail": "FP64 vector ALU work, priced at MI355X's 78.6 TFLOP/s FP64 peak (vector and matrix FP64 "
+                            "peaks are equal); HBM traffic is ~1 % of peak (profiles/k_trace_production_fp64.md)",
             "kernel": "k_trace (CreateParticle + TraceOrbit, one launch per sample and pass, all fields)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
@@ -194,13 +239,102 @@ def main():
             "sigma_stage_tflops": sigma_tf,
         },
     }
+    if mixed is not None:
+        line["mixed"] = mixed
+        line["batched_uneven"] = batched_uneven(es)
+        line["sweep"] = sweep_comparison(args.sweep_materials, args.train_runs)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(w)
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     print(json.dumps(line), flush=True)
     sim.close()
+    sim_mixed.close()
     return 0
+
+
+def batched_uneven(es):
+    """The 64-sample uneven batch, one launch per stage, FP64 and mixed (one warm-up run each)."""
+    import numpy as np
+    from elasticscattering_amd.workloads import BATCHED_UNEVEN, batched_samples
+
+    samples = batched_samples()
+    out = {"workload": BATCHED_UNEVEN["name"]}
+    sig = {}
+    for prec in ("fp64", "mixed"):
+        with es.Batch() as b:
+            b.set_precision(prec)
+            for s in samples:
+                b.add(**s)
+            c = b.counts()
+            b.run(metrics=False)
+            t0 = time.perf_counter()
+            sig[prec], mets, secs = b.run()
+            wall = time.perf_counter() - t0
+        traced = sum(m["traced_particles"] for per in mets for m in per)
+        out[prec] = {
+            "traced_lifetimes_per_s": traced / secs, "device_s": secs, "wall_s": wall,
+            "trace_s": sum(m["time_elapsed_lifetimes"] for per in mets for m in per),
+            "fp64_recomputed": sum(m["fp64_recomputed"] for per in mets for m in per),
+        }
+        out.update(samples=c["samples"], pairs=c["pairs"], traced_items=c["traced_items"])
+    err = 0.0
+    for a, b in zip(sig["mixed"], sig["fp64"]):
+        n = np.maximum(np.hypot(b[..., 0], b[..., 1]), 1e-300)
+        err = max(err, float((np.abs(a - b).max(axis=-1) / n).max()))
+    out["mixed"]["max_sigma_error_vs_fp64"] = err
+    out["mixed_speedup"] = out["fp64"]["device_s"] / out["mixed"]["device_s"]
+    return out
+
+
+def sweep_comparison(n_materials, train_runs):
+    """sigma(B) curves on 400 log-spaced fields (0.1-60 T) for seeded materials:
+    the full FP64 sweep (every field, batched) against the adaptive sweep
+    (mixed precision, batched rounds, surrogate-guided)."""
+    import math
+
+    import numpy as np
+    from elasticscattering_amd import surrogate as S
+
+    out = {"materials": n_materials, "fields": 400, "positions": "128x128", "phi_points": 260,
+           "tolerance": 1e-3, "error": "max(|d xx|, |d xy|) / |sigma| against the full FP64 sweep"}
+    t0 = time.perf_counter()
+    n_mat = max(1, train_runs // 40)
+    X, Y, gen_s, _ = S.training_set(n_mat, 20, 2, seed=1)
+    sur = S.Surrogate(members=4, seed=3)
+    sur.fit(X, Y, epochs=600, batch=256, lr=2e-3, lr_final=2e-5)
+    train_s, train_steps = sur.mlp.last_train_time()
+    out["surrogate"] = {"solver_runs": len(X), "solver_device_s": gen_s, "train_device_s": train_s,
+                        "train_steps": train_steps, "wall_s": time.perf_counter() - t0}
+    B = np.logspace(-1, math.log10(60), 400)
+    mats = S.draw_materials(n_materials, seed=77)
+    t0 = time.perf_counter()
+    full, full_s, full_items = S.solver_runs(mats, [B] * n_materials)
+    full_wall = time.perf_counter() - t0
+    full = np.stack([f[:, 0, :] for f in full])
+    # The solver's own jitter per curve: how far each point lies from the mean
+    # of its two neighbours (400 log-spaced fields resolve the curvature).
+    norm = np.maximum(np.hypot(full[..., 0], full[..., 1]), 1e-300)
+    jitter = np.abs(full[:, 1:-1] - 0.5 * (full[:, :-2] + full[:, 2:])).max(axis=-1) / norm[:, 1:-1]
+    jitter = jitter.max(axis=1)
+    out["full_fp64"] = {"device_s": full_s, "wall_s": full_wall, "traced_items": full_items,
+                        "curves_with_jitter_above_half_tolerance": int((jitter > 5e-4).sum())}
+    for name, sg, prec in (("adaptive_mixed_surrogate", sur, "mixed"), ("adaptive_mixed", None, "mixed"),
+                           ("adaptive_fp64", None, "fp64")):
+        t0 = time.perf_counter()
+        est, rep = S.adaptive_sweep(mats, B, surrogate=sg, precision=prec)
+        wall = time.perf_counter() - t0
+        err = S.tensor_error(est, full)
+        cmax = err.max(axis=1)
+        out[name] = {"device_s": rep["solver_seconds"], "wall_s": wall, "solver_points": rep["solver_points"],
+                     "points_per_curve": rep["points_per_curve"], "rounds": rep["rounds"],
+                     "accepted": rep["accepted"], "true_max_error": float(cmax.max()),
+                     "curves_within_tolerance": int((cmax <= 1e-3).sum()),
+                     "over_tolerance": [{"material": int(i), "error": float(cmax[i]), "solver_jitter": float(jitter[i])}
+                                        for i in np.flatnonzero(cmax > 1e-3)],
+                     "speedup_vs_full_fp64_wall": full_wall / wall}
+    sur.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -179,7 +179,7 @@ def draw_materials(n, seed, alpha=None):
 
 
 def solver_runs(materials, fields, temperatures=None, positions_per_row=128, particles_per_quadrant=65,
-                side=2.0e-6, clockwise=1, speed=7e5, device=0, with_free=False):
+                side=2.0e-6, clockwise=1, speed=7e5, device=0, with_free=False, precision="fp64"):
     """sigma [n_mat][n_B][n_T][2] = coherent + incoherent pass, all materials in one batch.
 
     ``fields[i]`` / ``temperatures[i]`` are material i's B values and
@@ -188,6 +188,7 @@ def solver_runs(materials, fields, temperatures=None, positions_per_row=128, par
     """
     rs, ext = side / 2.0, side / 4.0
     with Batch(device) as b:
+        b.set_precision(precision)
         for i, m in enumerate(materials):
             Ts = [m["T"]] if temperatures is None else list(temperatures[i])
             kw = dict(seed=m["seed"], region_size=rs, region_extends=ext, impurity_radius=m["radius"],
