@@ -161,6 +161,8 @@ _SIGNATURES = {
     "es_solver_compute_sigmas": (C.c_int, [_P, C.c_double, _PD, C.c_int32, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_solver_compute_sigmas_sweep": (C.c_int, [_P, _PD, C.c_int32, _PD, C.c_int32, C.POINTER(Sigma),
                                                  C.POINTER(Metrics)]),
+    "es_solver_compute_sigmas_with_images": (C.c_int, [_P, C.c_double, _PD, C.c_int32, C.POINTER(Sigma),
+                                                       C.POINTER(Metrics), _PD]),
     "es_batch_create": (C.c_int, [C.c_int32, C.POINTER(_P)]),
     "es_batch_destroy": (None, [_P]),
     "es_batch_last_error": (C.c_char_p, [_P]),
@@ -190,6 +192,7 @@ _SIGNATURES = {
     # include/es_runner.h
     "es_config_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
     "es_run": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
+    "es_log_images": (C.c_int, [C.c_char_p, C.c_int, _PD, _PD, _PD]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 
@@ -235,6 +238,15 @@ def run(config_path, verbose=False):
     if rc != ES_OK:
         raise EsError(rc, "es_run failed (see stderr with verbose=True)")
     return buf.value.decode()
+
+
+def log_images(path, lifetimes, sigma_xx, sigma_xy):
+    """Logger::LogImages (Logger.cpp:320-334): three dim x dim images side by side as an RGB PNG."""
+    lt, xx, xy = (np.ascontiguousarray(a, dtype=np.float64) for a in (lifetimes, sigma_xx, sigma_xy))
+    dim = lt.shape[0]
+    rc = load_library().es_log_images(str(path).encode(), dim, _dptr(lt), _dptr(xx), _dptr(xy))
+    if rc != ES_OK:
+        raise EsError(rc, f"cannot write {path}")
 
 
 def _dptr(a):
@@ -319,6 +331,17 @@ class Simulation:
         self._check(self.lib.es_solver_compute_sigmas(self.h, float(B), _dptr(T), len(T), out, C.byref(m),
                                                       _dptr(lts)))
         return [(out[i].xx, out[i].xy) for i in range(len(T))], m.as_dict(), lts
+
+    def compute_sigmas_with_images(self, B, temperatures):
+        """Simulation::ComputeSigmasWithImages (Simulation.h:129): ([(xx, xy)] per T, metrics dict,
+        images [nT][3][ppr][ppr] = phi-integrated lifetime, sigma_xx, sigma_xy per position)."""
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        out = (Sigma * max(len(T), 1))()
+        m = Metrics()
+        img = np.empty((len(T), 3, self.ppr, self.ppr), dtype=np.float64)
+        self._check(self.lib.es_solver_compute_sigmas_with_images(self.h, float(B), _dptr(T), len(T), out,
+                                                                  C.byref(m), _dptr(img)))
+        return [(out[i].xx, out[i].xy) for i in range(len(T))], m.as_dict(), img
 
     def compute_sweep(self, fields, temperatures):
         """All fields in one launch: (sigma [nB, nT, 2], list of metrics dicts)."""

@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libes_hip.so")
 RUNNER = os.path.join(PKG, "es_run")
 
 HIP_SOURCES = ["es_kernels.hip", "es_abi.hip", "es_mlp.hip"]
-HOST_SOURCES = ["host/es_config.cpp", "host/es_runner.cpp"]
+HOST_SOURCES = ["host/es_config.cpp", "host/es_runner.cpp", "host/es_png.cpp"]
 RUNNER_MAIN = "host/es_run_main.cpp"
 
 ARCH = os.environ.get("ES_OFFLOAD_ARCH", "gfx950")

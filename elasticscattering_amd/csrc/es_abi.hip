@@ -669,6 +669,7 @@ es::SampleDev sample_dev(Field& f, int ppr, int ppq, double dist, double offx, d
     d.alpha = st.alpha;
     d.speed = st.particle_speed;
     d.coherent = coherent;
+    d.phi_integrand_factor = ps.phi_integrand_factor;
     return d;
 }
 
@@ -690,6 +691,7 @@ struct es_solver {
     es_solver_desc desc{};
     Field f;
     Stages s;
+    DevBuf img, img_aux;  // images mode: [nT][3][ppr^2] and the taus / caps
     bool has_sample = false;
     es_settings st{};
     int coherent = 0;
@@ -870,6 +872,8 @@ void es_solver_destroy(es_solver* s)
     if (s->c.stream) (void)hipStreamSynchronize(s->c.stream);
     field_release(s->c, s->f);
     stages_release(s->c, s->s);
+    release(s->c, s->img);
+    release(s->c, s->img_aux);
     ctx_destroy(s->c);
     delete s;
 }
@@ -944,6 +948,34 @@ int es_solver_compute_sigmas_sweep(es_solver* s, const double* B, int32_t nB, co
     if (!s) return ES_ERR_INVALID_ARG;
     ES_HIP(s->c, hipSetDevice(s->c.device));
     return solver_compute(s, B, nB, T, nT, out, metrics, nullptr);
+}
+
+int es_solver_compute_sigmas_with_images(es_solver* s, double B, const double* T, int32_t nT, es_sigma* out,
+                                         es_metrics* metrics, double* images)
+{
+    if (!s) return ES_ERR_INVALID_ARG;
+    Ctx& c = s->c;
+    ES_HIP(c, hipSetDevice(c.device));
+    if (nT <= 0 || !T || !out || !images) return fail(c, ES_ERR_INVALID_ARG, "need temperatures, out and images");
+    int rc = solver_compute(s, &B, 1, T, nT, out, metrics, nullptr);
+    if (rc) return rc;
+    // The stage buffers still hold this field's single-pair plan: its item
+    // lifetimes (one segment) and its sample / pair tables.
+    std::vector<double> aux;
+    if ((rc = taus_for(c, s->st, s->coherent, T, nT, aux))) return rc;
+    for (int t = 0; t < nT; t++) aux.push_back(15.0 * aux[t]);  // Simulation.h:91
+    const int ppr = s->desc.positions_per_row;
+    const size_t plane = (size_t)ppr * ppr, bytes = (size_t)nT * 3 * plane * sizeof(double);
+    if ((rc = ensure(c, s->img, bytes, "images"))) return rc;
+    if ((rc = ensure(c, s->img_aux, aux.size() * sizeof(double), "image taus"))) return rc;
+    ES_HIP(c, hipMemcpyAsync(s->img_aux.p, aux.data(), aux.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemsetAsync(s->img.p, 0, bytes, c.stream));
+    ES_HIP(c, es::launch_images(P<es::SampleDev>(s->s.samples), P<es::PairDev>(s->s.pairs),
+                                s->st.is_clockwise ? 1 : 0, P<double>(s->s.lt_item), P<double>(s->img_aux),
+                                P<double>(s->img_aux) + nT, nT, s->f.nfree, ppr, P<double>(s->img), c.stream));
+    ES_HIP(c, hipMemcpyAsync(images, s->img.p, bytes, hipMemcpyDeviceToHost, c.stream));
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    return ES_OK;
 }
 
 }  // extern "C"

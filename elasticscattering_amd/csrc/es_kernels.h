@@ -38,6 +38,7 @@ struct SampleDev {
     int ppr, ppq;
     double dist, offx, offy;         // distance_between_positions, small_offset
     double phi_start, phi_step, alpha, speed;
+    double phi_integrand_factor;     // Simulation.h:116-120
     int coherent;
     int nT;
     double tau[kMaxT];
@@ -123,6 +124,13 @@ hipError_t trace32_blocks_per_cu(int clockwise, int* blocks);
 hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks32, int nblocks64,
                               hipStream_t st);
 hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st);
+
+// Images mode (SimulationCPU.cpp:159-171, 224-243): for one pair, per free
+// position and temperature, the phi-integrated truncated lifetime, sigma_xx and
+// sigma_xy: img[t][0..2][j * ppr + i]; positions inside an impurity stay 0.
+hipError_t launch_images(const SampleDev* sample, const PairDev* pair, int clockwise, const double* lt_item,
+                         const double* tau, const double* max_lt, int nT, int nfree, int ppr, double* img,
+                         hipStream_t st);
 
 // One block per pair: out_sig [out][kMaxT][2], out_lt_sum [out], out_m [out][kMetricSlots].
 hipError_t launch_finalize(const SampleDev* samples, const PairDev* pairs, int npairs, int NT, int with_metrics,

@@ -802,6 +802,57 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 
 __global__ void k_reset(unsigned long long* q) { *q = 0ull; }
 
+// ------------------------------------------------------------------ images
+
+// One thread per free position: IntegratePosition (SimulationCPU.cpp:224-243)
+// of the truncated lifetimes (GetTruncatedLifetimes, Simulation.h:78-86) and
+// of ApplySigma's sigma_xx / sigma_xy terms (SimulationCPU.cpp:178-204), each
+// summed serially over q, p in the reference's order and scaled by
+// phi_integrand_factor.  A zero lifetime gives GetSigma = +-0 exactly, so it
+// is skipped as in k_sigma.
+__global__ __launch_bounds__(256) void k_images(const SampleDev* sample, const PairDev* pair, int clockwise,
+                                                const double* lt_item, const double* tau, const double* max_lt,
+                                                int nT, int nfree, int ppr, double* img)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nfree) return;
+    const SampleDev& sd = *sample;
+    const PairDev& pr = *pair;
+    const int ppq = sd.ppq, ppp = 4 * ppq;
+    const int pidx = sd.free_pos[slot];
+    const double ws = clockwise ? -pr.w : pr.w;
+    const long long base = pr.item_off + (long long)slot * ppp;
+    const size_t plane = (size_t)ppr * ppr;
+    for (int t = 0; t < nT; t++) {
+        const double ml = max_lt[t], tt = tau[t];
+        double lt_tot = 0.0, xx_tot = 0.0, xy_tot = 0.0;
+        for (int q = 0; q < 4; q++) {
+            for (int p = 0; p < ppq; p++) {
+                const double lt_raw = lt_item[base + q * ppq + p];
+                const double lt = (lt_raw < ml) ? lt_raw : ml;
+                const double w = simpson_weight(p, ppq);
+                lt_tot += lt * w;
+                if (lt != 0.0) {
+                    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
+                    double sphi, cphi, sa, ca;
+                    esm_sincos(phi, &sphi, &cphi);
+                    const double z = esm_exp(-lt / tt);
+                    esm_sincos(phi + ws * lt, &sa, &ca);
+                    double r = cphi - ca * z;
+                    r += ws * tt * sa * z;
+                    r -= ws * tt * sphi;
+                    xx_tot += (r * cphi) * w;
+                    xy_tot += (r * sphi) * w;
+                }
+            }
+        }
+        double* o = img + (size_t)t * 3 * plane;
+        o[pidx] = lt_tot * sd.phi_integrand_factor;
+        o[plane + pidx] = xx_tot * sd.phi_integrand_factor;
+        o[2 * plane + pidx] = xy_tot * sd.phi_integrand_factor;
+    }
+}
+
 __global__ void k_reset2(unsigned long long* q, int32_t* count)
 {
     *q = 0ull;
@@ -975,6 +1026,16 @@ hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int
         else
             hipLaunchKernelGGL((k_trace_w3<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_images(const SampleDev* sample, const PairDev* pair, int clockwise, const double* lt_item,
+                         const double* tau, const double* max_lt, int nT, int nfree, int ppr, double* img,
+                         hipStream_t st)
+{
+    if (nfree <= 0 || nT <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_images, dim3(blocks_for(nfree, 256)), dim3(256), 0, st, sample, pair, clockwise, lt_item,
+                       tau, max_lt, nT, nfree, ppr, img);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 #include "../../../include/es_abi.h"
 #include "../../../include/es_runner.h"
 #include "es_config.hpp"
+#include "es_png.hpp"
 
 namespace es_host {
 
@@ -267,7 +268,26 @@ struct Runner {
         }
         out.assign(nB, std::vector<SigmaPair>(nT));
         std::vector<es_sigma> sig((size_t)nB * std::max(nT, 1));
-        if (!cfg.dump_lifetimes) {
+        if (cfg.output_type == OutputType::All && !cfg.dump_lifetimes) {
+            // ComputeSigmasWithImages + LogImages per field (SimulationRunner.cpp:134-141)
+            const size_t plane = (size_t)cfg.positions_per_row * cfg.positions_per_row;
+            std::vector<double> img((size_t)nT * 3 * plane);
+            for (int b = 0; b < nB; b++) {
+                rc = check(es_solver_compute_sigmas_with_images(s, cfg.magnetic_fields[b], cfg.temperatures.data(), nT,
+                                                                sig.data() + (size_t)b * nT, &sm.iterations[b],
+                                                                img.data()));
+                if (rc) return rc;
+                for (int t = 0; t < nT; t++) {
+                    const double* im = img.data() + (size_t)t * 3 * plane;
+                    const std::string path = sample_path(cfg, sample_index) + (coherent ? "/Coherent/" : "/Incoherent/") +
+                                             "T" + std::to_string(t) + " MF" + std::to_string(b) + ".png";
+                    if (!es_host::log_images(path, cfg.positions_per_row, im, im + plane, im + 2 * plane)) {
+                        err = "cannot write " + path;
+                        return ES_ERR_INVALID_ARG;
+                    }
+                }
+            }
+        } else if (!cfg.dump_lifetimes) {
             rc = check(es_solver_compute_sigmas_sweep(s, cfg.magnetic_fields.data(), nB, cfg.temperatures.data(), nT,
                                                       sig.data(), sm.iterations.data()));
             if (rc) return rc;
@@ -393,6 +413,13 @@ struct Runner {
 }  // namespace
 
 }  // namespace es_host
+
+extern "C" int es_log_images(const char* path, int dim, const double* lifetimes, const double* sigma_xx,
+                             const double* sigma_xy)
+{
+    if (!path || dim <= 0 || !lifetimes || !sigma_xx || !sigma_xy) return ES_ERR_INVALID_ARG;
+    return es_host::log_images(path, dim, lifetimes, sigma_xx, sigma_xy) ? ES_OK : ES_ERR_INVALID_ARG;
+}
 
 extern "C" int es_config_parse(const char* path, char* out, int cap)
 {

@@ -161,6 +161,16 @@ int es_solver_compute_sigmas(es_solver* s, double B, const double* T, int32_t nT
 int es_solver_compute_sigmas_sweep(es_solver* s, const double* B, int32_t nB, const double* T, int32_t nT,
                                    es_sigma* out, es_metrics* metrics);
 
+/*
+ * Simulation::ComputeSigmasWithImages (Simulation.h:129; SimulationCPU.cpp:159-171):
+ * the sigmas of es_solver_compute_sigmas plus, per temperature, the images of
+ * IterationResult — images[t][0][j * ppr + i] the phi-integrated truncated
+ * lifetime, [t][1][...] sigma_xx and [t][2][...] sigma_xy per position
+ * (IntegratePosition, SimulationCPU.cpp:224-243): nT * 3 * ppr^2 doubles.
+ */
+int es_solver_compute_sigmas_with_images(es_solver* s, double B, const double* T, int32_t nT, es_sigma* out,
+                                         es_metrics* metrics, double* images);
+
 /* ---------------------------------------------------------------- batches
  *
  * Many samples of uneven size (own field, cell grid, positions, fields B and

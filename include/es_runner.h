@@ -28,6 +28,14 @@ int es_config_parse(const char* path, char* out, int cap);
  */
 int es_run(const char* config_path, int verbose, char* out_dir, int cap);
 
+/*
+ * Logger::LogImages (Logger.cpp:320-334): writes the three dim x dim images of
+ * one temperature (phi-integrated lifetime, sigma_xx, sigma_xy; as returned by
+ * es_solver_compute_sigmas_with_images) side by side as an 8-bit RGB PNG.
+ */
+int es_log_images(const char* path, int dim, const double* lifetimes, const double* sigma_xx,
+                  const double* sigma_xy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1110,6 +1110,45 @@ EXPORT int orc_derive_temperatures(const orc_grid* g, const orc_sim_params* prm,
     return 0;
 }
 
+/*
+ * SimulationCPU::DeriveTemperatureWithImages (SimulationCPU.cpp:159-171) for one
+ * temperature: IntegratePosition (:224-243) of the truncated lifetimes
+ * (Simulation.h:78-86) and of ApplySigma's sigma_xx / sigma_xy buffers
+ * (:178-204), each summed in reference order.  images: [3][ppr * ppr].
+ */
+EXPORT int orc_derive_images(const orc_grid* g, const orc_sim_params* prm, int coherent, double B,
+                             const double* raw_lifetimes, double temperature, double* images)
+{
+    orc_state st;
+    init_state(&st, prm, g, coherent, B);
+    const double tau = get_tau(&st, temperature, prm->incoherent_tau);
+    const double max_lt = 15.0 * tau; /* Simulation.h:91 */
+    const int64_t plane = (int64_t)st.ppr * st.ppr;
+    for (int j = 0; j < st.ppr; j++) {
+        for (int i = 0; i < st.ppr; i++) {
+            double lt_total = 0, xx_total = 0, xy_total = 0;
+            for (int q = 0; q < 4; q++) {
+                for (int p = 0; p < st.ppq; p++) {
+                    const int64_t idx = sim_index(&st, i, j, q, p);
+                    const double r = raw_lifetimes[idx];
+                    const double lt = (r < max_lt) ? r : max_lt;
+                    const double phi = st.ps.phi_start + q * C_HALF_PI + p * st.ps.phi_step_size;
+                    const double sigma_base = orc_get_sigma(lt, phi, tau, st.signed_angular_speed);
+                    const double w = orc_simpson_weight(p, st.ppq);
+                    lt_total += lt * w;
+                    xx_total += (sigma_base * O_COS(phi)) * w;
+                    xy_total += (sigma_base * O_SIN(phi)) * w;
+                }
+            }
+            const int64_t k = (int64_t)j * st.ppr + i;
+            images[k] = lt_total * st.phi_integrand_factor;
+            images[plane + k] = xx_total * st.phi_integrand_factor;
+            images[2 * plane + k] = xy_total * st.phi_integrand_factor;
+        }
+    }
+    return 0;
+}
+
 /* One particle: CreateParticle + TraceOrbit (lifetime.h:36-148). */
 EXPORT double orc_particle_lifetime(const orc_grid* g, int q, int p, double px, double py,
                                     double particle_speed, double angular_speed, double alpha,

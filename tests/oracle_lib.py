@@ -104,6 +104,7 @@ _ORC_ONLY = {
     "compute_lifetimes": (_I, [C.c_void_p, C.POINTER(OrcSimParams), _I, _D, _PD, C.POINTER(OrcMetrics)]),
     "derive_temperatures": (_I, [C.c_void_p, C.POINTER(OrcSimParams), _I, _D, _PD, _PD, _I,
                                  C.POINTER(OrcSigma)]),
+    "derive_images": (_I, [C.c_void_p, C.POINTER(OrcSimParams), _I, _D, _PD, _D, _PD]),
     "particle_lifetime": (_D, [C.c_void_p, _I, _I, _D, _D, _D, _D, _D, _I, _I, _D, _D,
                                C.POINTER(OrcMetrics)]),
 }
@@ -237,3 +238,12 @@ def read_points(path):
             xs.append(float(parts[0]))
             ys.append(float(parts[1]) if len(parts) > 1 else 0.0)
     return np.array(xs), np.array(ys)
+
+
+def derive_images(lib, grid, params, coherent, B, raw_lifetimes, temperature, ppr):
+    """IterationResult images of one temperature: [3][ppr][ppr] (lifetime, sigma_xx, sigma_xy)."""
+    lt = np.ascontiguousarray(raw_lifetimes, dtype=np.float64)
+    out = np.empty(3 * ppr * ppr, dtype=np.float64)
+    rc = lib.derive_images(grid.h, C.byref(params), int(coherent), float(B), dptr(lt), float(temperature), dptr(out))
+    assert rc == 0
+    return out.reshape(3, ppr, ppr)
