@@ -782,6 +782,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
         for (int t = 0; t < es::kMaxT; t++) {
             sd.tau[t] = t < nt ? taus[t0 + t] : 1.0;
             sd.max_lt[t] = 15.0 * sd.tau[t];  // Simulation.h:91
+            sd.inv_tau[t] = 1.0 / sd.tau[t];
         }
         pl.factor.assign((size_t)nB * es::kMaxT, 0.0);
         for (int b = 0; b < nB; b++)
@@ -1003,6 +1004,7 @@ int batch_plan(es_batch* bt)
         for (int t = 0; t < es::kMaxT; t++) {
             sd.tau[t] = t < sd.nT ? en.taus[t] : 1.0;
             sd.max_lt[t] = 15.0 * sd.tau[t];
+            sd.inv_tau[t] = 1.0 / sd.tau[t];
         }
         pl.samples.push_back(sd);
         for (size_t b = 0; b < en.B.size(); b++) {

@@ -43,6 +43,7 @@ struct SampleDev {
     int nT;
     double tau[kMaxT];
     double max_lt[kMaxT];            // 15 * tau (Simulation.h:91)
+    double inv_tau[kMaxT];           // RN(1 / tau), for esm_div_rcp
 };
 
 // One (sample, pass, field).

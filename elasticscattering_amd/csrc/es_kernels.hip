@@ -454,6 +454,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
                     g.start = sd.start;
                     g.size = sd.size;
                     g.L = sd.L;
+                    g.inv_L = 1.0 / sd.L;
                     g.radius = sd.radius;
                     g.cpr = sd.cpr;
                     cap = 4 * sd.cpr * sd.cpr + 16;
@@ -479,6 +480,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
                     o.r2 = g.radius * g.radius;
                     o.pa = phi;
                     o.w = w;
+                    o.inv_R = 1.0 / R;
+                    o.inv_w = 1.0 / w;
                     o.u0x = px - o.cx;
                     o.u0y = py - o.cy;
                     o.r = (float)g.radius;
@@ -740,7 +743,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
                 const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
                 if (lt != 0.0) {
                     const double tau = sd.tau[t];
-                    const double z = esm_exp(-lt / tau);
+                    const double z = esm_exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
                     double sa, ca;
                     esm_sincos(phi + ws * lt, &sa, &ca);
                     double r = cphi - ca * z;

@@ -251,4 +251,22 @@ ESM_FN double esm_exp(double x)
     return ldexp(y, ki + 1000) * 9.332636185032188789900895447238171696171e-302; /* 2^-1000 */
 }
 
+/*
+ * a / b given y = RN(1/b), in three operations:  q0 = RN(a * y),
+ * r = a - b * q0 (exact with one fma), RN(q0 + r * y).  By Markstein's
+ * theorem (IA-64 division; Muller et al., Handbook of Floating-Point
+ * Arithmetic, 2nd ed., sec. 5.3) this is the correctly rounded quotient,
+ * i.e. bit-identical to IEEE a / b, whenever y is the correctly rounded
+ * reciprocal and nothing overflows or underflows.  The kernels use it where
+ * the divisor is fixed for a particle or a launch (y computed once with an
+ * IEEE division); the CPU restatement divides, and the bit-exact lifetime
+ * tests compare the two.
+ */
+ESM_FN double esm_div_rcp(double a, double b, double y)
+{
+    const double q0 = a * y;
+    const double r = fma(-b, q0, a);
+    return fma(r, y, q0);
+}
+
 #endif /* ES_MATH_H */

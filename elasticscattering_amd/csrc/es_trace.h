@@ -33,6 +33,7 @@
 #define ES_PI2 6.283185307179586
 #define ES_HALF_PI 1.570796326794896619231
 #define ES_INF 1.7976931348623157e308   /* DBL_MAX (constants.h:36) */
+#define ES_INV_PI2 (1.0 / ES_PI2)          /* RN(1 / 2 pi), folded by the compiler */
 
 // Device view of the binned impurity field (Grid::GetSettings, Grid.h:82-91).
 struct DevGrid {
@@ -41,6 +42,7 @@ struct DevGrid {
     double start;             // spawn_region_start
     double size;              // spawn_region_size
     double L;                 // cell_size
+    double inv_L;             // RN(1 / L), for esm_div_rcp
     double radius;            // impurity_radius
     int cpr;                  // cells_per_row
 };
@@ -54,6 +56,7 @@ struct Orb {
     double r2;              // r * r (details.h:160)
     double pa;              // particle_angle = phi
     double w;               // angular speed (unsigned)
+    double inv_R, inv_w;    // RN(1 / R), RN(1 / w), for esm_div_rcp
     double u0x, u0y;        // start position minus centre (screening only)
     float r;                // impurity radius (screening only)
 };
@@ -70,6 +73,12 @@ __device__ __forceinline__ double d_smod(double a, double b)
     return a - b * floor(a / b);  // details.h:79-83
 }
 
+// smod(a, 2 pi) with the quotient from the precomputed reciprocal (identical bits).
+__device__ __forceinline__ double d_smod_2pi(double a)
+{
+    return a - ES_PI2 * floor(esm_div_rcp(a, ES_PI2, ES_INV_PI2));
+}
+
 // GetAngle + AngleVelocity (details.h:85-107).
 template <bool CW>
 __device__ __forceinline__ double d_angle(double x, double y, const Orb& o)
@@ -78,19 +87,19 @@ __device__ __forceinline__ double d_angle(double x, double y, const Orb& o)
     const bool left_side = x < o.cx;
     double angle;
     if (fabs(y_dist) > o.eps_r) {
-        angle = esm_asin(y_dist / o.R);
+        angle = esm_asin(esm_div_rcp(y_dist, o.R, o.inv_R));
         angle = left_side ? ES_PI - angle : angle;
     } else {
         angle = left_side ? ES_PI : 0.0;
     }
-    return d_smod(angle + (CW ? ES_HALF_PI : -ES_HALF_PI), ES_PI2);
+    return d_smod_2pi(angle + (CW ? ES_HALF_PI : -ES_HALF_PI));
 }
 
 // GetCrossAngle (details.h:111-116).
 template <bool CW>
 __device__ __forceinline__ double d_cross_angle(double p, double q)
 {
-    return d_smod(CW ? (p - q) : (q - p), ES_PI2);
+    return d_smod_2pi(CW ? (p - q) : (q - p));
 }
 
 // AngleInRange (details.h:119-136).
@@ -111,9 +120,10 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
     const double qx = o.cx - px, qy = o.cy - py;
     const double dist_squared = qx * qx + qy * qy;
     const double dist = sqrt(dist_squared);
-    const double xs = (dist_squared + o.R2 - o.r2) / (2.0 * dist);
+    const double inv_d = 1.0 / dist;  // RN(1 / (2 dist)) = RN(1 / dist) / 2
+    const double xs = esm_div_rcp(dist_squared + o.R2 - o.r2, 2.0 * dist, 0.5 * inv_d);
     const double ys = sqrt(o.R2 - xs * xs);
-    const double ux = (px - o.cx) / dist, uy = (py - o.cy) / dist;
+    const double ux = esm_div_rcp(px - o.cx, dist, inv_d), uy = esm_div_rcp(py - o.cy, dist, inv_d);
     const double bx = o.cx + ux * xs, by = o.cy + uy * xs;
     double tt = ES_INF;
 #pragma unroll 1
@@ -123,7 +133,7 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
         const double ey = k ? ux : -ux;
         const double phi_k = d_angle<CW>(bx + ex * ys, by + ey * ys, o);
         if (d_in_range<CW>(phi_k, rx, ry)) {
-            const double t = d_cross_angle<CW>(o.pa, phi_k) / o.w;
+            const double t = esm_div_rcp(d_cross_angle<CW>(o.pa, phi_k), o.w, o.inv_w);
             tt = (tt < t) ? tt : t;
         }
     }
@@ -156,7 +166,7 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
         const int ncx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
         const int ncy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
 
-        const double ux = (p2x - p1x) / L, uy = (p2y - p1y) / L;
+        const double ux = esm_div_rcp(p2x - p1x, L, g.inv_L), uy = esm_div_rcp(p2y - p1y, L, g.inv_L);
         const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
         const double projx = p1x + ux * pd, projy = p1y + uy * pd;
         const double dx = projx - o.cx, dy = projy - o.cy;
@@ -173,9 +183,12 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
             const double a = horizontal ? ix : iy;
             if ((a > l0 && a < l1) || (a < l0 && a > l1)) {
                 edge_hits += 1;
+                // DifferentPoint (cell_grid.h:74-77) first: a point it rejects is
+                // rejected whatever its angle, so its asin is not needed.
+                if (!(fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps)) continue;
                 const double ia = d_angle<CW>(ix, iy, o);
                 const double dphi = d_cross_angle<CW>(last.ang, ia);
-                if (dphi < best_dphi && (fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps)) {
+                if (dphi < best_dphi) {
                     next.x = ix;
                     next.y = iy;
                     next.cx = ncx;
