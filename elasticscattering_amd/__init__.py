@@ -140,7 +140,8 @@ class BatchSample(C.Structure):
 
 
 class BatchShape(C.Structure):
-    _fields_ = [("waves_per_eu", C.c_int32), ("chunk", C.c_int32), ("grid_mult", C.c_int32)]
+    _fields_ = [("waves_per_eu", C.c_int32), ("chunk", C.c_int32), ("grid_mult", C.c_int32),
+                ("cooperative", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -434,8 +435,9 @@ class Batch:
         self._check(self.lib.es_batch_get_shape(self.h, C.byref(sh)))
         return sh.as_dict()
 
-    def set_shape(self, waves_per_eu=3, chunk=128, grid_mult=1):
-        sh = BatchShape(int(waves_per_eu), int(chunk), int(grid_mult))
+    def set_shape(self, waves_per_eu=3, chunk=128, grid_mult=1, cooperative=1):
+        """Trace launch shape; cooperative=1: a wave solves its lanes' crossing disks together."""
+        sh = BatchShape(int(waves_per_eu), int(chunk), int(grid_mult), int(cooperative))
         self._check(self.lib.es_batch_set_shape(self.h, C.byref(sh)))
 
     def set_capture(self, enable):

@@ -55,7 +55,7 @@ struct Ctx {
     size_t byte_cap = 0;  // 0: unlimited; else refuse allocations beyond this total
     size_t bytes_held = 0;
     int cus = 0;
-    int trace_bpc[2][4] = {{0}};  // resident trace blocks per CU [clockwise][waves_per_eu]
+    int trace_bpc[2][2][4] = {{{0}}};  // resident trace blocks per CU [clockwise][coop][waves_per_eu]
     int trace32_bpc[2] = {0, 0};  // resident FP32 trace blocks per CU [clockwise]
     long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
@@ -380,6 +380,7 @@ struct Shape {
     int waves_per_eu = 3;
     int chunk = 128;
     int grid_mult = 1;
+    int coop = 1;  // wave-cooperative crossing solves in the FP64 trace
 };
 
 // Device buffers and the optional captured graph of the stage kernels.
@@ -510,17 +511,19 @@ int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<in
                  int* nblocks64 = nullptr)
 {
     const int w = (sh.waves_per_eu == 2) ? 2 : 3;
-    if (c.trace_bpc[cw][w] <= 0) {
-        ES_HIP(c, es::trace_blocks_per_cu(cw, w, &c.trace_bpc[cw][w]));
-        if (c.trace_bpc[cw][w] <= 0) c.trace_bpc[cw][w] = 1;
+    const int co = sh.coop ? 1 : 0;
+    int& bpc64 = c.trace_bpc[cw][co][w];
+    if (bpc64 <= 0) {
+        ES_HIP(c, es::trace_blocks_per_cu(cw, w, co, &bpc64));
+        if (bpc64 <= 0) bpc64 = 1;
     }
-    int bpc = c.trace_bpc[cw][w];
+    int bpc = bpc64;
     if (pl.precision == ES_PRECISION_MIXED) {
         if (c.trace32_bpc[cw] <= 0) {
             ES_HIP(c, es::trace32_blocks_per_cu(cw, &c.trace32_bpc[cw]));
             if (c.trace32_bpc[cw] <= 0) c.trace32_bpc[cw] = 1;
         }
-        if (nblocks64) *nblocks64 = c.trace_bpc[cw][w] * c.cus;
+        if (nblocks64) *nblocks64 = bpc64 * c.cus;
         bpc = c.trace32_bpc[cw];
     }
     nblocks.clear();
@@ -560,9 +563,10 @@ int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trac
             if (pl.precision == ES_PRECISION_MIXED) {
                 a.list = P<int32_t>(s.list);
                 a.list_count = P<int32_t>(s.list_count);
-                ES_HIP(c, es::launch_trace_mixed(a, pl.clockwise, sh.waves_per_eu, nblocks[si], nblocks64, c.stream));
+                ES_HIP(c, es::launch_trace_mixed(a, pl.clockwise, sh.waves_per_eu, sh.coop, nblocks[si], nblocks64,
+                                                    c.stream));
             } else {
-                ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, nblocks[si], c.stream));
+                ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, sh.coop, nblocks[si], c.stream));
             }
             if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 1], c.stream));
         }
@@ -1202,11 +1206,13 @@ int es_batch_set_shape(es_batch* bt, const es_batch_shape* shape)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
     if (!shape || (shape->waves_per_eu != 2 && shape->waves_per_eu != 3) || shape->chunk < 1 || shape->chunk > 4096 ||
-        shape->grid_mult < 1 || shape->grid_mult > 8)
-        return fail(bt->c, ES_ERR_INVALID_ARG, "shape: waves_per_eu 2 or 3, chunk 1..4096, grid_mult 1..8");
+        shape->grid_mult < 1 || shape->grid_mult > 8 || (shape->cooperative != 0 && shape->cooperative != 1))
+        return fail(bt->c, ES_ERR_INVALID_ARG,
+                    "shape: waves_per_eu 2 or 3, chunk 1..4096, grid_mult 1..8, cooperative 0 or 1");
     bt->shape.waves_per_eu = shape->waves_per_eu;
     bt->shape.chunk = shape->chunk;
     bt->shape.grid_mult = shape->grid_mult;
+    bt->shape.coop = shape->cooperative;
     batch_release_graph(bt);
     return ES_OK;
 }
@@ -1217,6 +1223,7 @@ int es_batch_get_shape(es_batch* bt, es_batch_shape* shape)
     shape->waves_per_eu = bt->shape.waves_per_eu;
     shape->chunk = bt->shape.chunk;
     shape->grid_mult = bt->shape.grid_mult;
+    shape->cooperative = bt->shape.coop;
     return ES_OK;
 }
 
@@ -1299,17 +1306,20 @@ int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
     bt->capture = false;  // the candidates are timed on direct launches
     Shape best_sh = bt->shape;
     double best_t = 1e300;
+    const int coops[] = {1, 0};
     const int waves[] = {2, 3};
     const int chunks[] = {32, 64, 128, 256, 512};
     const int mults[] = {1, 2};
     StageOut so;
     double t = 0;
     int rc = batch_run_once(bt, bt->shape, so, &t);  // warm-up (plan, allocation, code load)
+    for (int co : coops)
     for (int w : waves)
         for (int ch : chunks)
             for (int m : mults) {
                 if (rc) break;
                 Shape sh;
+                sh.coop = co;
                 sh.waves_per_eu = w;
                 sh.chunk = ch;
                 sh.grid_mult = m;
@@ -1331,6 +1341,7 @@ int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
         best->waves_per_eu = best_sh.waves_per_eu;
         best->chunk = best_sh.chunk;
         best->grid_mult = best_sh.grid_mult;
+        best->cooperative = best_sh.coop;
     }
     if (best_seconds) *best_seconds = best_t;
     return ES_OK;

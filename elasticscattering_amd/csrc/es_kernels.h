@@ -116,14 +116,15 @@ hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev
 hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
                             int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st);
 
-// Trace kernel variants (register budget 2 or 3 waves per SIMD).
-hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int* blocks);
+// Trace kernel variants: register budget 2 or 3 waves per SIMD; coop = 1 for
+// wave-cooperative crossing solves, 0 for one lane per particle throughout.
+hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* blocks);
 // Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
-hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks, hipStream_t st);
+hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st);
 // Mixed precision: FP32 trace of every item, then FP64 trace of the flagged ones.
 hipError_t trace32_blocks_per_cu(int clockwise, int* blocks);
-hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks32, int nblocks64,
-                              hipStream_t st);
+hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks32,
+                              int nblocks64, hipStream_t st);
 hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st);
 
 // Images mode (SimulationCPU.cpp:159-171, 224-243): for one pair, per free

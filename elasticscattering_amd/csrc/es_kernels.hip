@@ -399,11 +399,125 @@ __device__ __forceinline__ int pair_of_block(const PairDev* pairs, int npairs, i
     return lo;
 }
 
+// CreateParticle (lifetime.h:36-58) and the trace start for work item `item`:
+// the lane's grid view, orbit, start angle, bound time and output slot.
 template <bool CW>
-__device__ __forceinline__ void trace_body(const TraceArgs& a)
+__device__ __forceinline__ void d_setup_item(const TraceArgs& a, long long item, DevGrid& g, Orb& o, double& phi0,
+                                             double& bound_time, long long& out_idx, int& cap, TraceState& st)
+{
+    const int pi = pair_of_item(a.pairs, a.npairs, item);
+    const PairDev& pr = a.pairs[pi];
+    const SampleDev& sd = a.samples[pr.sample];
+    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
+    const long long k = item - pr.item_off;
+    const long long slot = k / ppp;
+    const int v = (int)(k - slot * ppp);
+    const int q = v / ppq, p = v - (v / ppq) * ppq;
+    const int pidx = sd.free_pos[slot];
+    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+    g.ixy = sd.ixy;
+    g.cell_end = sd.cell_end;
+    g.start = sd.start;
+    g.size = sd.size;
+    g.L = sd.L;
+    g.inv_L = 1.0 / sd.L;
+    g.radius = sd.radius;
+    g.cpr = sd.cpr;
+    cap = 4 * sd.cpr * sd.cpr + 16;
+    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
+    const double py = (double)j * sd.dist + sd.offy;
+    const double w = pr.w;
+    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
+    double sphi, cphi;
+    esm_sincos(phi, &sphi, &cphi);
+    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
+    const double R = sd.speed / w;
+    const double sx = vy * R / sd.speed;
+    const double sy = -vx * R / sd.speed;
+    const double radd = R + g.radius, rmin = R - g.radius;
+    o.cx = CW ? (px - sx) : (px + sx);
+    o.cy = CW ? (py - sy) : (py + sy);
+    o.R = R;
+    o.R2 = R * R;
+    o.eps_r = 1e-10 * R;
+    o.radd2 = radd * radd;
+    o.rmin2 = rmin * rmin;
+    o.r2 = g.radius * g.radius;
+    o.pa = phi;
+    o.w = w;
+    o.inv_R = 1.0 / R;
+    o.inv_w = 1.0 / w;
+    o.u0x = px - o.cx;
+    o.u0y = py - o.cy;
+    o.r = (float)g.radius;
+    phi0 = phi;
+    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
+    if (!sd.coherent) {
+        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
+        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
+        bound_time = dphi / w;
+    }
+    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
+    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
+    d_trace_begin(st, px, py, phi, c0x, c0y);
+}
+
+// Writes the per-item results of an ended trace (lifetime.h:142-147).
+__device__ __forceinline__ void d_store_item(const TraceArgs& a, long long item, long long out_idx,
+                                             double bound_time, const TraceState& st, int flags)
+{
+    double lt = st.lifetime;
+    if (lt > bound_time) {  // lifetime.h:142-146
+        lt = bound_time;
+        flags |= ES_F_AT_BOUND;
+    }
+    a.lt_item[item] = lt;
+    a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
+    a.x_item[item] = st.cross_exact;
+    a.f_item[item] = (uint8_t)(flags | (a.list_mode ? ES_F_RECOMPUTE : 0));
+    if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+}
+
+// Lane refill: every idle lane takes the next item of the wave's chunk of the
+// work queue (a fresh chunk of a.chunk items when the current one is used up).
+// Returns the item to a lane that got one, -1 to the others.  Wave-uniform
+// state: q_next, q_end, exhausted.
+__device__ __forceinline__ void d_refill(const TraceArgs& a, long long total, bool active, long long& q_next,
+                                         long long& q_end, bool& exhausted, long long& got)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    got = -1;
+    unsigned long long need = __ballot(!active);
+    while (need != 0ull && !exhausted) {
+        if (q_next >= q_end) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+            base = __shfl(base, 0, 64);
+            if ((long long)base >= total) {
+                exhausted = true;
+                break;
+            }
+            q_next = (long long)base;
+            q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
+        }
+        const long long left = q_end - q_next;
+        const int n = __popcll(need);
+        const int take = (int)((long long)n < left ? (long long)n : left);
+        const bool idle = (need >> lane) & 1ull;
+        if (idle) {
+            const int rank = __popcll(need & below);
+            if (rank < take) got = a.list_mode ? (long long)a.list[q_next + rank] : q_next + rank;
+        }
+        q_next += take;
+        need = __ballot(!active && got < 0);
+    }
+}
+
+template <bool CW>
+__device__ __forceinline__ void trace_body(const TraceArgs& a)
+{
     const long long total = a.list_mode ? (long long)*a.list_count : a.total_items;
 
     long long q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
@@ -417,105 +531,203 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
     TraceState st;
 
     while (true) {
-        // Give every idle lane the next item of the wave's chunk.
-        unsigned long long need = __ballot(!active);
-        while (need != 0ull && !exhausted) {
-            if (q_next >= q_end) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-                base = __shfl(base, 0, 64);
-                if ((long long)base >= total) {
-                    exhausted = true;
-                    break;
-                }
-                q_next = (long long)base;
-                q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
-            }
-            const long long left = q_end - q_next;
-            const int n = __popcll(need);
-            const int take = (int)((long long)n < left ? (long long)n : left);
-            if (!active) {
-                const int rank = __popcll(need & below);
-                if (rank < take) {
-                    item = a.list_mode ? (long long)a.list[q_next + rank] : q_next + rank;
-                    const int pi = pair_of_item(a.pairs, a.npairs, item);
-                    const PairDev& pr = a.pairs[pi];
-                    const SampleDev& sd = a.samples[pr.sample];
-                    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
-                    const long long k = item - pr.item_off;
-                    const long long slot = k / ppp;
-                    const int v = (int)(k - slot * ppp);
-                    const int q = v / ppq, p = v - (v / ppq) * ppq;
-                    const int pidx = sd.free_pos[slot];
-                    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
-                    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
-                    g.ixy = sd.ixy;
-                    g.cell_end = sd.cell_end;
-                    g.start = sd.start;
-                    g.size = sd.size;
-                    g.L = sd.L;
-                    g.inv_L = 1.0 / sd.L;
-                    g.radius = sd.radius;
-                    g.cpr = sd.cpr;
-                    cap = 4 * sd.cpr * sd.cpr + 16;
-                    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
-                    const double py = (double)j * sd.dist + sd.offy;
-                    const double w = pr.w;
-                    // CreateParticle (lifetime.h:36-58)
-                    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
-                    double sphi, cphi;
-                    esm_sincos(phi, &sphi, &cphi);
-                    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
-                    const double R = sd.speed / w;
-                    const double sx = vy * R / sd.speed;
-                    const double sy = -vx * R / sd.speed;
-                    const double radd = R + g.radius, rmin = R - g.radius;
-                    o.cx = CW ? (px - sx) : (px + sx);
-                    o.cy = CW ? (py - sy) : (py + sy);
-                    o.R = R;
-                    o.R2 = R * R;
-                    o.eps_r = 1e-10 * R;
-                    o.radd2 = radd * radd;
-                    o.rmin2 = rmin * rmin;
-                    o.r2 = g.radius * g.radius;
-                    o.pa = phi;
-                    o.w = w;
-                    o.inv_R = 1.0 / R;
-                    o.inv_w = 1.0 / w;
-                    o.u0x = px - o.cx;
-                    o.u0y = py - o.cy;
-                    o.r = (float)g.radius;
-                    phi0 = phi;
-                    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
-                    if (!sd.coherent) {
-                        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
-                        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
-                        bound_time = dphi / w;
-                    }
-                    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
-                    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
-                    d_trace_begin(st, px, py, phi, c0x, c0y);
-                    active = true;
-                }
-            }
-            q_next += take;
-            need = __ballot(!active);
+        long long got;
+        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        if (got >= 0) {
+            item = got;
+            d_setup_item<CW>(a, item, g, o, phi0, bound_time, out_idx, cap, st);
+            active = true;
         }
         if (__ballot(active) == 0ull) break;  // queue drained and every lane done
 
         if (active) {
             int flags = 0;
             if (d_step<CW>(o, phi0, g, cap, st, flags)) {
-                double lt = st.lifetime;
-                if (lt > bound_time) {  // lifetime.h:142-146
-                    lt = bound_time;
-                    flags |= ES_F_AT_BOUND;
+                d_store_item(a, item, out_idx, bound_time, st, flags);
+                active = false;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------ wave-cooperative FP64 trace
+//
+// Same lane refill and per-lane cell walk as trace_body, but the crossing
+// disks of all 64 lanes' current cells go to a per-wave stack in LDS and are
+// solved 64 at a time, each lane taking any lane's disk: the owner's orbit and
+// angle range are read from LDS and the time is folded into the owner's
+// running minimum with an LDS atomic min on an order-preserving integer key.
+// Each solve runs the same operations on the same operands as in trace_body
+// and a minimum does not depend on the order, so lifetimes are bit-identical;
+// what changes is that a wave no longer waits, per group of impurities, for
+// its lane with the most crossing disks.
+
+constexpr int kCoopStack = 320;  // < 64 left over + up to 4 new disks per lane per group
+
+struct CoopWave {
+    double orb[12][64];              // per lane: cx cy R R2 r2 eps_r pa w inv_R inv_w rx ry
+    double qx[kCoopStack], qy[kCoopStack];  // queued crossing disks
+    unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
+    uint8_t qown[kCoopStack];        // lane owning each queued disk
+};
+
+// Unsigned order of the keys = numeric order of the doubles (signs included).
+__device__ __forceinline__ unsigned long long d_order_key(double t)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(t);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double d_from_order_key(unsigned long long k)
+{
+    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
+}
+
+// Orders this wave's LDS accesses before and after (lanes of one wave only).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// GetFirstCrossTime of queued disk k for its owner, folded into the owner's
+// minimum as `lifetime = (t < lifetime) ? t : lifetime` would (lifetime.h:117-123):
+// NaN and times >= DBL_MAX never lower it, and no solve yields -0.
+template <bool CW>
+__device__ __forceinline__ void d_coop_solve(CoopWave& L, int k)
+{
+    const int own = L.qown[k];
+    Orb ot;
+    ot.cx = L.orb[0][own];
+    ot.cy = L.orb[1][own];
+    ot.R = L.orb[2][own];
+    ot.R2 = L.orb[3][own];
+    ot.r2 = L.orb[4][own];
+    ot.eps_r = L.orb[5][own];
+    ot.pa = L.orb[6][own];
+    ot.w = L.orb[7][own];
+    ot.inv_R = L.orb[8][own];
+    ot.inv_w = L.orb[9][own];
+    ot.radd2 = ot.rmin2 = ot.u0x = ot.u0y = 0.0;
+    ot.r = 0.0f;
+    const double t = d_first_cross_time<CW>(ot, L.qx[k], L.qy[k], L.orb[10][own], L.orb[11][own]);
+    if (t < ES_INF) atomicMin(&L.tmin[own], d_order_key(t));
+}
+
+template <bool CW>
+__device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
+{
+    __shared__ CoopWave sh_coop[kBlock / 64];
+    CoopWave& L = sh_coop[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const long long total = a.list_mode ? (long long)*a.list_count : a.total_items;
+    const unsigned long long key_inf = d_order_key(ES_INF);
+    L.tmin[lane] = key_inf;
+
+    long long q_next = 0, q_end = 0;
+    bool exhausted = false;
+    bool active = false;
+    long long item = 0, out_idx = -1;
+    int cap = 0;
+    DevGrid g;
+    Orb o;
+    double phi0 = 0.0, bound_time = 0.0;
+    TraceState st;
+
+    while (true) {
+        long long got;
+        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        if (got >= 0) {
+            item = got;
+            d_setup_item<CW>(a, item, g, o, phi0, bound_time, out_idx, cap, st);
+            L.orb[0][lane] = o.cx;
+            L.orb[1][lane] = o.cy;
+            L.orb[2][lane] = o.R;
+            L.orb[3][lane] = o.R2;
+            L.orb[4][lane] = o.r2;
+            L.orb[5][lane] = o.eps_r;
+            L.orb[6][lane] = o.pa;
+            L.orb[7][lane] = o.w;
+            L.orb[8][lane] = o.inv_R;
+            L.orb[9][lane] = o.inv_w;
+            active = true;
+        }
+        if (__ballot(active) == 0ull) break;
+
+        // This lane's cell: the next cell and angle range (lifetime.h:106-115).
+        int s = 0, n = 0;
+        bool avail = false;
+        if (active) {
+            const Isect entry = st.nx;
+            avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
+            L.orb[10][lane] = entry.ang;
+            L.orb[11][lane] = avail ? st.nx.ang : phi0;
+            const int ci = entry.cy * g.cpr + entry.cx;
+            s = (ci > 0) ? g.cell_end[ci - 1] : 0;
+            n = g.cell_end[ci] - s;
+            st.cells += 1;
+            st.tested += n;
+        }
+        wave_lds_sync();
+
+        // CirclesCross (details.h:141-151) per lane, four impurities at a time;
+        // crossing disks go on the wave's stack, which is solved 64 at a time.
+        int top = 0;  // wave-uniform
+        for (int i = 0;; i += 4) {
+            const bool more = __ballot(i < n) != 0ull;
+            if (more) {
+                double2 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
+                int hits = 0;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
+                    const double d2 = qx * qx + qy * qy;
+                    hits |= ((i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2)) ? (1 << u) : 0;
                 }
-                a.lt_item[item] = lt;
-                a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
-                a.x_item[item] = st.cross_exact;
-                a.f_item[item] = (uint8_t)(flags | (a.list_mode ? ES_F_RECOMPUTE : 0));
-                if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+                st.cross_calls += __popc(hits);
+                st.cross_exact += __popc(hits);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool h = (hits >> u) & 1;
+                    const unsigned long long m = __ballot(h);
+                    if (h) {
+                        const int pos = top + __popcll(m & below);
+                        L.qx[pos] = p[u].x;
+                        L.qy[pos] = p[u].y;
+                        L.qown[pos] = (uint8_t)lane;
+                    }
+                    top += __popcll(m);
+                }
+                wave_lds_sync();
+            }
+            while (top >= 64 || (!more && top > 0)) {
+                const int m = (top < 64) ? top : 64;
+                if (lane < m) d_coop_solve<CW>(L, top - m + lane);
+                top -= m;
+                wave_lds_sync();
+            }
+            if (!more) break;
+        }
+
+        if (active) {
+            st.lifetime = d_from_order_key(L.tmin[lane]);
+            L.tmin[lane] = key_inf;
+            int flags = -1;
+            if (st.lifetime < ES_INF) {
+                flags = 0;
+            } else if (!avail) {
+                flags = ES_F_ESCAPED;
+                st.lifetime = 0.0;
+            } else if (st.cells >= cap) {
+                flags = ES_F_CAP;
+                st.lifetime = 0.0;
+            }
+            if (flags >= 0) {
+                d_store_item(a, item, out_idx, bound_time, st, flags);
                 active = false;
             }
         }
@@ -673,6 +885,18 @@ template <bool CW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_w2(TraceArgs a)
 {
     trace_body<CW>(a);
+}
+
+template <bool CW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_c3(TraceArgs a)
+{
+    trace_coop_body<CW>(a);
+}
+
+template <bool CW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_c2(TraceArgs a)
+{
+    trace_coop_body<CW>(a);
 }
 
 template <bool CW>
@@ -1003,33 +1227,34 @@ hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx,
     return hipGetLastError();
 }
 
-hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int* blocks)
+// The trace kernel of a launch shape: rotation sense x register budget x
+// per-lane or wave-cooperative crossing solves.
+static const void* trace_kernel(int clockwise, int waves_per_eu, int coop)
 {
-    if (waves_per_eu == 2)
-        return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w2<true>, kBlock, 0)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w2<false>, kBlock, 0);
-    return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w3<true>, kBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace_w3<false>, kBlock, 0);
+    if (coop) {
+        if (waves_per_eu == 2)
+            return clockwise ? (const void*)k_trace_c2<true> : (const void*)k_trace_c2<false>;
+        return clockwise ? (const void*)k_trace_c3<true> : (const void*)k_trace_c3<false>;
+    }
+    if (waves_per_eu == 2) return clockwise ? (const void*)k_trace_w2<true> : (const void*)k_trace_w2<false>;
+    return clockwise ? (const void*)k_trace_w3<true> : (const void*)k_trace_w3<false>;
 }
 
-hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks, hipStream_t st)
+hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* blocks)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, trace_kernel(clockwise, waves_per_eu, coop), kBlock,
+                                                        0);
+}
+
+hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st)
 {
     if (a.npairs <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || (!a.list_mode && a.total_items <= 0)) return e;
-    if (waves_per_eu == 2) {
-        if (clockwise)
-            hipLaunchKernelGGL((k_trace_w2<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_trace_w2<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
-    } else {
-        if (clockwise)
-            hipLaunchKernelGGL((k_trace_w3<true>), dim3(nblocks), dim3(kBlock), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_trace_w3<false>), dim3(nblocks), dim3(kBlock), 0, st, a);
-    }
-    return hipGetLastError();
+    TraceArgs arg = a;
+    void* params[] = {&arg};
+    return hipLaunchKernel(trace_kernel(clockwise, waves_per_eu, coop), dim3(nblocks), dim3(kBlock), params, 0, st);
 }
 
 hipError_t launch_images(const SampleDev* sample, const PairDev* pair, int clockwise, const double* lt_item,
@@ -1048,8 +1273,8 @@ hipError_t trace32_blocks_per_cu(int clockwise, int* blocks)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace32<false>, kBlock, 0);
 }
 
-hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int nblocks32, int nblocks64,
-                              hipStream_t st)
+hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks32,
+                              int nblocks64, hipStream_t st)
 {
     if (a.npairs <= 0 || nblocks32 <= 0 || nblocks64 <= 0 || a.chunk <= 0 || !a.list || !a.list_count)
         return hipErrorInvalidValue;
@@ -1068,7 +1293,7 @@ hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_e
     TraceArgs a64 = a;
     a64.list_mode = 1;
     a64.chunk = 16;
-    return launch_trace(a64, clockwise, waves_per_eu, nblocks64, st);
+    return launch_trace(a64, clockwise, waves_per_eu, coop, nblocks64, st);
 }
 
 template <int NT>

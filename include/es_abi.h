@@ -204,6 +204,7 @@ typedef struct es_batch_shape {
     int32_t waves_per_eu;             /* register budget: 2 or 3 waves per SIMD */
     int32_t chunk;                    /* work items a wave takes from the queue at a time */
     int32_t grid_mult;                /* resident blocks per CU times this = grid size */
+    int32_t cooperative;              /* 1: a wave solves its lanes' crossing disks together (default) */
 } es_batch_shape;
 
 int es_batch_create(int32_t device, es_batch** out);
