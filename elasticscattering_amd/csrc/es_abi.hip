@@ -440,7 +440,15 @@ void plan_offsets(Plan& pl, long long seg_items)
     Segment sg;
     for (size_t i = 0; i < pl.pairs.size(); i++) {
         es::PairDev& p = pl.pairs[i];
-        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nitems + es::kBlock - 1) / es::kBlock, 1024));
+        // Per-orbit quotients the trace would otherwise form per particle: IEEE
+        // divisions here give the same bits as the device's.
+        p.R = pl.samples[p.sample].speed / p.w;  // lifetime.h:41-44
+        p.inv_R = 1.0 / p.R;
+        p.inv_w = 1.0 / p.w;
+        // About kSigmaItemsPerThread items per thread, so the block's fixed-order
+        // double-double reduction is amortised over many items.
+        const long long per_blk = (long long)es::kBlock * es::kSigmaItemsPerThread;
+        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nitems + per_blk - 1) / per_blk, 1024));
         if ((int)i > sg.p0 && item + p.nitems > seg_items) {
             sg.p1 = (int)i;
             sg.items = item;
@@ -674,6 +682,9 @@ es::SampleDev sample_dev(Field& f, int ppr, int ppq, double dist, double offx, d
     d.speed = st.particle_speed;
     d.coherent = coherent;
     d.phi_integrand_factor = ps.phi_integrand_factor;
+    d.inv_L = 1.0 / d.L;
+    d.inv_size = 1.0 / d.size;
+    d.inv_speed = 1.0 / d.speed;
     return d;
 }
 

@@ -12,6 +12,7 @@ namespace es {
 
 constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kMaxT = 8;           // temperatures per sample
+constexpr int kSigmaItemsPerThread = 8;  // temperature stage: work items per thread (block count rule)
 // int64 metric partials per block: 0 cells, 1 tested, 2 escaped, 3 at_bound,
 // 4 cap hits, 5 max tested (max), 6 max cells (max), 7 traced particles,
 // 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time
@@ -39,6 +40,7 @@ struct SampleDev {
     double dist, offx, offy;         // distance_between_positions, small_offset
     double phi_start, phi_step, alpha, speed;
     double phi_integrand_factor;     // Simulation.h:116-120
+    double inv_L, inv_size, inv_speed;  // RN(1 / cell size), RN(1 / size), RN(1 / speed)
     int coherent;
     int nT;
     double tau[kMaxT];
@@ -51,6 +53,7 @@ struct PairDev {
     int sample;
     int out;                         // this pair's row in the outputs
     double w;                        // angular speed E*B/M
+    double R, inv_R, inv_w;          // orbit radius speed / w, RN(1 / R), RN(1 / w) (set by the host)
     long long item_off;              // first work item (ascending over pairs)
     long long nitems;                // nfree * 4 * ppq
     int blk_off;                     // first temperature-stage block (ascending)

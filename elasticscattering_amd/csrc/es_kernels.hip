@@ -421,7 +421,7 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, long long item,
     g.start = sd.start;
     g.size = sd.size;
     g.L = sd.L;
-    g.inv_L = 1.0 / sd.L;
+    g.inv_L = sd.inv_L;
     g.radius = sd.radius;
     g.cpr = sd.cpr;
     cap = 4 * sd.cpr * sd.cpr + 16;
@@ -432,9 +432,9 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, long long item,
     double sphi, cphi;
     esm_sincos(phi, &sphi, &cphi);
     const double vx = cphi * sd.speed, vy = sphi * sd.speed;
-    const double R = sd.speed / w;
-    const double sx = vy * R / sd.speed;
-    const double sy = -vx * R / sd.speed;
+    const double R = pr.R;  // speed / w
+    const double sx = esm_div_rcp(vy * R, sd.speed, sd.inv_speed);
+    const double sy = esm_div_rcp(-vx * R, sd.speed, sd.inv_speed);
     const double radd = R + g.radius, rmin = R - g.radius;
     o.cx = CW ? (px - sx) : (px + sx);
     o.cy = CW ? (py - sy) : (py + sy);
@@ -446,20 +446,21 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, long long item,
     o.r2 = g.radius * g.radius;
     o.pa = phi;
     o.w = w;
-    o.inv_R = 1.0 / R;
-    o.inv_w = 1.0 / w;
+    o.inv_R = pr.inv_R;
+    o.inv_w = pr.inv_w;
     o.u0x = px - o.cx;
     o.u0y = py - o.cy;
     o.r = (float)g.radius;
     phi0 = phi;
     bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
     if (!sd.coherent) {
-        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
+        const double a = phi + sd.alpha;  // smod(a, pi / 2), details.h:79-83
+        const double remaining = a - ES_HALF_PI * floor(esm_div_rcp(a, ES_HALF_PI, ES_INV_HALF_PI));
         const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
-        bound_time = dphi / w;
+        bound_time = esm_div_rcp(dphi, w, o.inv_w);
     }
-    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
-    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
+    const int c0x = (int)(esm_div_rcp(px - g.start, g.size, sd.inv_size) * (double)(g.cpr));
+    const int c0y = (int)(esm_div_rcp(py - g.start, g.size, sd.inv_size) * (double)(g.cpr));
     d_trace_begin(st, px, py, phi, c0x, c0y);
 }
 
@@ -956,10 +957,15 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 
         // ApplySigma + IntegrateResult terms (SimulationCPU.cpp:178-222): a zero
         // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
+        if (lt_raw == 0.0) continue;  // every temperature's term is +-0
         const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
         double sphi, cphi;
         esm_sincos(phi, &sphi, &cphi);
         const double wgt = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr) * simpson_weight(p, ppq);
+        // sin, cos of the end angle for the untruncated lifetime, shared by every
+        // temperature whose cap does not cut it (the same argument, so the same bits).
+        double sa0, ca0;
+        esm_sincos(phi + ws * lt_raw, &sa0, &ca0);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             if (t < nT) {
@@ -968,8 +974,8 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
                 if (lt != 0.0) {
                     const double tau = sd.tau[t];
                     const double z = esm_exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
-                    double sa, ca;
-                    esm_sincos(phi + ws * lt, &sa, &ca);
+                    double sa = sa0, ca = ca0;
+                    if (lt != lt_raw) esm_sincos(phi + ws * lt, &sa, &ca);
                     double r = cphi - ca * z;
                     r += ws * tau * sa * z;
                     r -= ws * tau * sphi;

@@ -34,6 +34,7 @@
 #define ES_HALF_PI 1.570796326794896619231
 #define ES_INF 1.7976931348623157e308   /* DBL_MAX (constants.h:36) */
 #define ES_INV_PI2 (1.0 / ES_PI2)          /* RN(1 / 2 pi), folded by the compiler */
+#define ES_INV_HALF_PI (1.0 / ES_HALF_PI)  /* RN(2 / pi) */
 
 // Device view of the binned impurity field (Grid::GetSettings, Grid.h:82-91).
 struct DevGrid {
