@@ -56,7 +56,7 @@ struct Ctx {
     size_t bytes_held = 0;
     int cus = 0;
     int trace_bpc[2][2][4] = {{{0}}};  // resident trace blocks per CU [clockwise][coop][waves_per_eu]
-    int trace32_bpc[2] = {0, 0};  // resident FP32 trace blocks per CU [clockwise]
+    int trace32_bpc[2][2] = {{0, 0}, {0, 0}};  // resident FP32 trace blocks per CU [clockwise][coop]
     long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
 };
@@ -527,12 +527,13 @@ int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<in
     }
     int bpc = bpc64;
     if (pl.precision == ES_PRECISION_MIXED) {
-        if (c.trace32_bpc[cw] <= 0) {
-            ES_HIP(c, es::trace32_blocks_per_cu(cw, &c.trace32_bpc[cw]));
-            if (c.trace32_bpc[cw] <= 0) c.trace32_bpc[cw] = 1;
+        int& bpc32 = c.trace32_bpc[cw][co];
+        if (bpc32 <= 0) {
+            ES_HIP(c, es::trace32_blocks_per_cu(cw, co, &bpc32));
+            if (bpc32 <= 0) bpc32 = 1;
         }
         if (nblocks64) *nblocks64 = bpc64 * c.cus;
-        bpc = c.trace32_bpc[cw];
+        bpc = bpc32;
     }
     nblocks.clear();
     for (const Segment& g : pl.segs) {

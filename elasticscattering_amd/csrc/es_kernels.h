@@ -125,7 +125,7 @@ hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* b
 // Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
 hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st);
 // Mixed precision: FP32 trace of every item, then FP64 trace of the flagged ones.
-hipError_t trace32_blocks_per_cu(int clockwise, int* blocks);
+hipError_t trace32_blocks_per_cu(int clockwise, int coop, int* blocks);
 hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks32,
                               int nblocks64, hipStream_t st);
 hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st);

@@ -735,15 +735,228 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
     }
 }
 
-// The FP32 trace (es_trace32.h) over the same work items: same lane refill,
-// same start data (position, phi, orbit centre, bound time in FP64); items it
-// cannot decide are appended to a.list for the FP64 trace.
+// CreateParticle for the FP32 trace: the same start data (position, phi,
+// orbit centre, bound time in FP64) in the start point's frame.
+template <bool CW>
+__device__ __forceinline__ void t32_setup_item(const TraceArgs& a, long long item, DevGrid& g, Orb32& o, double& w,
+                                               double& bound_time, long long& out_idx, int& cap, Trace32& st)
+{
+    const int pi = pair_of_item(a.pairs, a.npairs, item);
+    const PairDev& pr = a.pairs[pi];
+    const SampleDev& sd = a.samples[pr.sample];
+    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
+    const long long k = item - pr.item_off;
+    const long long slot = k / ppp;
+    const int v = (int)(k - slot * ppp);
+    const int q = v / ppq, p = v - (v / ppq) * ppq;
+    const int pidx = sd.free_pos[slot];
+    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+    g.ixy = sd.ixy;
+    g.cell_end = sd.cell_end;
+    g.start = sd.start;
+    g.size = sd.size;
+    g.L = sd.L;
+    g.inv_L = sd.inv_L;
+    g.radius = sd.radius;
+    g.cpr = sd.cpr;
+    cap = 4 * sd.cpr * sd.cpr + 16;
+    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
+    const double py = (double)j * sd.dist + sd.offy;
+    w = pr.w;
+    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;  // lifetime.h:36-58
+    double sphi, cphi;
+    esm_sincos(phi, &sphi, &cphi);
+    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
+    const double R = pr.R;
+    const double sx = esm_div_rcp(vy * R, sd.speed, sd.inv_speed);
+    const double sy = esm_div_rcp(-vx * R, sd.speed, sd.inv_speed);
+    const double ocx = CW ? (px - sx) : (px + sx);
+    const double ocy = CW ? (py - sy) : (py + sy);
+    o.sx = px;
+    o.sy = py;
+    o.inv_r = pr.inv_R;
+    o.nxd = (ocx - px) * o.inv_r;
+    o.nyd = (ocy - py) * o.inv_r;
+    o.nx = (float)o.nxd;
+    o.ny = (float)o.nyd;
+    o.r2 = g.radius * g.radius;
+    // The reference's travel angle (GetCrossAngle, details.h:111-116)
+    // runs along the orbit against (cos phi, sin phi), in both senses.
+    o.txd = -cphi;
+    o.tyd = -sphi;
+    o.tx = (float)o.txd;
+    o.ty = (float)o.tyd;
+    o.rr = (float)(g.radius * o.inv_r);
+    o.rr2 = o.rr * o.rr;
+    o.ccw = !CW;
+    // A negative raw angle changes the reference's ranges (es_trace32.h).
+    o.quirk = phi < 0.0;
+    const float hm = tanf((float)(-0.5 * phi));
+    o.km = t32_key(Root32{hm, false});
+    o.km2 = t32_key(Root32{-hm, false});
+    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
+    if (!sd.coherent) {
+        const double a0 = phi + sd.alpha;
+        const double remaining = a0 - ES_HALF_PI * floor(esm_div_rcp(a0, ES_HALF_PI, ES_INV_HALF_PI));
+        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
+        bound_time = esm_div_rcp(dphi, w, pr.inv_w);
+    }
+    const int c0x = (int)(esm_div_rcp(px - g.start, g.size, sd.inv_size) * (double)(g.cpr));
+    const int c0y = (int)(esm_div_rcp(py - g.start, g.size, sd.inv_size) * (double)(g.cpr));
+    t32_begin(st, c0x, c0y);
+}
+
+// Lifetime of an ended FP32 trace; items FP32 cannot decide go on the list
+// for the FP64 trace.
+__device__ __forceinline__ void t32_store_item(const TraceArgs& a, long long item, long long out_idx, double w,
+                                               double bound_time, const Trace32& st, int flags, double theta)
+{
+    double lt = 0.0;
+    if (flags == 0) {
+        lt = theta / w;
+        if (bound_time < ES_INF && fabs(lt - bound_time) <= 1e-5 * bound_time) {
+            flags = ES_F_RECOMPUTE;  // FP32 cannot tell which side of the bound
+        } else if (lt > bound_time) {  // lifetime.h:142-146
+            lt = bound_time;
+            flags = ES_F_AT_BOUND;
+        }
+    }
+    if (flags & ES_F_RECOMPUTE) {
+        const int idx = atomicAdd(a.list_count, 1);
+        a.list[idx] = (int32_t)item;
+    }
+    a.lt_item[item] = lt;
+    a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
+    a.x_item[item] = 0;
+    a.f_item[item] = (uint8_t)flags;
+    if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+}
+
+// The FP32 trace (es_trace32.h) over the same work items, one lane per
+// particle: same lane refill, same start data; items it cannot decide are
+// appended to a.list for the FP64 trace.
 template <bool CW>
 __device__ __forceinline__ void trace32_body(const TraceArgs& a)
 {
+    const long long total = a.total_items;
+    long long q_next = 0, q_end = 0;
+    bool exhausted = false;
+    bool active = false;
+    long long item = 0, out_idx = -1;
+    int cap = 0;
+    DevGrid g;
+    Orb32 o;
+    double w = 1.0, bound_time = 0.0;
+    Trace32 st;
+
+    while (true) {
+        long long got;
+        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        if (got >= 0) {
+            item = got;
+            t32_setup_item<CW>(a, item, g, o, w, bound_time, out_idx, cap, st);
+            active = true;
+        }
+        if (__ballot(active) == 0ull) break;
+
+        if (active) {
+            int flags = 0;
+            double theta = 0.0;
+            if (t32_step(o, g, cap, st, flags, theta)) {
+                t32_store_item(a, item, out_idx, w, bound_time, st, flags, theta);
+                active = false;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------ wave-cooperative FP32 trace
+//
+// As trace_coop_body for the FP32 trace: each lane screens its cell's disks
+// in FP32 (t32_screen); the disks that pass go on the wave's stack and their
+// FP64-formed test (t32_disk) runs 64 at a time for any lane.  Per owner the
+// LDS keeps: the smallest (key, disk position * 2 + root) of the roots in
+// range — the order the one-lane loop keeps on equal keys — the smallest key
+// within error of a range end, an "undecidable" bit and the crossing count.
+// The owner then solves its winning disk again for the root itself.  A
+// crossing near a range end is flagged when it comes within error of the
+// cell's first hit (the one-lane loop compares with the first hit so far, a
+// superset of the same cases).
+
+struct Coop32Wave {
+    double dctx[8][64];              // per lane: sx sy inv_r nxd nyd txd tyd r2
+    float fctx[10][64];              // per lane: glo ghi rr k_in band_in b_hi a1 b1 a2 b2
+    const double2* ixy[64];          // per lane: its sample's binned field
+    unsigned long long best[64];     // per lane: (key bits << 32) | (position * 2 + root), ~0 when none
+    unsigned int atend[64];          // per lane: key bits of the first root near a range end
+    unsigned int bad[64];            // per lane: undecidable disk seen
+    unsigned int cross[64];          // per lane: crossing disks (CirclesCross true)
+    int32_t qidx[kCoopStack];        // queued disk: position in its owner's field
+    uint8_t qown[kCoopStack];        // queued disk: owner lane
+};
+
+__device__ __forceinline__ Disk32Ctx t32_ctx_lds(const Coop32Wave& L, int own)
+{
+    Disk32Ctx d;
+    d.sx = L.dctx[0][own];
+    d.sy = L.dctx[1][own];
+    d.inv_r = L.dctx[2][own];
+    d.nxd = L.dctx[3][own];
+    d.nyd = L.dctx[4][own];
+    d.txd = L.dctx[5][own];
+    d.tyd = L.dctx[6][own];
+    d.r2 = L.dctx[7][own];
+    d.glo = L.fctx[0][own];
+    d.ghi = L.fctx[1][own];
+    d.rr = L.fctx[2][own];
+    d.k_in = L.fctx[3][own];
+    d.band_in = L.fctx[4][own];
+    d.b_hi = L.fctx[5][own];
+    d.a1 = L.fctx[6][own];
+    d.b1 = L.fctx[7][own];
+    d.a2 = L.fctx[8][own];
+    d.b2 = L.fctx[9][own];
+    return d;
+}
+
+__device__ __forceinline__ void t32_coop_solve(Coop32Wave& L, int k)
+{
+    const int own = L.qown[k];
+    const int gi = L.qidx[k];
+    const Disk32Ctx d = t32_ctx_lds(L, own);
+    const double2 p = L.ixy[own][gi];
+    bool cross;
+    Root32 rt[2];
+    float kk[2];
+    bool in[2], at_end[2];
+    const bool ok = t32_disk(d, p.x - d.sx, p.y - d.sy, cross, rt, kk, in, at_end);
+    if (cross) atomicAdd(&L.cross[own], 1u);
+    if (!ok) {
+        atomicOr(&L.bad[own], 1u);
+    } else if (cross) {
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            if (in[r])
+                atomicMin(&L.best[own],
+                          ((unsigned long long)__float_as_uint(kk[r]) << 32) | (unsigned long long)(2 * gi + r));
+            if (at_end[r]) atomicMin(&L.atend[own], __float_as_uint(kk[r]));
+        }
+    }
+}
+
+template <bool CW>
+__device__ __forceinline__ void trace32_coop_body(const TraceArgs& a)
+{
+    __shared__ Coop32Wave sh_coop32[kBlock / 64];
+    Coop32Wave& L = sh_coop32[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const long long total = a.total_items;
+    L.best[lane] = ~0ull;
+    L.atend[lane] = 0xFFFFFFFFu;
+    L.bad[lane] = 0u;
+    L.cross[lane] = 0u;
 
     long long q_next = 0, q_end = 0;
     bool exhausted = false;
@@ -756,119 +969,120 @@ __device__ __forceinline__ void trace32_body(const TraceArgs& a)
     Trace32 st;
 
     while (true) {
-        unsigned long long need = __ballot(!active);
-        while (need != 0ull && !exhausted) {
-            if (q_next >= q_end) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-                base = __shfl(base, 0, 64);
-                if ((long long)base >= total) {
-                    exhausted = true;
-                    break;
-                }
-                q_next = (long long)base;
-                q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
-            }
-            const long long left = q_end - q_next;
-            const int n = __popcll(need);
-            const int take = (int)((long long)n < left ? (long long)n : left);
-            if (!active) {
-                const int rank = __popcll(need & below);
-                if (rank < take) {
-                    item = q_next + rank;
-                    const int pi = pair_of_item(a.pairs, a.npairs, item);
-                    const PairDev& pr = a.pairs[pi];
-                    const SampleDev& sd = a.samples[pr.sample];
-                    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
-                    const long long k = item - pr.item_off;
-                    const long long slot = k / ppp;
-                    const int v = (int)(k - slot * ppp);
-                    const int q = v / ppq, p = v - (v / ppq) * ppq;
-                    const int pidx = sd.free_pos[slot];
-                    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
-                    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
-                    g.ixy = sd.ixy;
-                    g.cell_end = sd.cell_end;
-                    g.start = sd.start;
-                    g.size = sd.size;
-                    g.L = sd.L;
-                    g.radius = sd.radius;
-                    g.cpr = sd.cpr;
-                    cap = 4 * sd.cpr * sd.cpr + 16;
-                    const double px = (double)i * sd.dist + sd.offx;  // SimulationCPU.cpp:114
-                    const double py = (double)j * sd.dist + sd.offy;
-                    w = pr.w;
-                    const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;  // lifetime.h:36-58
-                    double sphi, cphi;
-                    esm_sincos(phi, &sphi, &cphi);
-                    const double vx = cphi * sd.speed, vy = sphi * sd.speed;
-                    const double R = sd.speed / w;
-                    const double sx = vy * R / sd.speed;
-                    const double sy = -vx * R / sd.speed;
-                    const double ocx = CW ? (px - sx) : (px + sx);
-                    const double ocy = CW ? (py - sy) : (py + sy);
-                    o.sx = px;
-                    o.sy = py;
-                    o.inv_r = 1.0 / R;
-                    o.nxd = (ocx - px) * o.inv_r;
-                    o.nyd = (ocy - py) * o.inv_r;
-                    o.nx = (float)o.nxd;
-                    o.ny = (float)o.nyd;
-                    o.r2 = g.radius * g.radius;
-                    // The reference's travel angle (GetCrossAngle, details.h:111-116)
-                    // runs along the orbit against (cos phi, sin phi), in both senses.
-                    o.txd = -cphi;
-                    o.tyd = -sphi;
-                    o.tx = (float)o.txd;
-                    o.ty = (float)o.tyd;
-                    o.rr = (float)(g.radius * o.inv_r);
-                    o.rr2 = o.rr * o.rr;
-                    o.ccw = !CW;
-                    // A negative raw angle changes the reference's ranges (es_trace32.h).
-                    o.quirk = phi < 0.0;
-                    const float hm = tanf((float)(-0.5 * phi));
-                    o.km = t32_key(Root32{hm, false});
-                    o.km2 = t32_key(Root32{-hm, false});
-                    bound_time = ES_INF;  // GetBoundTime (details.h:219-228)
-                    if (!sd.coherent) {
-                        const double remaining = d_smod(phi + sd.alpha, ES_HALF_PI);
-                        const double dphi = CW ? remaining : (2.0 * sd.alpha - remaining);
-                        bound_time = dphi / w;
-                    }
-                    const int c0x = (int)((px - g.start) / g.size * (double)(g.cpr));
-                    const int c0y = (int)((py - g.start) / g.size * (double)(g.cpr));
-                    t32_begin(st, c0x, c0y);
-                    active = true;
-                }
-            }
-            q_next += take;
-            need = __ballot(!active);
+        long long got;
+        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        if (got >= 0) {
+            item = got;
+            t32_setup_item<CW>(a, item, g, o, w, bound_time, out_idx, cap, st);
+            L.ixy[lane] = g.ixy;
+            active = true;
         }
         if (__ballot(active) == 0ull) break;
 
+        // This lane's cell exit and ranges; a cell FP32 cannot decide ends the trace.
+        Cell32 c;
+        int s = 0, n = 0;
+        bool go = false;
         if (active) {
+            if (t32_exit(o, g, st, c)) {
+                const Disk32Ctx d = t32_ctx(o, c);
+                L.dctx[0][lane] = d.sx;
+                L.dctx[1][lane] = d.sy;
+                L.dctx[2][lane] = d.inv_r;
+                L.dctx[3][lane] = d.nxd;
+                L.dctx[4][lane] = d.nyd;
+                L.dctx[5][lane] = d.txd;
+                L.dctx[6][lane] = d.tyd;
+                L.dctx[7][lane] = d.r2;
+                L.fctx[0][lane] = d.glo;
+                L.fctx[1][lane] = d.ghi;
+                L.fctx[2][lane] = d.rr;
+                L.fctx[3][lane] = d.k_in;
+                L.fctx[4][lane] = d.band_in;
+                L.fctx[5][lane] = d.b_hi;
+                L.fctx[6][lane] = d.a1;
+                L.fctx[7][lane] = d.b1;
+                L.fctx[8][lane] = d.a2;
+                L.fctx[9][lane] = d.b2;
+                const int ci = st.cy * g.cpr + st.cx;
+                s = (ci > 0) ? g.cell_end[ci - 1] : 0;
+                n = g.cell_end[ci] - s;
+                st.cells += 1;
+                st.tested += n;
+                go = true;
+            } else {
+                t32_store_item(a, item, out_idx, w, bound_time, st, ES_F_RECOMPUTE, 0.0);
+                active = false;
+            }
+        }
+        const float glo = -2.0f * o.rr + o.rr2, ghi = 2.0f * o.rr + o.rr2;
+        wave_lds_sync();
+
+        int top = 0;  // wave-uniform
+        for (int i = 0;; i += 4) {
+            const bool more = __ballot(i < n) != 0ull;
+            if (more) {
+                double2 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
+                int hits = 0;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    hits |= ((i + u < n) && t32_screen(o, glo, ghi, p[u].x - o.sx, p[u].y - o.sy)) ? (1 << u) : 0;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool h = (hits >> u) & 1;
+                    const unsigned long long m = __ballot(h);
+                    if (h) {
+                        const int pos = top + __popcll(m & below);
+                        L.qidx[pos] = s + i + u;
+                        L.qown[pos] = (uint8_t)lane;
+                    }
+                    top += __popcll(m);
+                }
+                wave_lds_sync();
+            }
+            while (top >= 64 || (!more && top > 0)) {
+                const int m = (top < 64) ? top : 64;
+                if (lane < m) t32_coop_solve(L, top - m + lane);
+                top -= m;
+                wave_lds_sync();
+            }
+            if (!more) break;
+        }
+
+        if (go) {
+            st.cross_calls += (int)L.cross[lane];
+            bool bad = L.bad[lane] != 0u;
+            const unsigned long long b = L.best[lane];
+            const unsigned int ae = L.atend[lane];
+            L.best[lane] = ~0ull;
+            L.atend[lane] = 0xFFFFFFFFu;
+            L.bad[lane] = 0u;
+            L.cross[lane] = 0u;
+            float best_k = 2.0f;
+            Root32 best = st.best;
+            if (b != ~0ull) {
+                best_k = __uint_as_float((unsigned int)(b >> 32));
+                const int gi = (int)((unsigned int)b >> 1);
+                const int r = (int)(b & 1ull);
+                const Disk32Ctx d = t32_ctx_lds(L, lane);
+                const double2 p = g.ixy[gi];
+                bool cross;
+                Root32 rt[2];
+                float kk[2];
+                bool in[2], at_end[2];
+                (void)t32_disk(d, p.x - d.sx, p.y - d.sy, cross, rt, kk, in, at_end);
+                best = r ? rt[1] : rt[0];
+            }
+            if (ae != 0xFFFFFFFFu) {
+                const float ak = __uint_as_float(ae);
+                if (ak < best_k + (2e-6f * ak + T32_ABS)) bad = true;
+            }
             int flags = 0;
             double theta = 0.0;
-            if (t32_step(o, g, cap, st, flags, theta)) {
-                double lt = 0.0;
-                if (flags == 0) {
-                    lt = theta / w;
-                    if (bound_time < ES_INF && fabs(lt - bound_time) <= 1e-5 * bound_time) {
-                        flags = ES_F_RECOMPUTE;  // FP32 cannot tell which side of the bound
-                    } else if (lt > bound_time) {  // lifetime.h:142-146
-                        lt = bound_time;
-                        flags = ES_F_AT_BOUND;
-                    }
-                }
-                if (flags & ES_F_RECOMPUTE) {
-                    const int idx = atomicAdd(a.list_count, 1);
-                    a.list[idx] = (int32_t)item;
-                }
-                a.lt_item[item] = lt;
-                a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
-                a.x_item[item] = 0;
-                a.f_item[item] = (uint8_t)flags;
-                if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+            if (t32_finish(o, cap, c, st, bad, best_k, best, flags, theta)) {
+                t32_store_item(a, item, out_idx, w, bound_time, st, flags, theta);
                 active = false;
             }
         }
@@ -904,6 +1118,12 @@ template <bool CW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_trace32(TraceArgs a)
 {
     trace32_body<CW>(a);
+}
+
+template <bool CW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_trace32_c(TraceArgs a)
+{
+    trace32_coop_body<CW>(a);
 }
 
 // --------------------------------------------------------------------- sigma
@@ -1273,10 +1493,15 @@ hipError_t launch_images(const SampleDev* sample, const PairDev* pair, int clock
     return hipGetLastError();
 }
 
-hipError_t trace32_blocks_per_cu(int clockwise, int* blocks)
+static const void* trace32_kernel(int clockwise, int coop)
 {
-    return clockwise ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace32<true>, kBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_trace32<false>, kBlock, 0);
+    if (coop) return clockwise ? (const void*)k_trace32_c<true> : (const void*)k_trace32_c<false>;
+    return clockwise ? (const void*)k_trace32<true> : (const void*)k_trace32<false>;
+}
+
+hipError_t trace32_blocks_per_cu(int clockwise, int coop, int* blocks)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, trace32_kernel(clockwise, coop), kBlock, 0);
 }
 
 hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks32,
@@ -1289,11 +1514,10 @@ hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_e
     if (e != hipSuccess || a.total_items <= 0) return e;
     TraceArgs a32 = a;
     a32.list_mode = 0;
-    if (clockwise)
-        hipLaunchKernelGGL((k_trace32<true>), dim3(nblocks32), dim3(kBlock), 0, st, a32);
-    else
-        hipLaunchKernelGGL((k_trace32<false>), dim3(nblocks32), dim3(kBlock), 0, st, a32);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    void* params[] = {&a32};
+    if ((e = hipLaunchKernel(trace32_kernel(clockwise, coop), dim3(nblocks32), dim3(kBlock), params, 0, st)) !=
+        hipSuccess)
+        return e;
     // The flagged items, in FP64: the count is read on the device, so no host
     // round trip separates the two traces.
     TraceArgs a64 = a;

@@ -153,11 +153,20 @@ __device__ __forceinline__ void t32_begin(Trace32& st, int c0x, int c0y)
 
 __device__ __forceinline__ bool t32_near(float a, float b, float band) { return fabsf(a - b) <= band; }
 
-// One cell.  Returns true when the trace has ended; *flags then holds
-// ES_F_ESCAPED / ES_F_CAP / ES_F_RECOMPUTE, and *theta the travel angle of
-// the hit (0 without one).
-__device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int cap, Trace32& st, int& flags,
-                                         double& theta)
+// Exit of the current cell and the reference's accepted travel angles in it.
+struct Cell32 {
+    float k_in, band_in;    // entry key and its error band
+    float k_out, band_out;  // exit key (2 without an exit) and its band
+    float a1, b1, a2, b2;   // accepted key ranges (a1, b1), (a2, b2)
+    float b_hi;             // upper end of both ranges plus the band
+    int e_out;              // exit edge (-1: none)
+    int ncx, ncy;           // next cell
+    bool avail;             // a next cell exists and the trace goes on there
+};
+
+// First half of one cell: the exit over the 4 edges and the key ranges in
+// which disks count.  Returns false when FP32 cannot decide them.
+__device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace32& st, Cell32& c)
 {
     const double L = g.L;
     const double x0 = g.start + (double)st.cx * L, y0 = g.start + (double)st.cy * L;
@@ -177,15 +186,15 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
 #pragma unroll 1
     for (int e = 0; e < 4; e++) {
         const bool vert = (e >= 2);                  // right / left edges are x = const
-        const float c = (e == 0) ? yh : (e == 1) ? yl : (e == 2) ? xh : xl;
+        const float cc = (e == 0) ? yh : (e == 1) ? yl : (e == 2) ? xh : xl;
         const float n_c = vert ? o.nx : o.ny, t_c = vert ? o.tx : o.ty;     // across the edge
         const float n_o = vert ? o.ny : o.nx, t_o = vert ? o.ty : o.tx;     // along the edge
         const float lo = vert ? yl : xl, hi = vert ? yh : xh;
         const float cband = T32_REL * (fabsf(lo) + fabsf(hi));
-        // n_c (1 - cos) + t_c sin = c  ->  (2 n_c - c) w^2 + 2 t_c w - c = 0
-        const float A = 2.0f * n_c - c;
-        const float disc = t_c * t_c + c * (2.0f * n_c - c);
-        const float dscale = t_c * t_c + fabsf(2.0f * n_c * c) + c * c;
+        // n_c (1 - cos) + t_c sin = cc  ->  (2 n_c - cc) w^2 + 2 t_c w - cc = 0
+        const float A = 2.0f * n_c - cc;
+        const float disc = t_c * t_c + cc * (2.0f * n_c - cc);
+        const float dscale = t_c * t_c + fabsf(2.0f * n_c * cc) + cc * cc;
         if (disc <= T32_REL * dscale) {
             if (disc > -T32_REL * dscale) {
                 // near tangency: undecidable if the touching point lies on the edge
@@ -199,7 +208,7 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
             continue;
         }
         Root32 rt[2];
-        if (!t32_roots(A, t_c, -c, disc, rt[0], rt[1])) {
+        if (!t32_roots(A, t_c, -cc, disc, rt[0], rt[1])) {
             bad = true;
             continue;
         }
@@ -269,88 +278,117 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
             }
         }
     }
-    if (bad) {
-        flags = ES_F_RECOMPUTE;
-        theta = 0.0;
-        return true;
-    }
+    c.k_in = k_in;
+    c.band_in = band_in;
+    c.k_out = k_out;
+    c.band_out = band_out;
+    c.a1 = a1;
+    c.b1 = b1;
+    c.a2 = a2;
+    c.b2 = b2;
+    c.b_hi = fmaxf(b1, b2) + band_out;
+    c.e_out = e_out;
+    c.ncx = ncx;
+    c.ncy = ncy;
+    c.avail = avail;
+    return !bad;
+}
 
-    // ---- impurities of the cell: crossings with keys in the accepted ranges
-    const int ci = st.cy * g.cpr + st.cx;
-    const int s = (ci > 0) ? g.cell_end[ci - 1] : 0;
-    const int en = g.cell_end[ci];
-    st.cells += 1;
-    st.tested += (en - s);
-    const float glo = -2.0f * o.rr + o.rr2, ghi = 2.0f * o.rr + o.rr2;
-    const float b_hi = fmaxf(b1, b2) + band_out;
-    float best_k = st.best_k;
-    Root32 best = st.best;
-    for (int i = s; i < en; i += 4) {
-        double2 p[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) p[u] = (i + u < en) ? g.ixy[i + u] : make_double2(0.0, 0.0);
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (i + u >= en) continue;
-            const double dx = p[u].x - o.sx, dy = p[u].y - o.sy;
-            // FP32 screen of CirclesCross: |P - C|^2 / R^2 - 1 = |p|^2 - 2 p.n
-            {
-                const float px = (float)(dx * o.inv_r), py = (float)(dy * o.inv_r);
-                const float b0 = px * o.nx + py * o.ny;
-                const float pp0 = px * px + py * py;
-                const float g0 = pp0 - 2.0f * b0;
-                const float band0 = 1e-5f * (pp0 + 2.0f * fabsf(b0)) + 1e-12f;
-                if (g0 < glo - band0 || g0 > ghi + band0) continue;
-            }
-            // candidates: the quantities that cancel, from FP64 differences
-            const double ppn = (dx * dx + dy * dy) * (o.inv_r * o.inv_r);
-            const double bn = (dx * o.nxd + dy * o.nyd) * o.inv_r;
-            const float a = (float)((dx * o.txd + dy * o.tyd) * o.inv_r);  // along the travel
-            const double qd = (dx * dx + dy * dy - o.r2) * (o.inv_r * o.inv_r);
-            const float q = (float)qd;
-            const float gg = (float)(ppn - 2.0 * bn);       // |P - C|^2 / R^2 - 1
-            const float A = (float)((4.0 - 4.0 * bn) + qd);  // 4 (1 - b) + q (small near theta = pi)
-            const float gband = T32_REL * (fabsf(gg) + 2.0f * o.rr) + 1e-14f * (float)(ppn + 2.0 * fabs(bn));
-            if (fabsf(gg - glo) <= gband || fabsf(gg - ghi) <= gband) {
-                // within error of tangency: undecidable if the touching point is in range
-                Root32 tp;
-                tp.inv = fabsf(2.0f * a) > fabsf(A);
-                tp.z = tp.inv ? (A / (2.0f * a)) : (2.0f * a / A);
-                const float kt = t32_key(tp);
-                if (kt > k_in - band_in && kt < b_hi) bad = true;
-                continue;
-            }
-            if (!(gg > glo && gg < ghi)) continue;
-            st.cross_calls += 1;
-            if (!(q > 0.0f)) {  // the start lies outside every disk (k_positions)
-                bad = true;
-                continue;
-            }
-            // |X - P|^2 = rr^2  ->  A w^2 - 4 a w + q = 0, disc = (2a)^2 - A q
-            const float disc = (gg - glo) * (ghi - gg);
-            Root32 rt[2];
-            if (!t32_roots(A, -2.0f * a, q, disc, rt[0], rt[1])) {
-                bad = true;
-                continue;
-            }
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const float kk = t32_key(rt[k]);
-                const bool in = (kk > a1 && kk < b1) || (kk > a2 && kk < b2);
-                // A crossing within error of a range end (a cell boundary) may
-                // belong to either cell, and FP32 keys can tie there.
-                const float rb = 2e-6f * kk + T32_ABS;
-                const bool at_end = (a1 > 0.0f && t32_near(kk, a1, rb)) || (b1 < 2.0f && t32_near(kk, b1, rb)) ||
-                                    (a2 > 0.0f && a2 < 2.0f && t32_near(kk, a2, rb)) ||
-                                    (b2 < 2.0f && t32_near(kk, b2, rb));
-                if (at_end && kk < best_k + rb) bad = true;
-                if (in && kk < best_k) {
-                    best_k = kk;
-                    best = rt[k];
-                }
-            }
-        }
+// What the FP64-formed disk test needs of the orbit and the cell.
+struct Disk32Ctx {
+    double sx, sy, inv_r, nxd, nyd, txd, tyd, r2;
+    float glo, ghi, rr;
+    float k_in, band_in, b_hi, a1, b1, a2, b2;
+};
+
+__device__ __forceinline__ Disk32Ctx t32_ctx(const Orb32& o, const Cell32& c)
+{
+    Disk32Ctx d;
+    d.sx = o.sx;
+    d.sy = o.sy;
+    d.inv_r = o.inv_r;
+    d.nxd = o.nxd;
+    d.nyd = o.nyd;
+    d.txd = o.txd;
+    d.tyd = o.tyd;
+    d.r2 = o.r2;
+    d.glo = -2.0f * o.rr + o.rr2;
+    d.ghi = 2.0f * o.rr + o.rr2;
+    d.rr = o.rr;
+    d.k_in = c.k_in;
+    d.band_in = c.band_in;
+    d.b_hi = c.b_hi;
+    d.a1 = c.a1;
+    d.b1 = c.b1;
+    d.a2 = c.a2;
+    d.b2 = c.b2;
+    return d;
+}
+
+// FP32 screen of CirclesCross (details.h:141-151) for the disk at offset
+// (dx, dy) from the start: |P - C|^2 / R^2 - 1 = |p|^2 - 2 p.n, with a band.
+// False only for disks that certainly miss the orbit.
+__device__ __forceinline__ bool t32_screen(const Orb32& o, float glo, float ghi, double dx, double dy)
+{
+    const float px = (float)(dx * o.inv_r), py = (float)(dy * o.inv_r);
+    const float b0 = px * o.nx + py * o.ny;
+    const float pp0 = px * px + py * py;
+    const float g0 = pp0 - 2.0f * b0;
+    const float band0 = 1e-5f * (pp0 + 2.0f * fabsf(b0)) + 1e-12f;
+    return !(g0 < glo - band0 || g0 > ghi + band0);
+}
+
+// One disk that passed the screen, with the quantities that cancel formed
+// from FP64 differences.  Returns false when FP32 cannot decide it.  Else:
+// *cross = CirclesCross; for a crossing disk, its two roots rt[k] with keys
+// kk[k], whether each lies in an accepted range (in[k]) and whether it lies
+// within error of a range end (at_end[k]).
+__device__ __forceinline__ bool t32_disk(const Disk32Ctx& c, double dx, double dy, bool& cross, Root32 rt[2],
+                                         float kk[2], bool in[2], bool at_end[2])
+{
+    cross = false;
+    const double ppn = (dx * dx + dy * dy) * (c.inv_r * c.inv_r);
+    const double bn = (dx * c.nxd + dy * c.nyd) * c.inv_r;
+    const float a = (float)((dx * c.txd + dy * c.tyd) * c.inv_r);  // along the travel
+    const double qd = (dx * dx + dy * dy - c.r2) * (c.inv_r * c.inv_r);
+    const float q = (float)qd;
+    const float gg = (float)(ppn - 2.0 * bn);       // |P - C|^2 / R^2 - 1
+    const float A = (float)((4.0 - 4.0 * bn) + qd);  // 4 (1 - b) + q (small near theta = pi)
+    const float gband = T32_REL * (fabsf(gg) + 2.0f * c.rr) + 1e-14f * (float)(ppn + 2.0 * fabs(bn));
+    if (fabsf(gg - c.glo) <= gband || fabsf(gg - c.ghi) <= gband) {
+        // within error of tangency: undecidable if the touching point is in range
+        Root32 tp;
+        tp.inv = fabsf(2.0f * a) > fabsf(A);
+        tp.z = tp.inv ? (A / (2.0f * a)) : (2.0f * a / A);
+        const float kt = t32_key(tp);
+        return !(kt > c.k_in - c.band_in && kt < c.b_hi);
     }
+    if (!(gg > c.glo && gg < c.ghi)) return true;
+    cross = true;
+    if (!(q > 0.0f)) return false;  // the start lies outside every disk (k_positions)
+    // |X - P|^2 = rr^2  ->  A w^2 - 4 a w + q = 0, disc = (2a)^2 - A q
+    const float disc = (gg - c.glo) * (c.ghi - gg);
+    if (!t32_roots(A, -2.0f * a, q, disc, rt[0], rt[1])) return false;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        kk[k] = t32_key(rt[k]);
+        in[k] = (kk[k] > c.a1 && kk[k] < c.b1) || (kk[k] > c.a2 && kk[k] < c.b2);
+        // A crossing within error of a range end (a cell boundary) may belong
+        // to either cell, and FP32 keys can tie there.
+        const float rb = 2e-6f * kk[k] + T32_ABS;
+        at_end[k] = (c.a1 > 0.0f && t32_near(kk[k], c.a1, rb)) || (c.b1 < 2.0f && t32_near(kk[k], c.b1, rb)) ||
+                    (c.a2 > 0.0f && c.a2 < 2.0f && t32_near(kk[k], c.a2, rb)) ||
+                    (c.b2 < 2.0f && t32_near(kk[k], c.b2, rb));
+    }
+    return true;
+}
+
+// Second half of one cell, after its disks: best_k / best the first hit in
+// range (best_k = 2 without one), bad whether anything was undecidable.
+// Returns true when the trace has ended (flags, theta as for t32_step).
+__device__ __forceinline__ bool t32_finish(const Orb32& o, int cap, const Cell32& c, Trace32& st, bool bad,
+                                           float best_k, Root32 best, int& flags, double& theta)
+{
     st.best_k = best_k;
     st.best = best;
     if (!bad && o.quirk && best_k < 2.0f &&
@@ -367,7 +405,7 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
         theta = t32_theta(best);
         return true;
     }
-    if (!avail) {
+    if (!c.avail) {
         flags = ES_F_ESCAPED;
         theta = 0.0;
         return true;
@@ -377,9 +415,66 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
         theta = 0.0;
         return true;
     }
-    st.cx = ncx;
-    st.cy = ncy;
+    st.cx = c.ncx;
+    st.cy = c.ncy;
+    const int e_out = c.e_out;
     st.in_edge = (e_out == 0) ? 1 : (e_out == 1) ? 0 : (e_out == 2) ? 3 : 2;  // the shared edge, seen from the new cell
-    st.k_in = k_out;
+    st.k_in = c.k_out;
     return false;
+}
+
+// One cell, one lane.  Returns true when the trace has ended; *flags then
+// holds ES_F_ESCAPED / ES_F_CAP / ES_F_RECOMPUTE, and *theta the travel angle
+// of the hit (0 without one).
+__device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int cap, Trace32& st, int& flags,
+                                         double& theta)
+{
+    Cell32 c;
+    if (!t32_exit(o, g, st, c)) {
+        flags = ES_F_RECOMPUTE;
+        theta = 0.0;
+        return true;
+    }
+    // ---- impurities of the cell: crossings with keys in the accepted ranges
+    const int ci = st.cy * g.cpr + st.cx;
+    const int s = (ci > 0) ? g.cell_end[ci - 1] : 0;
+    const int en = g.cell_end[ci];
+    st.cells += 1;
+    st.tested += (en - s);
+    const Disk32Ctx d = t32_ctx(o, c);
+    bool bad = false;
+    float best_k = st.best_k;
+    Root32 best = st.best;
+    for (int i = s; i < en; i += 4) {
+        double2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = (i + u < en) ? g.ixy[i + u] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i + u >= en) continue;
+            const double dx = p[u].x - o.sx, dy = p[u].y - o.sy;
+            if (!t32_screen(o, d.glo, d.ghi, dx, dy)) continue;
+            bool cross;
+            Root32 rt[2];
+            float kk[2];
+            bool in[2], at_end[2];
+            if (!t32_disk(d, dx, dy, cross, rt, kk, in, at_end)) {
+                bad = true;
+                st.cross_calls += cross ? 1 : 0;
+                continue;
+            }
+            if (!cross) continue;
+            st.cross_calls += 1;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const float rb = 2e-6f * kk[k] + T32_ABS;
+                if (at_end[k] && kk[k] < best_k + rb) bad = true;
+                if (in[k] && kk[k] < best_k) {
+                    best_k = kk[k];
+                    best = rt[k];
+                }
+            }
+        }
+    }
+    return t32_finish(o, cap, c, st, bad, best_k, best, flags, theta);
 }

@@ -199,10 +199,11 @@ def test_tuner_shapes_give_same_results(es, batch_eager):
     announce("tuner over launch shapes")
     best, t = b.tune()
     assert best["waves_per_eu"] in (2, 3) and best["chunk"] in (32, 64, 128, 256, 512)
-    assert best["grid_mult"] in (1, 2) and t > 0
+    assert best["grid_mult"] in (1, 2) and best["cooperative"] in (0, 1) and t > 0
     assert b.shape == best
     _same_results(eager, b.run())
-    for shape in ((2, 32, 2), (3, 512, 1)):
+    # one lane per particle and the wave-cooperative crossing solves: same bits
+    for shape in ((2, 32, 2, 1), (3, 512, 1, 1), (2, 32, 2, 0), (3, 512, 1, 0), (3, 64, 1, 0)):
         b.set_shape(*shape)
         _same_results(eager, b.run())
     b.set_shape(3, 128, 1)
