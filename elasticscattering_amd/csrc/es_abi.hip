@@ -55,7 +55,7 @@ struct Ctx {
     size_t byte_cap = 0;  // 0: unlimited; else refuse allocations beyond this total
     size_t bytes_held = 0;
     int cus = 0;
-    int trace_bpc[2][2][4] = {{{0}}};  // resident trace blocks per CU [clockwise][coop][waves_per_eu]
+    int trace_bpc[2][3][4] = {{{0}}};  // resident trace blocks per CU [clockwise][coop][waves_per_eu]
     int trace32_bpc[2][2] = {{0, 0}, {0, 0}};  // resident FP32 trace blocks per CU [clockwise][coop]
     long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
@@ -380,8 +380,12 @@ struct Shape {
     int waves_per_eu = 3;
     int chunk = 128;
     int grid_mult = 1;
-    int coop = 1;  // wave-cooperative crossing solves in the FP64 trace
+    int coop = -1;  // FP64 trace: 0 per lane, 1 wave-cooperative, 2 cooperative + screened, -1 by the field
 };
+
+// Screened solves pay where many disks cross the orbit in a cell: dense fields.
+constexpr double kScreenPopulation = 10.0;
+
 
 // Device buffers and the optional captured graph of the stage kernels.
 struct Stages {
@@ -511,6 +515,20 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     return ES_OK;
 }
 
+// The trace variant of a shape for a plan: `coop` as set, or for -1 the
+// screened cooperative trace when the work items' fields hold on average at
+// least kScreenPopulation indexed impurities per cell, else the cooperative one.
+int resolve_coop(const Shape& sh, const Plan& pl)
+{
+    if (sh.coop >= 0) return sh.coop;
+    double items = 0.0, pop = 0.0;
+    for (const es::PairDev& p : pl.pairs) {
+        items += (double)p.nitems;
+        pop += (double)p.nitems * pl.samples[p.sample].cell_population;
+    }
+    return (items > 0.0 && pop >= kScreenPopulation * items) ? 2 : 1;
+}
+
 // Trace grid size of every segment: enough resident blocks to fill the chip
 // (times grid_mult), never more blocks than the segment has items for.  With
 // mixed precision, nblocks holds the FP32 trace's grid and nblocks64 the FP64
@@ -519,7 +537,7 @@ int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<in
                  int* nblocks64 = nullptr)
 {
     const int w = (sh.waves_per_eu == 2) ? 2 : 3;
-    const int co = sh.coop ? 1 : 0;
+    const int co = resolve_coop(sh, pl);
     int& bpc64 = c.trace_bpc[cw][co][w];
     if (bpc64 <= 0) {
         ES_HIP(c, es::trace_blocks_per_cu(cw, w, co, &bpc64));
@@ -572,10 +590,10 @@ int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trac
             if (pl.precision == ES_PRECISION_MIXED) {
                 a.list = P<int32_t>(s.list);
                 a.list_count = P<int32_t>(s.list_count);
-                ES_HIP(c, es::launch_trace_mixed(a, pl.clockwise, sh.waves_per_eu, sh.coop, nblocks[si], nblocks64,
+                ES_HIP(c, es::launch_trace_mixed(a, pl.clockwise, sh.waves_per_eu, resolve_coop(sh, pl), nblocks[si], nblocks64,
                                                     c.stream));
             } else {
-                ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, sh.coop, nblocks[si], c.stream));
+                ES_HIP(c, es::launch_trace(a, pl.clockwise, sh.waves_per_eu, resolve_coop(sh, pl), nblocks[si], c.stream));
             }
             if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 1], c.stream));
         }
@@ -686,6 +704,7 @@ es::SampleDev sample_dev(Field& f, int ppr, int ppq, double dist, double offx, d
     d.inv_L = 1.0 / d.L;
     d.inv_size = 1.0 / d.size;
     d.inv_speed = 1.0 / d.speed;
+    d.cell_population = (double)f.g.total / ((double)f.g.cpr * (double)f.g.cpr);
     return d;
 }
 
@@ -1218,9 +1237,9 @@ int es_batch_set_shape(es_batch* bt, const es_batch_shape* shape)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
     if (!shape || (shape->waves_per_eu != 2 && shape->waves_per_eu != 3) || shape->chunk < 1 || shape->chunk > 4096 ||
-        shape->grid_mult < 1 || shape->grid_mult > 8 || (shape->cooperative != 0 && shape->cooperative != 1))
+        shape->grid_mult < 1 || shape->grid_mult > 8 || shape->cooperative < -1 || shape->cooperative > 2)
         return fail(bt->c, ES_ERR_INVALID_ARG,
-                    "shape: waves_per_eu 2 or 3, chunk 1..4096, grid_mult 1..8, cooperative 0 or 1");
+                    "shape: waves_per_eu 2 or 3, chunk 1..4096, grid_mult 1..8, cooperative -1..2");
     bt->shape.waves_per_eu = shape->waves_per_eu;
     bt->shape.chunk = shape->chunk;
     bt->shape.grid_mult = shape->grid_mult;
@@ -1318,9 +1337,9 @@ int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
     bt->capture = false;  // the candidates are timed on direct launches
     Shape best_sh = bt->shape;
     double best_t = 1e300;
-    const int coops[] = {1, 0};
+    const int coops[] = {2, 1};
     const int waves[] = {2, 3};
-    const int chunks[] = {32, 64, 128, 256, 512};
+    const int chunks[] = {64, 128, 256, 512};
     const int mults[] = {1, 2};
     StageOut so;
     double t = 0;

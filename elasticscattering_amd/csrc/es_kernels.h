@@ -41,6 +41,7 @@ struct SampleDev {
     double phi_start, phi_step, alpha, speed;
     double phi_integrand_factor;     // Simulation.h:116-120
     double inv_L, inv_size, inv_speed;  // RN(1 / cell size), RN(1 / size), RN(1 / speed)
+    double cell_population;          // indexed impurities per cell (host: picks the trace variant)
     int coherent;
     int nT;
     double tau[kMaxT];
@@ -119,8 +120,10 @@ hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev
 hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
                             int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st);
 
-// Trace kernel variants: register budget 2 or 3 waves per SIMD; coop = 1 for
-// wave-cooperative crossing solves, 0 for one lane per particle throughout.
+// Trace kernel variants: register budget 2 or 3 waves per SIMD; coop = 0 for
+// one lane per particle throughout, 1 for wave-cooperative crossing solves, 2
+// for wave-cooperative and screened (disks that cannot lower the minimum are
+// not solved).  The FP32 trace has coop 0 and 1 (2 counts as 1).
 hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* blocks);
 // Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
 hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st);

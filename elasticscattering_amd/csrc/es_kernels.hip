@@ -553,23 +553,42 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 
 // ------------------------------------------------ wave-cooperative FP64 trace
 //
-// Same lane refill and per-lane cell walk as trace_body, but the crossing
-// disks of all 64 lanes' current cells go to a per-wave stack in LDS and are
-// solved 64 at a time, each lane taking any lane's disk: the owner's orbit and
-// angle range are read from LDS and the time is folded into the owner's
-// running minimum with an LDS atomic min on an order-preserving integer key.
-// Each solve runs the same operations on the same operands as in trace_body
-// and a minimum does not depend on the order, so lifetimes are bit-identical;
-// what changes is that a wave no longer waits, per group of impurities, for
-// its lane with the most crossing disks.
+// Same lane refill and per-lane cell walk as trace_body; what changes is who
+// solves which crossing disk, and which disks need solving at all.  Per cell
+// step of a wave:
+//   1. each lane runs CirclesCross (details.h:141-151) on its cell's disks,
+//      gives each crossing disk a pseudo-angle of its centre along the orbit
+//      (d_disk_proxy), keeps the earliest one and queues the others on the
+//      wave's stack in LDS;
+//   2. each lane solves its earliest disk (GetFirstCrossTime, details.h:153-206)
+//      and turns the time into a pseudo-angle threshold: a disk whose centre
+//      lies at or beyond it cannot cross the orbit before that time
+//      (d_prune_threshold);
+//   3. a pass over the stack drops those disks and compacts the rest;
+//   4. the rest are solved 64 at a time, any lane taking any lane's disk, with
+//      the owner's orbit and angle range read from LDS and the time folded
+//      into the owner's minimum by an LDS atomic min on an order-preserving
+//      integer key.
+// Without SCREEN, steps 2 and 3 are skipped and every crossing disk is solved
+// (sparse fields, where few disks cross in a cell, are faster so).
+// A nearly full stack goes through 2-4 at once and the tests resume.  Every
+// solve runs the same operations on the same operands as in trace_body, a
+// minimum does not depend on the order, and a dropped disk cannot lower it, so
+// lifetimes are bit-identical; the work counter of exact solves (cross_exact)
+// is what shrinks.
 
-constexpr int kCoopStack = 320;  // < 64 left over + up to 4 new disks per lane per group
+constexpr int kCoopStack = 320;  // FP32 trace: < 64 left over + up to 4 new disks per lane per group
+constexpr int kCoopCap = 640;    // FP64 trace: queued disks; screened and solved past kCoopCap - 256
 
 struct CoopWave {
-    double orb[12][64];              // per lane: cx cy R R2 r2 eps_r pa w inv_R inv_w rx ry
-    double qx[kCoopStack], qy[kCoopStack];  // queued crossing disks
+    double orb[10][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry
     unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
-    uint8_t qown[kCoopStack];        // lane owning each queued disk
+    const double2* ixy[64];          // per lane: its sample's binned field
+    float thr[64];                   // per lane: pseudo-angle threshold of step 2
+    unsigned int nexact[64];         // per lane: cooperative exact solves this cell
+    int32_t qidx[kCoopCap];          // queued disk: index in its owner's field
+    float qpa[kCoopCap];             // queued disk: pseudo-angle
+    uint8_t qown[kCoopCap];          // queued disk: owner lane
 };
 
 // Unsigned order of the keys = numeric order of the doubles (signs included).
@@ -592,31 +611,81 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// GetFirstCrossTime of queued disk k for its owner, folded into the owner's
-// minimum as `lifetime = (t < lifetime) ? t : lifetime` would (lifetime.h:117-123):
-// NaN and times >= DBL_MAX never lower it, and no solve yields -0.
-template <bool CW>
-__device__ __forceinline__ void d_coop_solve(CoopWave& L, int k)
+// Pseudo-angle in [0, 4) of the direction (x, y), increasing with the angle
+// from +x towards +y: 0 at +x, 1 at +y, 2 at -x, 3 at -y.  Its slope in the
+// angle lies in [1/2, 1].
+__device__ __forceinline__ float pseudo_angle(float x, float y)
 {
-    const int own = L.qown[k];
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float sum = ax + ay;
+    const float r = (sum > 0.0f) ? ay * __builtin_amdgcn_rcpf(sum) : 0.0f;
+    if (y >= 0.0f) return (x >= 0.0f) ? r : 2.0f - r;
+    return (x < 0.0f) ? 2.0f + r : 4.0f - r;
+}
+
+// Pseudo-angle of the direction of a disk centre seen from the orbit centre,
+// measured from the start in the direction of travel, from q = centre - disk
+// (the CirclesCross operands); -1 when the start lies within about
+// r + 1e-6 R of the disk (no bound is used for it: its crossing angles may sit
+// at 0 / 2 pi within the error of their FP64 computation).  u0 = start -
+// centre; gmin2 = (1.001 r + 1.001e-6 R)^2.
+template <bool CW>
+__device__ __forceinline__ float d_disk_proxy(double u0x, double u0y, float gmin2, double qx, double qy)
+{
+    const float gx = (float)(u0x + qx), gy = (float)(u0y + qy);  // start - disk
+    const float ux = (float)u0x, uy = (float)u0y;
+    const float vx = (float)(-qx), vy = (float)(-qy);             // disk - centre
+    const float dot = ux * vx + uy * vy;
+    const float crs = ux * vy - uy * vx;
+    const float pa = pseudo_angle(dot, CW ? -crs : crs);
+    return (gx * gx + gy * gy < gmin2) ? -1.0f : pa;
+}
+
+// Pseudo-angle at and beyond which a crossing disk's centre lies too far
+// along the orbit to cross it before time t, or 5 (nothing is dropped).
+// A disk of radius r whose centre is at distance d from the orbit centre
+// meets the orbit within asin(r / d) <= 1.0473 r / d (r / d <= 1/2) of the
+// centre's direction, and a crossing disk has d > R - r.  The 1e-5 rad margin
+// covers the FP32 pseudo-angles (< 5e-6 rad) and the FP64 crossing angles.
+__device__ __forceinline__ float d_prune_threshold(const Orb& o, double radius, double t)
+{
+    if (!(t < ES_INF)) return 5.0f;
+    const float q = (float)radius / (float)(o.R - radius);
+    if (!(q > 0.0f && q <= 0.5f)) return 5.0f;
+    const double theta = o.w * t + (double)(1.0474f * q) + 1e-5;
+    if (!(theta < ES_PI2 - 1e-4)) return 5.0f;
+    float sc, cc;
+    sincosf((float)theta, &sc, &cc);
+    return pseudo_angle(cc, sc) - 1e-6f;
+}
+
+// GetFirstCrossTime of disk gi of owner `own`'s field, folded into the
+// owner's minimum as `lifetime = (t < lifetime) ? t : lifetime` would
+// (lifetime.h:117-123): NaN and times >= DBL_MAX never lower it, and no solve
+// yields -0.
+template <bool CW>
+__device__ __forceinline__ void d_coop_solve(CoopWave& L, int own, int gi)
+{
     Orb ot;
     ot.cx = L.orb[0][own];
     ot.cy = L.orb[1][own];
     ot.R = L.orb[2][own];
-    ot.R2 = L.orb[3][own];
-    ot.r2 = L.orb[4][own];
-    ot.eps_r = L.orb[5][own];
-    ot.pa = L.orb[6][own];
-    ot.w = L.orb[7][own];
-    ot.inv_R = L.orb[8][own];
-    ot.inv_w = L.orb[9][own];
+    ot.R2 = ot.R * ot.R;  // as d_setup_item forms them
+    ot.eps_r = 1e-10 * ot.R;
+    ot.r2 = L.orb[3][own];
+    ot.pa = L.orb[4][own];
+    ot.w = L.orb[5][own];
+    ot.inv_R = L.orb[6][own];
+    ot.inv_w = L.orb[7][own];
     ot.radd2 = ot.rmin2 = ot.u0x = ot.u0y = 0.0;
     ot.r = 0.0f;
-    const double t = d_first_cross_time<CW>(ot, L.qx[k], L.qy[k], L.orb[10][own], L.orb[11][own]);
+    const double2 p = L.ixy[own][gi];
+    const double t = d_first_cross_time<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own]);
     if (t < ES_INF) atomicMin(&L.tmin[own], d_order_key(t));
+    atomicAdd(&L.nexact[own], 1u);
 }
 
-template <bool CW>
+template <bool CW, bool SCREEN>
 __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
 {
     __shared__ CoopWave sh_coop[kBlock / 64];
@@ -626,6 +695,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
     const long long total = a.list_mode ? (long long)*a.list_count : a.total_items;
     const unsigned long long key_inf = d_order_key(ES_INF);
     L.tmin[lane] = key_inf;
+    L.nexact[lane] = 0u;
 
     long long q_next = 0, q_end = 0;
     bool exhausted = false;
@@ -635,6 +705,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
     DevGrid g;
     Orb o;
     double phi0 = 0.0, bound_time = 0.0;
+    float gmin2 = 0.0f;
     TraceState st;
 
     while (true) {
@@ -646,13 +717,14 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
             L.orb[0][lane] = o.cx;
             L.orb[1][lane] = o.cy;
             L.orb[2][lane] = o.R;
-            L.orb[3][lane] = o.R2;
-            L.orb[4][lane] = o.r2;
-            L.orb[5][lane] = o.eps_r;
-            L.orb[6][lane] = o.pa;
-            L.orb[7][lane] = o.w;
-            L.orb[8][lane] = o.inv_R;
-            L.orb[9][lane] = o.inv_w;
+            L.orb[3][lane] = o.r2;
+            L.orb[4][lane] = o.pa;
+            L.orb[5][lane] = o.w;
+            L.orb[6][lane] = o.inv_R;
+            L.orb[7][lane] = o.inv_w;
+            L.ixy[lane] = g.ixy;
+            const float gm = 1.001f * (float)g.radius + 1.001e-6f * (float)o.R;
+            gmin2 = gm * gm;
             active = true;
         }
         if (__ballot(active) == 0ull) break;
@@ -663,60 +735,113 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         if (active) {
             const Isect entry = st.nx;
             avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
-            L.orb[10][lane] = entry.ang;
-            L.orb[11][lane] = avail ? st.nx.ang : phi0;
+            L.orb[8][lane] = entry.ang;
+            L.orb[9][lane] = avail ? st.nx.ang : phi0;
             const int ci = entry.cy * g.cpr + entry.cx;
             s = (ci > 0) ? g.cell_end[ci - 1] : 0;
             n = g.cell_end[ci] - s;
             st.cells += 1;
             st.tested += n;
         }
-        wave_lds_sync();
 
-        // CirclesCross (details.h:141-151) per lane, four impurities at a time;
-        // crossing disks go on the wave's stack, which is solved 64 at a time.
-        int top = 0;  // wave-uniform
-        for (int i = 0;; i += 4) {
-            const bool more = __ballot(i < n) != 0ull;
-            if (more) {
+        int i = 0;
+        while (true) {
+            // 1. CirclesCross, four disks at a time; the earliest crossing disk
+            // stays with its lane, the others are queued.
+            int top = 0;  // wave-uniform
+            float first_pa = 3.0e38f;
+            int first_i = -1;
+            bool more;
+            while ((more = (__ballot(i < n) != 0ull)) && top <= kCoopCap - 256) {
                 double2 p[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
                 int hits = 0;
+                float pa[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
                     const double d2 = qx * qx + qy * qy;
-                    hits |= ((i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2)) ? (1 << u) : 0;
+                    const bool h = (i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2);
+                    hits |= h ? (1 << u) : 0;
+                    pa[u] = SCREEN ? d_disk_proxy<CW>(o.u0x, o.u0y, gmin2, qx, qy) : 0.0f;
                 }
                 st.cross_calls += __popc(hits);
-                st.cross_exact += __popc(hits);
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const bool h = (hits >> u) & 1;
+                    bool h = (hits >> u) & 1;
+                    int qi = s + i + u;
+                    float qp = pa[u];
+                    if (SCREEN && h && pa[u] < first_pa) {  // a new earliest: the previous one is queued
+                        qi = first_i;
+                        qp = first_pa;
+                        h = first_i >= 0;
+                        first_pa = pa[u];
+                        first_i = s + i + u;
+                    }
                     const unsigned long long m = __ballot(h);
                     if (h) {
                         const int pos = top + __popcll(m & below);
-                        L.qx[pos] = p[u].x;
-                        L.qy[pos] = p[u].y;
+                        L.qidx[pos] = qi;
+                        if (SCREEN) L.qpa[pos] = qp;
                         L.qown[pos] = (uint8_t)lane;
                     }
                     top += __popcll(m);
                 }
+                i += 4;
+            }
+
+            // 2. Each lane solves its earliest disk, unless the minimum so far
+            // already rules it out; the threshold that results.
+            if (SCREEN && active) {
+                float thr = d_prune_threshold(o, g.radius, d_from_order_key(L.tmin[lane]));
+                if (first_i >= 0 && !(first_pa >= thr)) {
+                    const double2 p = g.ixy[first_i];
+                    const double t = d_first_cross_time<CW>(o, p.x, p.y, L.orb[8][lane], L.orb[9][lane]);
+                    if (t < ES_INF) {
+                        const unsigned long long kt = d_order_key(t);
+                        if (kt < L.tmin[lane]) L.tmin[lane] = kt;
+                    }
+                    st.cross_exact += 1;
+                    thr = d_prune_threshold(o, g.radius, d_from_order_key(L.tmin[lane]));
+                }
+                L.thr[lane] = thr;
+            }
+            wave_lds_sync();
+
+            // 3. Drop the queued disks at or past their owner's threshold; compact.
+            int nsurv = SCREEN ? 0 : top;  // wave-uniform
+            for (int k0 = 0; SCREEN && k0 < top; k0 += 64) {
+                const int k = k0 + lane;
+                bool keep = false;
+                int own = 0, gi = 0;
+                if (k < top) {
+                    own = L.qown[k];
+                    gi = L.qidx[k];
+                    keep = !(L.qpa[k] >= L.thr[own]);
+                }
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int pos = nsurv + __popcll(m & below);
+                    L.qidx[pos] = gi;
+                    L.qown[pos] = (uint8_t)own;
+                }
+                nsurv += __popcll(m);
                 wave_lds_sync();
             }
-            while (top >= 64 || (!more && top > 0)) {
-                const int m = (top < 64) ? top : 64;
-                if (lane < m) d_coop_solve<CW>(L, top - m + lane);
-                top -= m;
-                wave_lds_sync();
-            }
+
+            // 4. The rest, 64 at a time.
+            for (int k0 = 0; k0 < nsurv; k0 += 64)
+                if (k0 + lane < nsurv) d_coop_solve<CW>(L, L.qown[k0 + lane], L.qidx[k0 + lane]);
+            wave_lds_sync();
             if (!more) break;
         }
 
         if (active) {
             st.lifetime = d_from_order_key(L.tmin[lane]);
+            st.cross_exact += (int)L.nexact[lane];
             L.tmin[lane] = key_inf;
+            L.nexact[lane] = 0u;
             int flags = -1;
             if (st.lifetime < ES_INF) {
                 flags = 0;
@@ -1102,16 +1227,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     trace_body<CW>(a);
 }
 
-template <bool CW>
+template <bool CW, bool SCREEN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_c3(TraceArgs a)
 {
-    trace_coop_body<CW>(a);
+    trace_coop_body<CW, SCREEN>(a);
 }
 
-template <bool CW>
+template <bool CW, bool SCREEN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_c2(TraceArgs a)
 {
-    trace_coop_body<CW>(a);
+    trace_coop_body<CW, SCREEN>(a);
 }
 
 template <bool CW>
@@ -1454,13 +1579,19 @@ hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx,
 }
 
 // The trace kernel of a launch shape: rotation sense x register budget x
-// per-lane or wave-cooperative crossing solves.
+// crossing solves per lane (coop 0), wave-cooperative (1) or wave-cooperative
+// and screened (2).
 static const void* trace_kernel(int clockwise, int waves_per_eu, int coop)
 {
+    if (coop == 2) {
+        if (waves_per_eu == 2)
+            return clockwise ? (const void*)k_trace_c2<true, true> : (const void*)k_trace_c2<false, true>;
+        return clockwise ? (const void*)k_trace_c3<true, true> : (const void*)k_trace_c3<false, true>;
+    }
     if (coop) {
         if (waves_per_eu == 2)
-            return clockwise ? (const void*)k_trace_c2<true> : (const void*)k_trace_c2<false>;
-        return clockwise ? (const void*)k_trace_c3<true> : (const void*)k_trace_c3<false>;
+            return clockwise ? (const void*)k_trace_c2<true, false> : (const void*)k_trace_c2<false, false>;
+        return clockwise ? (const void*)k_trace_c3<true, false> : (const void*)k_trace_c3<false, false>;
     }
     if (waves_per_eu == 2) return clockwise ? (const void*)k_trace_w2<true> : (const void*)k_trace_w2<false>;
     return clockwise ? (const void*)k_trace_w3<true> : (const void*)k_trace_w3<false>;
@@ -1495,7 +1626,7 @@ hipError_t launch_images(const SampleDev* sample, const PairDev* pair, int clock
 
 static const void* trace32_kernel(int clockwise, int coop)
 {
-    if (coop) return clockwise ? (const void*)k_trace32_c<true> : (const void*)k_trace32_c<false>;
+    if (coop != 0) return clockwise ? (const void*)k_trace32_c<true> : (const void*)k_trace32_c<false>;
     return clockwise ? (const void*)k_trace32<true> : (const void*)k_trace32<false>;
 }
 

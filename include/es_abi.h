@@ -204,7 +204,9 @@ typedef struct es_batch_shape {
     int32_t waves_per_eu;             /* register budget: 2 or 3 waves per SIMD */
     int32_t chunk;                    /* work items a wave takes from the queue at a time */
     int32_t grid_mult;                /* resident blocks per CU times this = grid size */
-    int32_t cooperative;              /* 1: a wave solves its lanes' crossing disks together (default) */
+    int32_t cooperative;              /* crossing solves: 0 per lane, 1 shared by the wave, 2 shared and
+                                         screened (disks that cannot lower the minimum are skipped),
+                                         -1 (default) 2 or 1 by the fields' impurities per cell */
 } es_batch_shape;
 
 int es_batch_create(int32_t device, es_batch** out);

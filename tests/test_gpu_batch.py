@@ -21,7 +21,10 @@ pytestmark = pytest.mark.gpu
 METRIC_KEYS = ("cells_passed", "impurities_tested", "particles_inside_impurity", "particles_escaped",
                "particles_at_bound", "max_impurities_tested", "max_cells_passed", "trace_cap_hits",
                "real_particles", "avg_particle_lifetime", "traced_particles", "edge_hits", "cross_calls",
-               "sigma_evals", "cross_exact")
+               "sigma_evals")
+# cross_exact counts the crossing solves a kernel actually ran: the screened
+# cooperative trace skips disks that cannot lower the minimum, so it depends on
+# the launch shape and is only bounded here.
 
 RS, EXT = 1e-6, 5e-7
 
@@ -83,6 +86,8 @@ def _same_results(r0, r1):
             for fa, fb in zip(ma, mb):
                 for k in METRIC_KEYS:
                     assert fa[k] == fb[k], (e, k, fa[k], fb[k])
+                for f in (fa, fb):
+                    assert 0 <= f["cross_exact"] <= f["cross_calls"], f
 
 
 @pytest.fixture(scope="module")
@@ -198,15 +203,16 @@ def test_tuner_shapes_give_same_results(es, batch_eager):
     entries, b, eager = batch_eager
     announce("tuner over launch shapes")
     best, t = b.tune()
-    assert best["waves_per_eu"] in (2, 3) and best["chunk"] in (32, 64, 128, 256, 512)
-    assert best["grid_mult"] in (1, 2) and best["cooperative"] in (0, 1) and t > 0
+    assert best["waves_per_eu"] in (2, 3) and best["chunk"] in (64, 128, 256, 512)
+    assert best["grid_mult"] in (1, 2) and best["cooperative"] in (1, 2) and t > 0
     assert b.shape == best
     _same_results(eager, b.run())
     # one lane per particle and the wave-cooperative crossing solves: same bits
-    for shape in ((2, 32, 2, 1), (3, 512, 1, 1), (2, 32, 2, 0), (3, 512, 1, 0), (3, 64, 1, 0)):
+    for shape in ((2, 32, 2, 1), (3, 512, 1, 1), (2, 64, 1, 2), (3, 32, 2, 2), (2, 32, 2, 0), (3, 512, 1, 0),
+                  (3, 64, 1, 0)):
         b.set_shape(*shape)
         _same_results(eager, b.run())
-    b.set_shape(3, 128, 1)
+    b.set_shape(3, 128, 1, -1)
 
 
 def test_batch_errors(es):
