@@ -399,13 +399,26 @@ __device__ __forceinline__ int pair_of_block(const PairDev* pairs, int npairs, i
     return lo;
 }
 
+// The pair of a chunk's first item, found once per chunk (wave-uniform), and
+// the item range it covers: items of the chunk inside it skip their own search.
+struct PairHint {
+    int pair = 0;
+    long long begin = 0, end = -1;
+};
+
+__device__ __forceinline__ int hinted_pair(const TraceArgs& a, const PairHint& h, long long item)
+{
+    return (item >= h.begin && item < h.end) ? h.pair : pair_of_item(a.pairs, a.npairs, item);
+}
+
 // CreateParticle (lifetime.h:36-58) and the trace start for work item `item`:
 // the lane's grid view, orbit, start angle, bound time and output slot.
 template <bool CW>
-__device__ __forceinline__ void d_setup_item(const TraceArgs& a, long long item, DevGrid& g, Orb& o, double& phi0,
-                                             double& bound_time, long long& out_idx, int& cap, TraceState& st)
+__device__ __forceinline__ void d_setup_item(const TraceArgs& a, const PairHint& hint, long long item, DevGrid& g,
+                                             Orb& o, double& phi0, double& bound_time, long long& out_idx, int& cap,
+                                             TraceState& st)
 {
-    const int pi = pair_of_item(a.pairs, a.npairs, item);
+    const int pi = hinted_pair(a, hint, item);
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
     const int ppq = sd.ppq, ppp = 4 * sd.ppq;
@@ -481,11 +494,11 @@ __device__ __forceinline__ void d_store_item(const TraceArgs& a, long long item,
 }
 
 // Lane refill: every idle lane takes the next item of the wave's chunk of the
-// work queue (a fresh chunk of a.chunk items when the current one is used up).
-// Returns the item to a lane that got one, -1 to the others.  Wave-uniform
-// state: q_next, q_end, exhausted.
+// work queue (a fresh chunk of a.chunk items when the current one is used up;
+// one atomic per chunk on the queue counter).  Returns the item to a lane that
+// got one, -1 to the others.  Wave-uniform state: q_next, q_end, exhausted.
 __device__ __forceinline__ void d_refill(const TraceArgs& a, long long total, bool active, long long& q_next,
-                                         long long& q_end, bool& exhausted, long long& got)
+                                         long long& q_end, bool& exhausted, long long& got, PairHint& hint)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -495,13 +508,19 @@ __device__ __forceinline__ void d_refill(const TraceArgs& a, long long total, bo
         if (q_next >= q_end) {
             unsigned long long base = 0;
             if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-            base = __shfl(base, 0, 64);
+            base = ((unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                   (unsigned long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)(unsigned int)base);
             if ((long long)base >= total) {
                 exhausted = true;
                 break;
             }
             q_next = (long long)base;
             q_end = (q_next + a.chunk < total) ? q_next + a.chunk : total;
+            if (!a.list_mode) {
+                hint.pair = pair_of_item(a.pairs, a.npairs, q_next);
+                hint.begin = a.pairs[hint.pair].item_off;
+                hint.end = hint.begin + a.pairs[hint.pair].nitems;
+            }
         }
         const long long left = q_end - q_next;
         const int n = __popcll(need);
@@ -522,6 +541,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
     const long long total = a.list_mode ? (long long)*a.list_count : a.total_items;
 
     long long q_next = 0, q_end = 0;  // this wave's current chunk of the queue (uniform)
+    PairHint hint;
     bool exhausted = false;
     bool active = false;
     long long item = 0, out_idx = -1;
@@ -533,10 +553,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 
     while (true) {
         long long got;
-        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        d_refill(a, total, active, q_next, q_end, exhausted, got, hint);
         if (got >= 0) {
             item = got;
-            d_setup_item<CW>(a, item, g, o, phi0, bound_time, out_idx, cap, st);
+            d_setup_item<CW>(a, hint, item, g, o, phi0, bound_time, out_idx, cap, st);
             active = true;
         }
         if (__ballot(active) == 0ull) break;  // queue drained and every lane done
@@ -698,6 +718,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
     L.nexact[lane] = 0u;
 
     long long q_next = 0, q_end = 0;
+    PairHint hint;
     bool exhausted = false;
     bool active = false;
     long long item = 0, out_idx = -1;
@@ -710,10 +731,10 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
 
     while (true) {
         long long got;
-        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        d_refill(a, total, active, q_next, q_end, exhausted, got, hint);
         if (got >= 0) {
             item = got;
-            d_setup_item<CW>(a, item, g, o, phi0, bound_time, out_idx, cap, st);
+            d_setup_item<CW>(a, hint, item, g, o, phi0, bound_time, out_idx, cap, st);
             L.orb[0][lane] = o.cx;
             L.orb[1][lane] = o.cy;
             L.orb[2][lane] = o.R;
@@ -863,10 +884,11 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
 // CreateParticle for the FP32 trace: the same start data (position, phi,
 // orbit centre, bound time in FP64) in the start point's frame.
 template <bool CW>
-__device__ __forceinline__ void t32_setup_item(const TraceArgs& a, long long item, DevGrid& g, Orb32& o, double& w,
-                                               double& bound_time, long long& out_idx, int& cap, Trace32& st)
+__device__ __forceinline__ void t32_setup_item(const TraceArgs& a, const PairHint& hint, long long item, DevGrid& g,
+                                               Orb32& o, double& w, double& bound_time, long long& out_idx, int& cap,
+                                               Trace32& st)
 {
-    const int pi = pair_of_item(a.pairs, a.npairs, item);
+    const int pi = hinted_pair(a, hint, item);
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
     const int ppq = sd.ppq, ppp = 4 * sd.ppq;
@@ -966,6 +988,7 @@ __device__ __forceinline__ void trace32_body(const TraceArgs& a)
 {
     const long long total = a.total_items;
     long long q_next = 0, q_end = 0;
+    PairHint hint;
     bool exhausted = false;
     bool active = false;
     long long item = 0, out_idx = -1;
@@ -977,10 +1000,10 @@ __device__ __forceinline__ void trace32_body(const TraceArgs& a)
 
     while (true) {
         long long got;
-        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        d_refill(a, total, active, q_next, q_end, exhausted, got, hint);
         if (got >= 0) {
             item = got;
-            t32_setup_item<CW>(a, item, g, o, w, bound_time, out_idx, cap, st);
+            t32_setup_item<CW>(a, hint, item, g, o, w, bound_time, out_idx, cap, st);
             active = true;
         }
         if (__ballot(active) == 0ull) break;
@@ -1084,6 +1107,7 @@ __device__ __forceinline__ void trace32_coop_body(const TraceArgs& a)
     L.cross[lane] = 0u;
 
     long long q_next = 0, q_end = 0;
+    PairHint hint;
     bool exhausted = false;
     bool active = false;
     long long item = 0, out_idx = -1;
@@ -1095,10 +1119,10 @@ __device__ __forceinline__ void trace32_coop_body(const TraceArgs& a)
 
     while (true) {
         long long got;
-        d_refill(a, total, active, q_next, q_end, exhausted, got);
+        d_refill(a, total, active, q_next, q_end, exhausted, got, hint);
         if (got >= 0) {
             item = got;
-            t32_setup_item<CW>(a, item, g, o, w, bound_time, out_idx, cap, st);
+            t32_setup_item<CW>(a, hint, item, g, o, w, bound_time, out_idx, cap, st);
             L.ixy[lane] = g.ixy;
             active = true;
         }

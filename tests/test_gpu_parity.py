@@ -142,6 +142,25 @@ def test_sparse_long_traces_bit_exact(es, oracle_esm):
               [2.0, 40.0, 100.0])
 
 
+@pytest.mark.parametrize("cw", [1, 0])
+def test_dense_field_screened_trace_bit_exact(es, oracle_esm, cw):
+    """Dense field (about 20 indexed disks per cell): the solver runs the screened
+    cooperative trace, which skips crossing disks that cannot lower the minimum;
+    lifetimes stay bit-exact and fewer exact solves than crossing disks run."""
+    rs, ext = 1e-6, 5e-7
+    n = 40000
+    x = np.empty(n)
+    y = np.empty(n)
+    oracle_esm.generate_impurities(23, n, -ext, rs + ext, O.dptr(x), O.dptr(y))
+    _run_pair(es, oracle_esm, x, y, rs, ext, 8e-9, 15, 64, 24, 9, 1e-12, 0.3, cw, 7e5, 0, [0.5, 6.0, 40.0],
+              [2.0, 40.0])
+    with es.Simulation(24, 9) as sim:
+        sim.set_impurities(x, y, rs, ext, 8e-9, 15, 64)
+        sim.init_sample(1e-12, 0.3, cw, 7e5, 0)
+        _, m, _ = sim.compute_sigmas(6.0, [2.0])
+    assert 0 < m["cross_exact"] < m["cross_calls"], m
+
+
 def test_sweep_equals_single_calls(es):
     w, rs, ext = _small_case()
     fields = [1.0, 7.5, 20.0, 40.0]
