@@ -52,7 +52,8 @@ SIGMA_WORK = {
 
 
 def algorithmic_flops(m, model):
-    return sum(model[k] * m[k] for k in model)
+    """Work of one metrics record, or of a record array (summed)."""
+    return float(sum(model[k] * m[k].sum() if hasattr(m[k], "sum") else model[k] * m[k] for k in model))
 
 
 def cpu_baseline(w, seconds_target=15.0):
@@ -157,26 +158,26 @@ def main():
             sim.init_sample(w["tau"], w["alpha"], w["clockwise"], w["particle_speed"], coherent, w["incoherent_tau"])
             sig, ms = sim.compute_sweep(fields, temps)
             if stats is not None and coherent == 0:
-                stats["launch_s"].append(sum(m["time_elapsed_lifetimes"] for m in ms))
-                stats["flops"].append(sum(algorithmic_flops(m, TRACE_WORK) for m in ms))
-                stats["ref_flops"].append(sum(algorithmic_flops(m, REFERENCE_TRACE_WORK) for m in ms))
-                stats["sigma_flops"].append(sum(algorithmic_flops(m, SIGMA_WORK) for m in ms))
-                stats["sigma_s"].append(sum(m["time_elapsed_temperatures"] for m in ms))
-                stats["traced"] += sum(m["traced_particles"] for m in ms)
-                stats["cross"] += sum(m["cross_calls"] for m in ms)
-                stats["exact"] += sum(m["cross_exact"] for m in ms)
-                stats["recomputed"] += sum(m["fp64_recomputed"] for m in ms)
+                stats["ms"].append(ms)  # reduced after the timed loop
                 stats["sigma"].append(sig)
 
     def timed(sim_):
         for s in range(args.warmup):
             step(s, None, sim_)
-        st = {"launch_s": [], "flops": [], "ref_flops": [], "sigma_flops": [], "sigma_s": [], "traced": 0,
-              "sigma": [], "cross": 0, "exact": 0, "recomputed": 0}
+        st = {"ms": [], "sigma": []}
         t0 = time.perf_counter()
         for s in range(args.steps):
             step(s, st, sim_)
-        return st, time.perf_counter() - t0
+        dt_ = time.perf_counter() - t0
+        st["launch_s"] = [float(ms["time_elapsed_lifetimes"].sum()) for ms in st["ms"]]
+        st["flops"] = [algorithmic_flops(ms, TRACE_WORK) for ms in st["ms"]]
+        st["ref_flops"] = [algorithmic_flops(ms, REFERENCE_TRACE_WORK) for ms in st["ms"]]
+        st["sigma_flops"] = [algorithmic_flops(ms, SIGMA_WORK) for ms in st["ms"]]
+        st["sigma_s"] = [float(ms["time_elapsed_temperatures"].sum()) for ms in st["ms"]]
+        for key, col in (("traced", "traced_particles"), ("cross", "cross_calls"), ("exact", "cross_exact"),
+                         ("recomputed", "fp64_recomputed")):
+            st[key] = int(sum(int(ms[col].sum()) for ms in st["ms"]))
+        return st, dt_
 
     stats, dt = timed(sim)
     mixed = None

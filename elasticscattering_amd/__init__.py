@@ -111,6 +111,9 @@ class Metrics(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+_METRICS_DTYPE = np.ctypeslib.as_array((Metrics * 1)()).dtype
+
+
 class SolverDesc(C.Structure):
     _fields_ = [
         ("positions_per_row", C.c_int32),
@@ -345,15 +348,17 @@ class Simulation:
         return [(out[i].xx, out[i].xy) for i in range(len(T))], m.as_dict(), img
 
     def compute_sweep(self, fields, temperatures):
-        """All fields in one launch: (sigma [nB, nT, 2], list of metrics dicts)."""
+        """All fields in one launch: (sigma [nB, nT, 2], metrics per field).
+
+        The metrics are a numpy record array with the es_metrics fields
+        (``ms[b]["cells_passed"]``, or a column ``ms["cells_passed"]``)."""
         Bs = np.ascontiguousarray(fields, dtype=np.float64)
         T = np.ascontiguousarray(temperatures, dtype=np.float64)
         out = (Sigma * max(len(Bs) * len(T), 1))()
         ms = (Metrics * len(Bs))()
         self._check(self.lib.es_solver_compute_sigmas_sweep(self.h, _dptr(Bs), len(Bs), _dptr(T), len(T), out, ms))
-        sig = np.array([[out[b * len(T) + t].xx, out[b * len(T) + t].xy] for b in range(len(Bs))
-                        for t in range(len(T))], dtype=np.float64).reshape(len(Bs), len(T), 2)
-        return sig, [ms[b].as_dict() for b in range(len(Bs))]
+        sig = np.frombuffer(out, dtype=np.float64, count=2 * len(Bs) * len(T)).reshape(len(Bs), len(T), 2).copy()
+        return sig, np.frombuffer(ms, dtype=_METRICS_DTYPE).copy()
 
     def close(self):
         if getattr(self, "h", None):

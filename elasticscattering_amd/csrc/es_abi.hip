@@ -59,6 +59,8 @@ struct Ctx {
     int trace32_bpc[2][2] = {{0, 0}, {0, 0}};  // resident FP32 trace blocks per CU [clockwise][coop]
     long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
+    void* hbuf = nullptr;             // pinned host staging of the stage tables and results
+    size_t hbuf_cap = 0;
 };
 
 
@@ -114,6 +116,25 @@ int ensure(Ctx& c, DevBuf& b, size_t bytes, const char* what)
     return ES_OK;
 }
 
+// Grows the pinned staging buffer (only between synchronised points: copies
+// from or into it must not be in flight).
+int ensure_host(Ctx& c, size_t bytes)
+{
+    if (c.hbuf_cap >= bytes) return ES_OK;
+    if (c.hbuf) (void)hipHostFree(c.hbuf);
+    c.hbuf = nullptr;
+    c.hbuf_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+    hipError_t e = hipHostMalloc(&c.hbuf, cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c.hbuf = nullptr;
+        return fail(c, ES_ERR_OOM, std::string("hipHostMalloc failed for staging: ") + hipGetErrorString(e));
+    }
+    c.hbuf_cap = cap;
+    return ES_OK;
+}
+
 void release(Ctx& c, DevBuf& b)
 {
     if (b.p) (void)hipFree(b.p);
@@ -162,6 +183,9 @@ void ctx_destroy(Ctx& c)
     for (hipEvent_t ev : c.ev)
         if (ev) (void)hipEventDestroy(ev);
     c.ev.clear();
+    if (c.hbuf) (void)hipHostFree(c.hbuf);
+    c.hbuf = nullptr;
+    c.hbuf_cap = 0;
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -190,6 +214,11 @@ struct Field {
     double pos_dist = 0, pos_offx = 0, pos_offy = 0;
     int nfree = 0;
     int64_t n_inside = 0;
+    // Read-backs in flight, collected by field_sync: [0] indexed count,
+    // [1] out-of-grid flag and free-position count (pinned host memory).
+    int64_t* hsmall = nullptr;
+    bool total_pending = false;
+    bool pos_pending = false;
 };
 
 void field_release(Ctx& c, Field& f)
@@ -197,6 +226,50 @@ void field_release(Ctx& c, Field& f)
     DevBuf* bufs[] = {&f.in_x, &f.in_y, &f.ixy, &f.cell_end, &f.b_targets, &f.b_ntarg, &f.b_counts, &f.b_fill,
                       &f.b_begin, &f.b_bsum, &f.b_order, &f.flags, &f.bcount, &f.free_pos, &f.small};
     for (DevBuf* b : bufs) release(c, *b);
+    if (f.hsmall) (void)hipHostFree(f.hsmall);
+    f.hsmall = nullptr;
+}
+
+int field_host_small(Ctx& c, Field& f)
+{
+    if (f.hsmall) return ES_OK;
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, ES_ERR_OOM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    }
+    f.hsmall = static_cast<int64_t*>(p);
+    return ES_OK;
+}
+
+// Waits for the stream and takes the field's read-backs in flight: the
+// indexed impurity count of a binning and the start test of the positions.
+// With pos_errors = false a start position outside the grid only leaves the
+// positions invalid (the next field_positions reports it).
+int field_sync(Ctx& c, Field& f, bool pos_errors = true)
+{
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    if (f.total_pending) {
+        f.total_pending = false;
+        const int64_t total = f.hsmall[0];
+        if (total > INT32_MAX) return fail(c, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
+        f.g.total = total;
+    }
+    if (f.pos_pending) {
+        f.pos_pending = false;
+        const int32_t* h = reinterpret_cast<const int32_t*>(f.hsmall + 1);
+        if (h[0] != 0 && !pos_errors) return ES_OK;
+        if (h[0] != 0)
+            return fail(c, ES_ERR_OUT_OF_GRID,
+                        "a start position lies outside the impurity cell grid (region_extends smaller than the "
+                        "position offset); the reference would read out of bounds (lifetime.h:88-91)");
+        const int64_t n = (int64_t)f.pos_ppr * f.pos_ppr;
+        f.nfree = h[1];
+        f.n_inside = n - h[1];
+        f.pos_valid = true;
+    }
+    return ES_OK;
 }
 
 // Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180).
@@ -231,25 +304,28 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* 
     const double* y = P<double>(f.in_y);
     ES_HIP(c, es::launch_bin_count(x, y, n, g, bs, c.stream));
     ES_HIP(c, es::launch_bin_scan(g, bs, c.stream));
-    int64_t total = 0;
-    ES_HIP(c, hipMemcpyAsync(&total, P<int64_t>(f.b_begin) + ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
-    ES_HIP(c, hipStreamSynchronize(c.stream));
-    if (total > INT32_MAX) return fail(c, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
-    if ((rc = ensure(c, f.b_order, total * sizeof(int32_t), "cell order"))) return rc;
-    if ((rc = ensure(c, f.ixy, total * sizeof(double2), "impurities"))) return rc;
+    // Every impurity lands in at most 9 cells (Grid.cpp:112-180), so the
+    // binned arrays are sized for that and the count is read back later.
+    const int64_t max_total = 9 * n;
+    if ((rc = ensure(c, f.b_order, max_total * sizeof(int32_t), "cell order"))) return rc;
+    if ((rc = ensure(c, f.ixy, max_total * sizeof(double2), "impurities"))) return rc;
+    if ((rc = field_host_small(c, f))) return rc;
     bs.order = P<int32_t>(f.b_order);
     g.ixy = P<double2>(f.ixy);
-    g.total = total;
-    ES_HIP(c, es::launch_bin_place(x, y, n, g, bs, c.stream));
-    ES_HIP(c, hipStreamSynchronize(c.stream));
+    const int64_t* total_dev = P<int64_t>(f.b_begin) + ncell;
+    ES_HIP(c, es::launch_bin_place(x, y, n, g, bs, total_dev, max_total, c.stream));
+    ES_HIP(c, hipMemcpyAsync(f.hsmall, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    f.total_pending = true;
     f.unique = n;
     f.has_grid = true;
     f.pos_valid = false;
+    f.pos_pending = false;
     if (out) {
+        if ((rc = field_sync(c, f))) return rc;
         out->region_size = d->region_size;
         out->region_extends = d->region_extends;
         out->cell_size = g.cell_size;
-        out->indexed_impurity_count = total;
+        out->indexed_impurity_count = g.total;
         out->unique_impurity_count = n;
         out->index_size = (int32_t)ncell;
         out->cells_per_row = cpr;
@@ -292,35 +368,39 @@ int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid
     return field_bin(c, f, n, d, out);
 }
 
-// Start cell + inside test for every position of a ppr x ppr grid.
-int field_positions(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
+// Launches the start cell + inside test for every position of a ppr x ppr
+// grid; field_sync collects the result.
+int field_positions_launch(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
 {
-    if (f.pos_valid && f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy)
-        return ES_OK;
     const int64_t n = (int64_t)ppr * ppr;
     int rc;
     if ((rc = ensure(c, f.flags, n, "position flags"))) return rc;
     if ((rc = ensure(c, f.free_pos, n * sizeof(int32_t), "free positions"))) return rc;
     if ((rc = ensure(c, f.bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
     if ((rc = ensure(c, f.small, 2 * sizeof(int32_t), "flags"))) return rc;
+    if ((rc = field_host_small(c, f))) return rc;
     int32_t* small = P<int32_t>(f.small);
     ES_HIP(c, es::launch_positions(f.g, ppr, dist, offx, offy, P<uint8_t>(f.flags), P<int32_t>(f.bcount), small,
                                    P<int32_t>(f.free_pos), small + 1, c.stream));
-    int32_t host[2] = {0, 0};
-    ES_HIP(c, hipMemcpyAsync(host, small, sizeof(host), hipMemcpyDeviceToHost, c.stream));
-    ES_HIP(c, hipStreamSynchronize(c.stream));
-    if (host[0] != 0)
-        return fail(c, ES_ERR_OUT_OF_GRID,
-                    "a start position lies outside the impurity cell grid (region_extends smaller than the "
-                    "position offset); the reference would read out of bounds (lifetime.h:88-91)");
-    f.nfree = host[1];
-    f.n_inside = n - host[1];
+    ES_HIP(c, hipMemcpyAsync(f.hsmall + 1, small, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    f.pos_valid = false;
+    f.pos_pending = true;
     f.pos_ppr = ppr;
     f.pos_dist = dist;
     f.pos_offx = offx;
     f.pos_offy = offy;
-    f.pos_valid = true;
     return ES_OK;
+}
+
+// Start cell + inside test for every position of a ppr x ppr grid (cached).
+int field_positions(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
+{
+    if (f.pos_valid && !f.pos_pending && f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx &&
+        f.pos_offy == offy && !f.total_pending)
+        return ES_OK;
+    int rc;
+    if ((rc = field_positions_launch(c, f, ppr, dist, offx, offy))) return rc;
+    return field_sync(c, f);
 }
 
 // ----------------------------------------------------------------- stages
@@ -482,6 +562,15 @@ void plan_offsets(Plan& pl, long long seg_items)
     pl.NT = maxT <= 1 ? 1 : maxT <= 2 ? 2 : maxT <= 4 ? 4 : 8;
 }
 
+size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of pinned staging the results of a plan take: sigma, lifetime sums, metrics.
+size_t collect_bytes(const Plan& pl)
+{
+    return align_up((size_t)pl.nout * es::kMaxT * 2 * sizeof(double)) + align_up((size_t)pl.nout * sizeof(double)) +
+           (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t);
+}
+
 int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
 {
     const int NQ = 2 * pl.NT + 1;
@@ -507,11 +596,19 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
         if ((rc = ensure(c, s.list_count, sizeof(int32_t), "recompute count"))) return rc;
     }
     if ((rc = ctx_events(c, 4 * pl.segs.size()))) return rc;
-    ES_HIP(c, hipMemcpyAsync(s.samples.p, pl.samples.data(), pl.samples.size() * sizeof(es::SampleDev),
-                             hipMemcpyHostToDevice, c.stream));
-    ES_HIP(c, hipMemcpyAsync(s.pairs.p, pl.pairs.data(), npairs * sizeof(es::PairDev), hipMemcpyHostToDevice, c.stream));
-    ES_HIP(c, hipMemcpyAsync(s.factor.p, pl.factor.data(), pl.factor.size() * sizeof(double), hipMemcpyHostToDevice,
-                             c.stream));
+    // The tables go up from pinned staging (asynchronous copies); the same
+    // staging later takes the results (stages_collect), so it is sized for both.
+    const size_t bs = pl.samples.size() * sizeof(es::SampleDev), bp = npairs * sizeof(es::PairDev);
+    const size_t bf = pl.factor.size() * sizeof(double);
+    const size_t op = align_up(bs), of = op + align_up(bp);
+    if ((rc = ensure_host(c, std::max(of + bf, collect_bytes(pl))))) return rc;
+    char* h = static_cast<char*>(c.hbuf);
+    std::memcpy(h, pl.samples.data(), bs);
+    std::memcpy(h + op, pl.pairs.data(), bp);
+    std::memcpy(h + of, pl.factor.data(), bf);
+    ES_HIP(c, hipMemcpyAsync(s.samples.p, h, bs, hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemcpyAsync(s.pairs.p, h + op, bp, hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemcpyAsync(s.factor.p, h + of, bf, hipMemcpyHostToDevice, c.stream));
     return ES_OK;
 }
 
@@ -632,13 +729,23 @@ int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, 
     o.sig.resize((size_t)pl.nout * es::kMaxT * 2);
     o.ltsum.resize(pl.nout);
     o.m.resize((size_t)pl.nout * es::kMetricSlots);
-    ES_HIP(c, hipMemcpyAsync(o.sig.data(), s.sig.p, o.sig.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    // Into pinned staging (sized by stages_prepare), then one wait.
+    const size_t bsig = o.sig.size() * sizeof(double), blt = o.ltsum.size() * sizeof(double);
+    const size_t bm = o.m.size() * sizeof(int64_t);
+    const size_t ol = align_up(bsig), om = ol + align_up(blt);
+    if (c.hbuf_cap < om + bm) return fail(c, ES_ERR_INVALID_ARG, "result staging smaller than the plan's results");
+    char* h = static_cast<char*>(c.hbuf);
+    ES_HIP(c, hipMemcpyAsync(h, s.sig.p, bsig, hipMemcpyDeviceToHost, c.stream));
     if (metrics) {
-        ES_HIP(c, hipMemcpyAsync(o.ltsum.data(), s.ltsum.p, o.ltsum.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                 c.stream));
-        ES_HIP(c, hipMemcpyAsync(o.m.data(), s.mout.p, o.m.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+        ES_HIP(c, hipMemcpyAsync(h + ol, s.ltsum.p, blt, hipMemcpyDeviceToHost, c.stream));
+        ES_HIP(c, hipMemcpyAsync(h + om, s.mout.p, bm, hipMemcpyDeviceToHost, c.stream));
     }
     ES_HIP(c, hipStreamSynchronize(c.stream));
+    std::memcpy(o.sig.data(), h, bsig);
+    if (metrics) {
+        std::memcpy(o.ltsum.data(), h + ol, blt);
+        std::memcpy(o.m.data(), h + om, bm);
+    }
     float ms = 0;
     o.t_trace = o.t_temp = 0;
     if (!events) return ES_OK;
@@ -916,31 +1023,54 @@ void es_solver_destroy(es_solver* s)
 
 const char* es_last_error(const es_solver* s) { return s ? s->c.err.c_str() : g_create_error.c_str(); }
 
+// After a new field: the solver's GridInformation (taken from sample 0's grid
+// when the solver was created without one, SimulationRunner.cpp:75-84), the
+// start test of its positions launched behind the binning, and one wait for
+// both read-backs.
+int solver_field_ready(es_solver* s, const es_grid_desc* desc, es_grid_info* out_info)
+{
+    Field& f = s->f;
+    es_grid_info info{};
+    info.region_size = desc->region_size;
+    info.region_extends = desc->region_extends;
+    info.cell_size = f.g.cell_size;
+    info.unique_impurity_count = f.unique;
+    info.index_size = (int32_t)((int64_t)f.g.cpr * f.g.cpr);
+    info.cells_per_row = f.g.cpr;
+    const bool adopt = !(s->desc.grid_info.cell_size > 0);
+    if (adopt) s->desc.grid_info = info;
+    int rc;
+    const int ppr = s->desc.positions_per_row;
+    if (s->desc.grid_info.region_size > 0) {
+        const double dist = s->desc.grid_info.region_size / (double)(ppr - 1);  // Simulation.h:142-143
+        if ((rc = field_positions_launch(s->c, f, ppr, dist, s->desc.grid_info.cell_size * 0.01,
+                                         s->desc.grid_info.cell_size * 0.005)))
+            return rc;
+    }
+    if ((rc = field_sync(s->c, f, false))) return rc;
+    info.indexed_impurity_count = f.g.total;
+    if (adopt) s->desc.grid_info.indexed_impurity_count = f.g.total;
+    if (out_info) *out_info = info;
+    return ES_OK;
+}
+
 int es_grid_set_impurities(es_solver* s, const double* x, const double* y, int64_t n, const es_grid_desc* desc,
                            es_grid_info* out_info)
 {
     if (!s) return ES_ERR_INVALID_ARG;
     ES_HIP(s->c, hipSetDevice(s->c.device));
-    es_grid_info info{};
-    const int rc = field_upload(s->c, s->f, x, y, n, desc, &info);
+    const int rc = field_upload(s->c, s->f, x, y, n, desc, nullptr);
     if (rc) return rc;
-    // GridInformation of sample 0's grid when the solver was created without one
-    // (SimulationRunner.cpp:75-84 builds the Simulation from the first grid).
-    if (!(s->desc.grid_info.cell_size > 0)) s->desc.grid_info = info;
-    if (out_info) *out_info = info;
-    return ES_OK;
+    return solver_field_ready(s, desc, out_info);
 }
 
 int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_desc* desc, es_grid_info* out_info)
 {
     if (!s) return ES_ERR_INVALID_ARG;
     ES_HIP(s->c, hipSetDevice(s->c.device));
-    es_grid_info info{};
-    const int rc = field_generate(s->c, s->f, seed, count, desc, &info);
+    const int rc = field_generate(s->c, s->f, seed, count, desc, nullptr);
     if (rc) return rc;
-    if (!(s->desc.grid_info.cell_size > 0)) s->desc.grid_info = info;
-    if (out_info) *out_info = info;
-    return ES_OK;
+    return solver_field_ready(s, desc, out_info);
 }
 
 int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index)

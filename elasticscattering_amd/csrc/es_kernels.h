@@ -112,8 +112,9 @@ struct BinScratch {
 hipError_t launch_bin_count(const double* x, const double* y, int64_t n, const GridDev& g, BinScratch& s,
                             hipStream_t st);
 hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st);
+// The indexed count is read on the device (total_dev, at most max_total).
 hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s,
-                            hipStream_t st);
+                            const int64_t* total_dev, int64_t max_total, hipStream_t st);
 
 // Start-cell test per position (lifetime.h:84-99) and ordered compaction.
 // bcount: one int per 256 positions.

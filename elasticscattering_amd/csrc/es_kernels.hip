@@ -294,10 +294,12 @@ __global__ __launch_bounds__(256) void k_bin_sort(int64_t ncell, const int64_t* 
     }
 }
 
-__global__ void k_bin_gather(int64_t total, const int32_t* order, const double* x, const double* y, double2* ixy)
+// Grid over the largest possible count; the actual one is read on the device.
+__global__ void k_bin_gather(const int64_t* total, const int32_t* order, const double* x, const double* y,
+                             double2* ixy)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= total) return;
+    if (k >= *total) return;
     const int32_t i = order[k];
     ixy[k] = make_double2(x[i], y[i]);
 }
@@ -1568,7 +1570,8 @@ hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s, hipStream_t st)
+hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s,
+                            const int64_t* total_dev, int64_t max_total, hipStream_t st)
 {
     const int64_t ncell = (int64_t)g.cpr * g.cpr;
     hipError_t e = hipMemsetAsync(s.fill, 0, ncell * sizeof(int32_t), st);
@@ -1580,8 +1583,8 @@ hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev
     }
     hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(ncell, 4)), dim3(256), 0, st, ncell, s.begin, s.order);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (g.total > 0) {
-        hipLaunchKernelGGL(k_bin_gather, dim3(blocks_for(g.total, 256)), dim3(256), 0, st, g.total, s.order, x, y,
+    if (max_total > 0) {
+        hipLaunchKernelGGL(k_bin_gather, dim3(blocks_for(max_total, 256)), dim3(256), 0, st, total_dev, s.order, x, y,
                            g.ixy);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
