@@ -531,8 +531,10 @@ void plan_offsets(Plan& pl, long long seg_items)
         p.inv_w = 1.0 / p.w;
         // About kSigmaItemsPerThread items per thread, so the block's fixed-order
         // double-double reduction is amortised over many items.
+        const es::SampleDev& sd = pl.samples[p.sample];
+        p.nall = p.nitems / sd.tpq * sd.ppq;
         const long long per_blk = (long long)es::kBlock * es::kSigmaItemsPerThread;
-        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nitems + per_blk - 1) / per_blk, 1024));
+        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nall + per_blk - 1) / per_blk, 1024));
         if ((int)i > sg.p0 && item + p.nitems > seg_items) {
             sg.p1 = (int)i;
             sg.items = item;
@@ -808,6 +810,7 @@ es::SampleDev sample_dev(Field& f, int ppr, int ppq, double dist, double offx, d
     d.speed = st.particle_speed;
     d.coherent = coherent;
     d.phi_integrand_factor = ps.phi_integrand_factor;
+    d.tpq = (ps.phi_step == 0.0) ? 1 : ppq;
     d.inv_L = 1.0 / d.L;
     d.inv_size = 1.0 / d.size;
     d.inv_speed = 1.0 / d.speed;
@@ -900,7 +903,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
         p.sample = 0;
         p.out = b;
         p.w = w[b];
-        p.nitems = (long long)sv->f.nfree * ppp;
+        p.nitems = (long long)sv->f.nfree * 4 * pl.samples[0].tpq;
         pl.pairs.push_back(p);
     }
     if (lifetimes_host) {
@@ -1177,7 +1180,7 @@ int batch_plan(es_batch* bt)
             p.sample = (int)e;
             p.out = out++;
             p.w = kE * en.B[b] / kM;
-            p.nitems = (long long)f.nfree * 4 * en.ppq;
+            p.nitems = (long long)f.nfree * 4 * sd.tpq;
             pl.pairs.push_back(p);
         }
     }

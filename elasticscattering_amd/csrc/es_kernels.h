@@ -37,6 +37,10 @@ struct SampleDev {
     const int32_t* free_pos;         // position ids j*ppr + i outside every impurity
     int nfree;
     int ppr, ppq;
+    // Phi points per quadrant that are traced: ppq, or 1 when the window has
+    // zero width (phi_step 0, the coherent pass at alpha = pi/4): its ppq
+    // particles per quadrant are then the same particle, traced once.
+    int tpq;
     double dist, offx, offy;         // distance_between_positions, small_offset
     double phi_start, phi_step, alpha, speed;
     double phi_integrand_factor;     // Simulation.h:116-120
@@ -56,7 +60,8 @@ struct PairDev {
     double w;                        // angular speed E*B/M
     double R, inv_R, inv_w;          // orbit radius speed / w, RN(1 / R), RN(1 / w) (set by the host)
     long long item_off;              // first work item (ascending over pairs)
-    long long nitems;                // nfree * 4 * ppq
+    long long nitems;                // traced work items: nfree * 4 * tpq
+    long long nall;                  // lifetimes: nfree * 4 * ppq (set by the host with the offsets)
     int blk_off;                     // first temperature-stage block (ascending)
     int nblk;
 };

@@ -413,6 +413,30 @@ __device__ __forceinline__ int hinted_pair(const TraceArgs& a, const PairHint& h
     return (item >= h.begin && item < h.end) ? h.pair : pair_of_item(a.pairs, a.npairs, item);
 }
 
+// Work item k of a pair: free-position slot, quadrant q and phi point p of
+// the traced points (tpq per quadrant), and the first lifetime slot it fills
+// in the reference layout [j][i][q][p] (with tpq = 1 it fills all ppq).
+struct ItemPos {
+    long long slot;
+    int q, p;
+    int pidx;
+    long long out_idx;
+};
+
+__device__ __forceinline__ ItemPos item_pos(const TraceArgs& a, const PairDev& pr, const SampleDev& sd, long long item)
+{
+    ItemPos r;
+    const int tpq = sd.tpq, tpp = 4 * sd.tpq;
+    const long long k = item - pr.item_off;
+    r.slot = k / tpp;
+    const int v = (int)(k - r.slot * tpp);
+    r.q = v / tpq;
+    r.p = v - r.q * tpq;
+    r.pidx = sd.free_pos[r.slot];
+    r.out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)r.pidx * (4 * sd.ppq) + r.q * sd.ppq + r.p : -1;
+    return r;
+}
+
 // CreateParticle (lifetime.h:36-58) and the trace start for work item `item`:
 // the lane's grid view, orbit, start angle, bound time and output slot.
 template <bool CW>
@@ -423,14 +447,10 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, const PairHint&
     const int pi = hinted_pair(a, hint, item);
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
-    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
-    const long long k = item - pr.item_off;
-    const long long slot = k / ppp;
-    const int v = (int)(k - slot * ppp);
-    const int q = v / ppq, p = v - (v / ppq) * ppq;
-    const int pidx = sd.free_pos[slot];
+    const ItemPos ip = item_pos(a, pr, sd, item);
+    const int q = ip.q, p = ip.p, pidx = ip.pidx;
     const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
-    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+    out_idx = ip.out_idx;
     g.ixy = sd.ixy;
     g.cell_end = sd.cell_end;
     g.start = sd.start;
@@ -479,6 +499,16 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, const PairHint&
     d_trace_begin(st, px, py, phi, c0x, c0y);
 }
 
+// The lifetime in the reference layout: one slot, or every phi point of the
+// quadrant when they coincide (tpq = 1).
+__device__ __forceinline__ void store_lifetime(const TraceArgs& a, long long item, long long out_idx, double lt)
+{
+    const PairDev& pr = a.pairs[pair_of_item(a.pairs, a.npairs, item)];
+    const SampleDev& sd = a.samples[pr.sample];
+    const int n = (sd.tpq == 1) ? sd.ppq : 1;
+    for (int d = 0; d < n; d++) a.lifetimes[out_idx + d] = lt;
+}
+
 // Writes the per-item results of an ended trace (lifetime.h:142-147).
 __device__ __forceinline__ void d_store_item(const TraceArgs& a, long long item, long long out_idx,
                                              double bound_time, const TraceState& st, int flags)
@@ -492,7 +522,7 @@ __device__ __forceinline__ void d_store_item(const TraceArgs& a, long long item,
     a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
     a.x_item[item] = st.cross_exact;
     a.f_item[item] = (uint8_t)(flags | (a.list_mode ? ES_F_RECOMPUTE : 0));
-    if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+    if (out_idx >= 0) store_lifetime(a, item, out_idx, lt);
 }
 
 // Lane refill: every idle lane takes the next item of the wave's chunk of the
@@ -893,14 +923,10 @@ __device__ __forceinline__ void t32_setup_item(const TraceArgs& a, const PairHin
     const int pi = hinted_pair(a, hint, item);
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
-    const int ppq = sd.ppq, ppp = 4 * sd.ppq;
-    const long long k = item - pr.item_off;
-    const long long slot = k / ppp;
-    const int v = (int)(k - slot * ppp);
-    const int q = v / ppq, p = v - (v / ppq) * ppq;
-    const int pidx = sd.free_pos[slot];
+    const ItemPos ip = item_pos(a, pr, sd, item);
+    const int q = ip.q, p = ip.p, pidx = ip.pidx;
     const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
-    out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)pidx * ppp + v : -1;
+    out_idx = ip.out_idx;
     g.ixy = sd.ixy;
     g.cell_end = sd.cell_end;
     g.start = sd.start;
@@ -979,7 +1005,7 @@ __device__ __forceinline__ void t32_store_item(const TraceArgs& a, long long ite
     a.m_item[item] = make_int4(st.cells, st.tested, st.edge_hits, st.cross_calls);
     a.x_item[item] = 0;
     a.f_item[item] = (uint8_t)flags;
-    if (out_idx >= 0) a.lifetimes[out_idx] = lt;
+    if (out_idx >= 0) store_lifetime(a, item, out_idx, lt);
 }
 
 // The FP32 trace (es_trace32.h) over the same work items, one lane per
@@ -1288,7 +1314,8 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
     const SampleDev& sd = a.samples[pr.sample];
     const int ppq = sd.ppq;
     const int ppp = 4 * ppq;
-    const long long nitems = pr.nitems;
+    const long long nall = pr.nall;
+    const int tpq = sd.tpq;
     const double w = pr.w;
     const double ws = a.clockwise ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
     const int nT = sd.nT;
@@ -1300,12 +1327,13 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
     int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0, m_recomp = 0;
 
     const long long stride = (long long)pr.nblk * kBlock;
-    for (long long k = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x; k < nitems; k += stride) {
-        const long long gi = pr.item_off + k;
-        const double lt_raw = a.lt_item[gi];
+    for (long long k = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x; k < nall; k += stride) {
         const long long slot = k / ppp;
         const int v = (int)(k - slot * ppp);
         const int q = v / ppq, p = v - (v / ppq) * ppq;
+        // the traced item of this lifetime (the quadrant's one item when tpq = 1)
+        const long long gi = pr.item_off + ((tpq == 1) ? slot * 4 + q : k);
+        const double lt_raw = a.lt_item[gi];
         const int pidx = sd.free_pos[slot];
         const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
         if (a.with_metrics) {
@@ -1422,17 +1450,18 @@ __global__ __launch_bounds__(256) void k_images(const SampleDev* sample, const P
     if (slot >= nfree) return;
     const SampleDev& sd = *sample;
     const PairDev& pr = *pair;
-    const int ppq = sd.ppq, ppp = 4 * ppq;
+    const int ppq = sd.ppq;
     const int pidx = sd.free_pos[slot];
     const double ws = clockwise ? -pr.w : pr.w;
-    const long long base = pr.item_off + (long long)slot * ppp;
+    const int tpq = sd.tpq;
+    const long long base = pr.item_off + (long long)slot * 4 * tpq;
     const size_t plane = (size_t)ppr * ppr;
     for (int t = 0; t < nT; t++) {
         const double ml = max_lt[t], tt = tau[t];
         double lt_tot = 0.0, xx_tot = 0.0, xy_tot = 0.0;
         for (int q = 0; q < 4; q++) {
             for (int p = 0; p < ppq; p++) {
-                const double lt_raw = lt_item[base + q * ppq + p];
+                const double lt_raw = lt_item[base + q * tpq + ((tpq == 1) ? 0 : p)];
                 const double lt = (lt_raw < ml) ? lt_raw : ml;
                 const double w = simpson_weight(p, ppq);
                 lt_tot += lt * w;

@@ -122,6 +122,22 @@ def test_file1_lifetimes_bit_exact(es, oracle_esm, cw, coherent):
               [0.01, 12.008, 60.0], [0.01, 0.1, 1, 10, 100])
 
 
+@pytest.mark.parametrize("cw", [1, 0])
+def test_zero_width_coherent_window_bit_exact(es, oracle_esm, cw):
+    """alpha = pi/4: the coherent window has zero width (Simulation.h:115-124), so the
+    ppq phi points of a quadrant are one particle; the GPU traces it once and every
+    lifetime, metric and the (zero) sigma must still equal the oracle's."""
+    x, y = _file_points("test_impurities.dat")
+    _run_pair(es, oracle_esm, x, y, 1e-7, 1e-7, 2e-9, 10, 0, 31, 7, 1.0, math.pi / 4, cw, 1.67834e5, 1,
+              [0.01, 12.008, 60.0], [0.01, 1, 100])
+    w, rs, ext = _small_case()
+    xs = np.empty(w["impurity_count"])
+    ys = np.empty(w["impurity_count"])
+    oracle_esm.generate_impurities(w["seeds"][0], w["impurity_count"], -ext, rs + ext, O.dptr(xs), O.dptr(ys))
+    _run_pair(es, oracle_esm, xs, ys, rs, ext, w["impurity_radius"], 15, w["cells_per_row"], 24, 9, w["tau"],
+              math.pi / 4, cw, w["particle_speed"], 1, [1.0, 20.0], [2.0, 40.0])
+
+
 def test_small_fp64_workload_bit_exact(es, oracle_esm):
     w, rs, ext = _small_case()
     x = np.empty(w["impurity_count"])
