@@ -630,10 +630,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 // is what shrinks.
 
 constexpr int kCoopStack = 320;  // FP32 trace: < 64 left over + up to 4 new disks per lane per group
-constexpr int kCoopCap = 640;    // FP64 trace: queued disks; screened and solved past kCoopCap - 256
+constexpr int kCoopCap = 576;    // FP64 trace: queued disks; screened and solved past kCoopCap - 64 G
 
 struct CoopWave {
-    double orb[10][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry
+    double orb[12][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry P0x P0y
     unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
     const double2* ixy[64];          // per lane: its sample's binned field
     float thr[64];                   // per lane: pseudo-angle threshold of step 2
@@ -699,12 +699,12 @@ __device__ __forceinline__ float d_disk_proxy(double u0x, double u0y, float gmin
 // meets the orbit within asin(r / d) <= 1.0473 r / d (r / d <= 1/2) of the
 // centre's direction, and a crossing disk has d > R - r.  The 1e-5 rad margin
 // covers the FP32 pseudo-angles (< 5e-6 rad) and the FP64 crossing angles.
-__device__ __forceinline__ float d_prune_threshold(const Orb& o, double radius, double t)
+__device__ __forceinline__ float d_prune_threshold(double R, double w, double radius, double t)
 {
     if (!(t < ES_INF)) return 5.0f;
-    const float q = (float)radius / (float)(o.R - radius);
+    const float q = (float)radius / (float)(R - radius);
     if (!(q > 0.0f && q <= 0.5f)) return 5.0f;
-    const double theta = o.w * t + (double)(1.0474f * q) + 1e-5;
+    const double theta = w * t + (double)(1.0474f * q) + 1e-5;
     if (!(theta < ES_PI2 - 1e-4)) return 5.0f;
     float sc, cc;
     sincosf((float)theta, &sc, &cc);
@@ -756,9 +756,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
     long long item = 0, out_idx = -1;
     int cap = 0;
     DevGrid g;
-    Orb o;
-    double phi0 = 0.0, bound_time = 0.0;
-    float gmin2 = 0.0f;
+    double bound_time = 0.0;
     TraceState st;
 
     while (true) {
@@ -766,21 +764,51 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         d_refill(a, total, active, q_next, q_end, exhausted, got, hint);
         if (got >= 0) {
             item = got;
-            d_setup_item<CW>(a, hint, item, g, o, phi0, bound_time, out_idx, cap, st);
-            L.orb[0][lane] = o.cx;
-            L.orb[1][lane] = o.cy;
-            L.orb[2][lane] = o.R;
-            L.orb[3][lane] = o.r2;
-            L.orb[4][lane] = o.pa;
-            L.orb[5][lane] = o.w;
-            L.orb[6][lane] = o.inv_R;
-            L.orb[7][lane] = o.inv_w;
+            // The orbit lives in LDS between cell steps (registers are scarce).
+            Orb os;
+            double phi0s;
+            d_setup_item<CW>(a, hint, item, g, os, phi0s, bound_time, out_idx, cap, st);
+            L.orb[0][lane] = os.cx;
+            L.orb[1][lane] = os.cy;
+            L.orb[2][lane] = os.R;
+            L.orb[3][lane] = os.r2;
+            L.orb[4][lane] = os.pa;
+            L.orb[5][lane] = os.w;
+            L.orb[6][lane] = os.inv_R;
+            L.orb[7][lane] = os.inv_w;
+            L.orb[10][lane] = st.nx.x;  // start point
+            L.orb[11][lane] = st.nx.y;
             L.ixy[lane] = g.ixy;
-            const float gm = 1.001f * (float)g.radius + 1.001e-6f * (float)o.R;
-            gmin2 = gm * gm;
             active = true;
         }
         if (__ballot(active) == 0ull) break;
+
+        // This cell step's view of the orbit, formed as d_setup_item forms it.
+        Orb o;
+        o.cx = L.orb[0][lane];
+        o.cy = L.orb[1][lane];
+        o.R = L.orb[2][lane];
+        o.r2 = L.orb[3][lane];
+        o.pa = L.orb[4][lane];
+        o.w = L.orb[5][lane];
+        o.inv_R = L.orb[6][lane];
+        o.inv_w = L.orb[7][lane];
+        o.R2 = o.R * o.R;
+        o.eps_r = 1e-10 * o.R;
+        {
+            const double radd = o.R + g.radius, rmin = o.R - g.radius;
+            o.radd2 = radd * radd;
+            o.rmin2 = rmin * rmin;
+        }
+        o.u0x = L.orb[10][lane] - o.cx;
+        o.u0y = L.orb[11][lane] - o.cy;
+        o.r = (float)g.radius;
+        const double phi0 = o.pa;
+        float gmin2;
+        {
+            const float gm = 1.001f * (float)g.radius + 1.001e-6f * (float)o.R;
+            gmin2 = gm * gm;
+        }
 
         // This lane's cell: the next cell and angle range (lifetime.h:106-115).
         int s = 0, n = 0;
@@ -799,20 +827,22 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
 
         int i = 0;
         while (true) {
-            // 1. CirclesCross, four disks at a time; the earliest crossing disk
-            // stays with its lane, the others are queued.
+            // 1. CirclesCross, G disks at a time (fewer registers when
+            // screening); the earliest crossing disk stays with its lane, the
+            // others are queued.
+            constexpr int G = SCREEN ? 2 : 4;
             int top = 0;  // wave-uniform
             float first_pa = 3.0e38f;
             int first_i = -1;
             bool more;
-            while ((more = (__ballot(i < n) != 0ull)) && top <= kCoopCap - 256) {
-                double2 p[4];
+            while ((more = (__ballot(i < n) != 0ull)) && top <= kCoopCap - 64 * G) {
+                double2 p[G];
 #pragma unroll
-                for (int u = 0; u < 4; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
+                for (int u = 0; u < G; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
                 int hits = 0;
-                float pa[4];
+                float pa[G];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < G; u++) {
                     const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
                     const double d2 = qx * qx + qy * qy;
                     const bool h = (i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2);
@@ -821,7 +851,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
                 }
                 st.cross_calls += __popc(hits);
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < G; u++) {
                     bool h = (hits >> u) & 1;
                     int qi = s + i + u;
                     float qp = pa[u];
@@ -841,22 +871,18 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
                     }
                     top += __popcll(m);
                 }
-                i += 4;
+                i += G;
             }
 
             // 2. Each lane solves its earliest disk, unless the minimum so far
             // already rules it out; the threshold that results.
+            // (The orbit is read back from LDS, as for a shared solve.)
             if (SCREEN && active) {
-                float thr = d_prune_threshold(o, g.radius, d_from_order_key(L.tmin[lane]));
+                const double R = L.orb[2][lane], w = L.orb[5][lane];
+                float thr = d_prune_threshold(R, w, g.radius, d_from_order_key(L.tmin[lane]));
                 if (first_i >= 0 && !(first_pa >= thr)) {
-                    const double2 p = g.ixy[first_i];
-                    const double t = d_first_cross_time<CW>(o, p.x, p.y, L.orb[8][lane], L.orb[9][lane]);
-                    if (t < ES_INF) {
-                        const unsigned long long kt = d_order_key(t);
-                        if (kt < L.tmin[lane]) L.tmin[lane] = kt;
-                    }
-                    st.cross_exact += 1;
-                    thr = d_prune_threshold(o, g.radius, d_from_order_key(L.tmin[lane]));
+                    d_coop_solve<CW>(L, lane, first_i);
+                    thr = d_prune_threshold(R, w, g.radius, d_from_order_key(L.tmin[lane]));
                 }
                 L.thr[lane] = thr;
             }
