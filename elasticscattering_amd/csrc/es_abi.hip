@@ -533,8 +533,11 @@ void plan_offsets(Plan& pl, long long seg_items)
         // double-double reduction is amortised over many items.
         const es::SampleDev& sd = pl.samples[p.sample];
         p.nall = p.nitems / sd.tpq * sd.ppq;
-        const long long per_blk = (long long)es::kBlock * es::kSigmaItemsPerThread;
-        const int nblk = (int)std::max<long long>(1, std::min<long long>((p.nall + per_blk - 1) / per_blk, 1024));
+        // temperature-stage units: lifetimes, or (position, quadrant) for a zero-width window
+        const bool dup = sd.tpq == 1 && sd.ppq > 1;
+        const long long units = dup ? p.nall / sd.ppq : p.nall;
+        const long long per_blk = (long long)es::kBlock * (dup ? 1 : es::kSigmaItemsPerThread);
+        const int nblk = (int)std::max<long long>(1, std::min<long long>((units + per_blk - 1) / per_blk, 1024));
         if ((int)i > sg.p0 && item + p.nitems > seg_items) {
             sg.p1 = (int)i;
             sg.items = item;

@@ -94,6 +94,24 @@ __device__ __forceinline__ double simpson_weight(int i, int dim)  // util.h:24-2
     return is_edge ? 1.0 : main_multiplier;
 }
 
+// Sum of the dim Simpson weights (an integer, exact in a double).
+__device__ __forceinline__ double simpson_weight_sum(int dim)
+{
+    double s = 0.0;
+    for (int i = 0; i < dim; i++) s += simpson_weight(i, dim);
+    return s;
+}
+
+// a += x * y with the product's rounding error added too (fma): the double-
+// double accumulator receives x * y exactly.
+__device__ __forceinline__ void dd_add_exact_product(DD& a, double x, double y)
+{
+    const double hi = x * y;
+    const double lo = fma(x, y, -hi);
+    dd_add_d(a, hi);
+    dd_add_d(a, lo);
+}
+
 // ------------------------------------------------------------------ generate
 
 __global__ void k_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y)
@@ -1331,6 +1349,99 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
 // --------------------------------------------------------------------- sigma
 
+struct SigmaMetrics {
+    int64_t cells = 0, tested = 0, max_cells = 0, max_tested = 0, edge = 0, cross = 0, sigma = 0;
+    int64_t escaped = 0, at_bound = 0, cap = 0, items = 0, exact = 0, recomp = 0;
+};
+
+// One unit of the temperature stage (see k_sigma): DUP for a zero-width phi
+// window, whose unit is a (position, quadrant) standing for ppq lifetimes.
+template <int NT, bool DUP>
+__device__ __forceinline__ void sigma_unit(const SigmaArgs& a, const PairDev& pr, const SampleDev& sd, long long u,
+                                           double ws, DD* acc, SigmaMetrics& ms)
+{
+    const int ppq = sd.ppq, ppp = 4 * ppq, nT = sd.nT;
+    const int reps = DUP ? ppq : 1;
+    long long slot;
+    int q, p0;
+    if (DUP) {
+        slot = u >> 2;
+        q = (int)(u & 3);
+        p0 = 0;
+    } else {
+        slot = u / ppp;
+        const int v = (int)(u - slot * ppp);
+        q = v / ppq;
+        p0 = v - q * ppq;
+    }
+    const long long gi = pr.item_off + u;
+    const double lt_raw = a.lt_item[gi];
+    const int pidx = sd.free_pos[slot];
+    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    if (a.with_metrics) {
+        const int4 m = a.m_item[gi];
+        const int f = a.f_item[gi];
+        ms.cells += (int64_t)reps * m.x;
+        ms.tested += (int64_t)reps * m.y;
+        ms.edge += (int64_t)reps * m.z;
+        ms.cross += (int64_t)reps * m.w;
+        ms.exact += (int64_t)reps * a.x_item[gi];
+        ms.max_cells = m.x > ms.max_cells ? m.x : ms.max_cells;
+        ms.max_tested = m.y > ms.max_tested ? m.y : ms.max_tested;
+        ms.escaped += (f & ES_F_ESCAPED) ? reps : 0;
+        ms.at_bound += (f & ES_F_AT_BOUND) ? reps : 0;
+        ms.cap += (f & ES_F_CAP) ? reps : 0;
+        ms.recomp += (f & ES_F_RECOMPUTE) ? reps : 0;
+        ms.items += reps;
+    }
+    if (DUP) {
+        dd_add_exact_product(acc[2 * NT], lt_raw, (double)reps);
+    } else {
+        dd_add_d(acc[2 * NT], lt_raw);
+    }
+
+    // ApplySigma + IntegrateResult terms (SimulationCPU.cpp:178-222): a zero
+    // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
+    if (lt_raw == 0.0) return;  // every temperature's term is +-0
+    const double phi = sd.phi_start + q * ES_HALF_PI + p0 * sd.phi_step;
+    double sphi, cphi;
+    esm_sincos(phi, &sphi, &cphi);
+    const double wji = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr);
+    // sin, cos of the end angle for the untruncated lifetime, shared by every
+    // temperature whose cap does not cut it (the same argument, so the same bits).
+    double sa0, ca0;
+    esm_sincos(phi + ws * lt_raw, &sa0, &ca0);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        if (t < nT) {
+            const double ml = sd.max_lt[t];
+            const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
+            if (lt != 0.0) {
+                const double tau = sd.tau[t];
+                const double z = esm_exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
+                double sa = sa0, ca = ca0;
+                if (lt != lt_raw) esm_sincos(phi + ws * lt, &sa, &ca);
+                double r = cphi - ca * z;
+                r += ws * tau * sa * z;
+                r -= ws * tau * sphi;
+                const double rc = r * cphi, rs = r * sphi;
+                if (DUP) {
+                    // sum over the ppq phi points of rc * wji * w_p: every product is
+                    // rc times a power of two, so the sum is exactly rc * (wji * W)
+                    const double wsum = wji * simpson_weight_sum(ppq);
+                    dd_add_exact_product(acc[2 * t], rc, wsum);
+                    dd_add_exact_product(acc[2 * t + 1], rs, wsum);
+                } else {
+                    const double wgt = wji * simpson_weight(p0, ppq);
+                    dd_add_d(acc[2 * t], rc * wgt);
+                    dd_add_d(acc[2 * t + 1], rs * wgt);
+                }
+                ms.sigma += reps;
+            }
+        }
+    }
+    }
+
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 {
@@ -1339,78 +1450,33 @@ __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
     const int ppq = sd.ppq;
-    const int ppp = 4 * ppq;
     const long long nall = pr.nall;
     const int tpq = sd.tpq;
     const double w = pr.w;
     const double ws = a.clockwise ? -w : w;  // signed_angular_speed (SimulationCPU.cpp:108)
-    const int nT = sd.nT;
 
     DD acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q].s = acc[q].c = 0.0;
-    int64_t m_cells = 0, m_tested = 0, m_max_cells = 0, m_max_tested = 0, m_edge = 0, m_cross = 0, m_sigma = 0;
-    int64_t m_escaped = 0, m_at_bound = 0, m_cap = 0, m_items = 0, m_exact = 0, m_recomp = 0;
+    SigmaMetrics ms;
 
+    // A unit is one lifetime, or with tpq = 1 (a zero-width phi window) one
+    // (position, quadrant) whose ppq lifetimes are the same particle: its
+    // exp / sincos terms are formed once and added ppq times with each phi
+    // point's Simpson weight (products by 1, 2, 4: the terms of tracing them
+    // one by one).  Either way the traced item of unit u is item_off + u.
     const long long stride = (long long)pr.nblk * kBlock;
-    for (long long k = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x; k < nall; k += stride) {
-        const long long slot = k / ppp;
-        const int v = (int)(k - slot * ppp);
-        const int q = v / ppq, p = v - (v / ppq) * ppq;
-        // the traced item of this lifetime (the quadrant's one item when tpq = 1)
-        const long long gi = pr.item_off + ((tpq == 1) ? slot * 4 + q : k);
-        const double lt_raw = a.lt_item[gi];
-        const int pidx = sd.free_pos[slot];
-        const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
-        if (a.with_metrics) {
-            const int4 m = a.m_item[gi];
-            const int f = a.f_item[gi];
-            m_cells += m.x;
-            m_tested += m.y;
-            m_edge += m.z;
-            m_cross += m.w;
-            m_exact += a.x_item[gi];
-            m_max_cells = m.x > m_max_cells ? m.x : m_max_cells;
-            m_max_tested = m.y > m_max_tested ? m.y : m_max_tested;
-            m_escaped += (f & ES_F_ESCAPED) ? 1 : 0;
-            m_at_bound += (f & ES_F_AT_BOUND) ? 1 : 0;
-            m_cap += (f & ES_F_CAP) ? 1 : 0;
-            m_recomp += (f & ES_F_RECOMPUTE) ? 1 : 0;
-            m_items += 1;
-        }
-        dd_add_d(acc[2 * NT], lt_raw);
-
-        // ApplySigma + IntegrateResult terms (SimulationCPU.cpp:178-222): a zero
-        // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
-        if (lt_raw == 0.0) continue;  // every temperature's term is +-0
-        const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
-        double sphi, cphi;
-        esm_sincos(phi, &sphi, &cphi);
-        const double wgt = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr) * simpson_weight(p, ppq);
-        // sin, cos of the end angle for the untruncated lifetime, shared by every
-        // temperature whose cap does not cut it (the same argument, so the same bits).
-        double sa0, ca0;
-        esm_sincos(phi + ws * lt_raw, &sa0, &ca0);
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            if (t < nT) {
-                const double ml = sd.max_lt[t];
-                const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
-                if (lt != 0.0) {
-                    const double tau = sd.tau[t];
-                    const double z = esm_exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
-                    double sa = sa0, ca = ca0;
-                    if (lt != lt_raw) esm_sincos(phi + ws * lt, &sa, &ca);
-                    double r = cphi - ca * z;
-                    r += ws * tau * sa * z;
-                    r -= ws * tau * sphi;
-                    dd_add_d(acc[2 * t], r * cphi * wgt);
-                    dd_add_d(acc[2 * t + 1], r * sphi * wgt);
-                    m_sigma += 1;
-                }
-            }
-        }
+    const long long u0 = (long long)(blockIdx.x - pr.blk_off) * kBlock + threadIdx.x;
+    if (tpq == 1) {
+        for (long long u = u0; u < nall / ppq; u += stride)
+            sigma_unit<NT, true>(a, pr, sd, u, ws, acc, ms);
+    } else {
+        for (long long u = u0; u < nall; u += stride)
+            sigma_unit<NT, false>(a, pr, sd, u, ws, acc, ms);
     }
+    int64_t m_cells = ms.cells, m_tested = ms.tested, m_max_cells = ms.max_cells, m_max_tested = ms.max_tested;
+    int64_t m_edge = ms.edge, m_cross = ms.cross, m_sigma = ms.sigma, m_escaped = ms.escaped;
+    int64_t m_at_bound = ms.at_bound, m_cap = ms.cap, m_items = ms.items, m_exact = ms.exact, m_recomp = ms.recomp;
 
     // Block reduction in a fixed order: wave shuffles, then the 4 waves in LDS.
     __shared__ double sh_s[NQ][kBlock / 64];

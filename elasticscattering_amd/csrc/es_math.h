@@ -183,6 +183,14 @@ ESM_FN double esm_asin_r(double t)
     return p / q;
 }
 
+/*
+ * Branch-free arrangement of fdlibm's three ranges (|x| < 0.5, 0.5 <= |x| <
+ * 0.975, 0.975 <= |x| < 1): the rational part is evaluated once, for the
+ * argument of whichever range x is in, and the results are selected.  Every
+ * range performs exactly the operations of e_asin.c, so the value is the
+ * same; a GPU wave whose lanes fall in different ranges then runs one
+ * rational evaluation instead of two.
+ */
 ESM_FN double esm_asin(double x)
 {
     const double pio2_hi = 1.57079632679489655800e+00; /* 0x3FF921FB54442D18 */
@@ -193,26 +201,21 @@ ESM_FN double esm_asin(double x)
         if (ax == 1.0) return x * pio2_hi + x * pio2_lo;
         return (x - x) / (x - x);  /* |x| > 1 or NaN: NaN */
     }
-    if (ax < 0.5) {
-        if (ax < 7.450580596923828125e-9) return x;  /* 2^-27 */
-        const double t = x * x;
-        return fma(x, esm_asin_r(t), x);
-    }
-    const double w = 1.0 - ax;
-    const double t = w * 0.5;
+    const int small = ax < 0.5;
+    const double t = small ? x * x : (1.0 - ax) * 0.5;
     const double r = esm_asin_r(t);
+    /* |x| < 0.5 (below 2^-27 asin(x) rounds to x) */
+    const double res_small = (ax < 7.450580596923828125e-9) ? x : fma(x, r, x);
+    /* 0.5 <= |x| < 1 */
     const double s = sqrt(t);
-    double res;
-    if (ax >= 0.975) {
-        res = pio2_hi - (2.0 * fma(s, r, s) - pio2_lo);
-    } else {
-        const double f = esm_from_bits(esm_bits(s) & 0xFFFFFFFF00000000ULL);
-        const double c = (t - f * f) / (s + f);
-        const double p = 2.0 * s * r - (pio2_lo - 2.0 * c);
-        const double q = pio4_hi - 2.0 * f;
-        res = pio4_hi - (p - q);
-    }
-    return (x < 0.0) ? -res : res;
+    const double f = esm_from_bits(esm_bits(s) & 0xFFFFFFFF00000000ULL);
+    const double c = (t - f * f) / (s + f);
+    const double res_top = pio2_hi - (2.0 * fma(s, r, s) - pio2_lo);
+    const double p = 2.0 * s * r - (pio2_lo - 2.0 * c);
+    const double q = pio4_hi - 2.0 * f;
+    const double res_mid = pio4_hi - (p - q);
+    const double big = (ax >= 0.975) ? res_top : res_mid;
+    return small ? res_small : ((x < 0.0) ? -big : big);
 }
 
 /* exp (fdlibm e_exp.c). */
