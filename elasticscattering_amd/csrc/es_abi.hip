@@ -214,11 +214,11 @@ struct Field {
     double pos_dist = 0, pos_offx = 0, pos_offy = 0;
     int nfree = 0;
     int64_t n_inside = 0;
-    // Read-backs in flight, collected by field_sync: [0] indexed count,
-    // [1] out-of-grid flag and free-position count (pinned host memory).
+    // Read-backs in flight, collected by field_sync (pinned copy of `small`).
     int64_t* hsmall = nullptr;
     bool total_pending = false;
     bool pos_pending = false;
+    bool counts_clean = false;  // b_counts all zero (left so by every binning's scan)
 };
 
 void field_release(Ctx& c, Field& f)
@@ -243,22 +243,34 @@ int field_host_small(Ctx& c, Field& f)
     return ES_OK;
 }
 
-// Waits for the stream and takes the field's read-backs in flight: the
-// indexed impurity count of a binning and the start test of the positions.
-// With pos_errors = false a start position outside the grid only leaves the
+// Device word block `small` (16 B): [0] out-of-grid flag (int32), [1] free
+// positions (int32), [2..3] indexed impurity count (int64); f.hsmall is its
+// pinned host copy.
+int field_small(Ctx& c, Field& f)
+{
+    int rc;
+    if ((rc = ensure(c, f.small, 16, "field read-back words"))) return rc;
+    return field_host_small(c, f);
+}
+
+// Takes the field's read-backs in flight (the indexed count of a binning, the
+// start test of the positions) with one copy and one wait.  With
+// pos_errors = false a start position outside the grid only leaves the
 // positions invalid (the next field_positions reports it).
 int field_sync(Ctx& c, Field& f, bool pos_errors = true)
 {
+    if (f.total_pending || f.pos_pending)
+        ES_HIP(c, hipMemcpyAsync(f.hsmall, f.small.p, 16, hipMemcpyDeviceToHost, c.stream));
     ES_HIP(c, hipStreamSynchronize(c.stream));
+    const int32_t* h = reinterpret_cast<const int32_t*>(f.hsmall);
     if (f.total_pending) {
         f.total_pending = false;
-        const int64_t total = f.hsmall[0];
+        const int64_t total = f.hsmall[1];
         if (total > INT32_MAX) return fail(c, ES_ERR_INVALID_ARG, "indexed impurity count exceeds int32");
         f.g.total = total;
     }
     if (f.pos_pending) {
         f.pos_pending = false;
-        const int32_t* h = reinterpret_cast<const int32_t*>(f.hsmall + 1);
         if (h[0] != 0 && !pos_errors) return ES_OK;
         if (h[0] != 0)
             return fail(c, ES_ERR_OUT_OF_GRID,
@@ -272,8 +284,11 @@ int field_sync(Ctx& c, Field& f, bool pos_errors = true)
     return ES_OK;
 }
 
-// Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180).
-int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* out)
+// Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180), drawing them
+// first when `generate` (key, range [lo, hi)).  Four launches, no wait: the
+// indexed count is read back by field_sync (at once when `out` is wanted).
+int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* out, bool generate = false,
+              uint64_t key = 0, double lo = 0.0, double hi = 0.0)
 {
     if (d->target_cell_population <= 0 && d->cells_per_row <= 0)
         return fail(c, ES_ERR_INVALID_ARG, "target_cell_population must be > 0");
@@ -289,32 +304,33 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* 
     g.cell_size = g.size / (double)cpr;              // Grid.cpp:55, Grid.h:89
     g.radius = d->impurity_radius;
     const int64_t ncell = (int64_t)cpr * cpr;
+    // Every impurity lands in at most 9 cells (Grid.cpp:112-180), so the
+    // binned arrays are sized for that and the count is read back later.
+    const int64_t max_total = 9 * n;
     int rc;
     if ((rc = ensure(c, f.b_targets, 9 * n * sizeof(int32_t), "bin targets"))) return rc;
     if ((rc = ensure(c, f.b_ntarg, n, "bin target counts"))) return rc;
+    const size_t had = f.b_counts.cap;
     if ((rc = ensure(c, f.b_counts, ncell * sizeof(int32_t), "cell counts"))) return rc;
+    if (f.b_counts.cap != had) f.counts_clean = false;
     if ((rc = ensure(c, f.b_fill, ncell * sizeof(int32_t), "cell fill"))) return rc;
     if ((rc = ensure(c, f.b_begin, (ncell + 1) * sizeof(int64_t), "cell begin"))) return rc;
     if ((rc = ensure(c, f.b_bsum, ((ncell + 255) / 256) * sizeof(int64_t), "scan block sums"))) return rc;
     if ((rc = ensure(c, f.cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
-    g.cell_end = P<int32_t>(f.cell_end);
-    es::BinScratch bs{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts), P<int32_t>(f.b_fill),
-                      P<int64_t>(f.b_begin),   P<int64_t>(f.b_bsum),   nullptr};
-    const double* x = P<double>(f.in_x);
-    const double* y = P<double>(f.in_y);
-    ES_HIP(c, es::launch_bin_count(x, y, n, g, bs, c.stream));
-    ES_HIP(c, es::launch_bin_scan(g, bs, c.stream));
-    // Every impurity lands in at most 9 cells (Grid.cpp:112-180), so the
-    // binned arrays are sized for that and the count is read back later.
-    const int64_t max_total = 9 * n;
     if ((rc = ensure(c, f.b_order, max_total * sizeof(int32_t), "cell order"))) return rc;
     if ((rc = ensure(c, f.ixy, max_total * sizeof(double2), "impurities"))) return rc;
-    if ((rc = field_host_small(c, f))) return rc;
-    bs.order = P<int32_t>(f.b_order);
+    if ((rc = field_small(c, f))) return rc;
+    // The counts start at zero: the scan of every binning leaves them so.
+    if (!f.counts_clean) ES_HIP(c, hipMemsetAsync(f.b_counts.p, 0, f.b_counts.cap, c.stream));
+    f.counts_clean = false;
+    g.cell_end = P<int32_t>(f.cell_end);
     g.ixy = P<double2>(f.ixy);
-    const int64_t* total_dev = P<int64_t>(f.b_begin) + ncell;
-    ES_HIP(c, es::launch_bin_place(x, y, n, g, bs, total_dev, max_total, c.stream));
-    ES_HIP(c, hipMemcpyAsync(f.hsmall, total_dev, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    es::BinScratch bs{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts), P<int32_t>(f.b_fill),
+                      P<int64_t>(f.b_begin),   P<int64_t>(f.b_bsum),   P<int32_t>(f.b_order)};
+    int64_t* total_dev = P<int64_t>(f.small) + 1;
+    ES_HIP(c, es::launch_bin(P<double>(f.in_x), P<double>(f.in_y), n, generate ? 1 : 0, key, lo, hi, g, bs, total_dev,
+                             c.stream));
+    f.counts_clean = true;
     f.total_pending = true;
     f.unique = n;
     f.has_grid = true;
@@ -364,8 +380,7 @@ int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid
     int rc;
     if ((rc = ensure(c, f.in_x, n * sizeof(double), "impurity input x"))) return rc;
     if ((rc = ensure(c, f.in_y, n * sizeof(double), "impurity input y"))) return rc;
-    ES_HIP(c, es::launch_generate(esr_key(seed), n, lo, hi, P<double>(f.in_x), P<double>(f.in_y), c.stream));
-    return field_bin(c, f, n, d, out);
+    return field_bin(c, f, n, d, out, true, esr_key(seed), lo, hi);
 }
 
 // Launches the start cell + inside test for every position of a ppr x ppr
@@ -377,12 +392,10 @@ int field_positions_launch(Ctx& c, Field& f, int ppr, double dist, double offx, 
     if ((rc = ensure(c, f.flags, n, "position flags"))) return rc;
     if ((rc = ensure(c, f.free_pos, n * sizeof(int32_t), "free positions"))) return rc;
     if ((rc = ensure(c, f.bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
-    if ((rc = ensure(c, f.small, 2 * sizeof(int32_t), "flags"))) return rc;
-    if ((rc = field_host_small(c, f))) return rc;
+    if ((rc = field_small(c, f))) return rc;
     int32_t* small = P<int32_t>(f.small);
     ES_HIP(c, es::launch_positions(f.g, ppr, dist, offx, offy, P<uint8_t>(f.flags), P<int32_t>(f.bcount), small,
                                    P<int32_t>(f.free_pos), small + 1, c.stream));
-    ES_HIP(c, hipMemcpyAsync(f.hsmall + 1, small, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     f.pos_valid = false;
     f.pos_pending = true;
     f.pos_ppr = ppr;
@@ -469,11 +482,34 @@ constexpr double kScreenPopulation = 10.0;
 
 // Device buffers and the optional captured graph of the stage kernels.
 struct Stages {
-    DevBuf samples, pairs, factor, dpart, mpart, sig, ltsum, mout, lt_item, m_item, x_item, f_item, queue, lifetimes;
+    // tables: sample table | pair table | factors, uploaded in one copy;
+    // outs: sigma | lifetime sums | metrics, downloaded in one copy.
+    DevBuf tables, outs, dpart, mpart, lt_item, m_item, x_item, f_item, queue, lifetimes;
     DevBuf list, list_count;  // mixed precision: items the FP32 trace hands to the FP64 one
+    size_t t_pairs = 0, t_factor = 0;  // byte offsets in tables
+    size_t o_ltsum = 0, o_mout = 0;    // byte offsets in outs
+    // The temperature stage leaves the work queue and the recompute count at
+    // zero for the next run; after a run that did not complete they are cleared.
+    bool queue_dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+es::SampleDev* d_samples(Stages& s) { return P<es::SampleDev>(s.tables); }
+es::PairDev* d_pairs(Stages& s) { return reinterpret_cast<es::PairDev*>(P<char>(s.tables) + s.t_pairs); }
+double* d_factor(Stages& s) { return reinterpret_cast<double*>(P<char>(s.tables) + s.t_factor); }
+double* d_sig(Stages& s) { return P<double>(s.outs); }
+double* d_ltsum(Stages& s) { return reinterpret_cast<double*>(P<char>(s.outs) + s.o_ltsum); }
+int64_t* d_mout(Stages& s) { return reinterpret_cast<int64_t*>(P<char>(s.outs) + s.o_mout); }
+
+int stages_reset_queue(Ctx& c, Stages& s)
+{
+    if (!s.queue_dirty) return ES_OK;
+    if (s.queue.p) ES_HIP(c, hipMemsetAsync(s.queue.p, 0, sizeof(unsigned long long), c.stream));
+    if (s.list_count.p) ES_HIP(c, hipMemsetAsync(s.list_count.p, 0, sizeof(int32_t), c.stream));
+    s.queue_dirty = false;
+    return ES_OK;
+}
 
 void stages_release(Ctx& c, Stages& s)
 {
@@ -481,7 +517,7 @@ void stages_release(Ctx& c, Stages& s)
     if (s.graph) (void)hipGraphDestroy(s.graph);
     s.exec = nullptr;
     s.graph = nullptr;
-    DevBuf* bufs[] = {&s.samples, &s.pairs,  &s.factor, &s.dpart,  &s.mpart, &s.sig,       &s.ltsum, &s.mout,
+    DevBuf* bufs[] = {&s.tables,  &s.outs,   &s.dpart,  &s.mpart,
                       &s.lt_item, &s.m_item, &s.x_item, &s.f_item, &s.queue, &s.lifetimes, &s.list,  &s.list_count};
     for (DevBuf* b : bufs) release(c, *b);
 }
@@ -569,51 +605,49 @@ void plan_offsets(Plan& pl, long long seg_items)
 
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Bytes of pinned staging the results of a plan take: sigma, lifetime sums, metrics.
-size_t collect_bytes(const Plan& pl)
-{
-    return align_up((size_t)pl.nout * es::kMaxT * 2 * sizeof(double)) + align_up((size_t)pl.nout * sizeof(double)) +
-           (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t);
-}
-
 int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
 {
     const int NQ = 2 * pl.NT + 1;
     const size_t npairs = pl.pairs.size();
     int rc;
-    if ((rc = ensure(c, s.samples, pl.samples.size() * sizeof(es::SampleDev), "sample table"))) return rc;
-    if ((rc = ensure(c, s.pairs, npairs * sizeof(es::PairDev), "pair table"))) return rc;
-    if ((rc = ensure(c, s.factor, (size_t)pl.nout * es::kMaxT * sizeof(double), "factors"))) return rc;
+    const size_t bs = pl.samples.size() * sizeof(es::SampleDev), bp = npairs * sizeof(es::PairDev);
+    const size_t bf = (size_t)pl.nout * es::kMaxT * sizeof(double);
+    s.t_pairs = align_up(bs);
+    s.t_factor = s.t_pairs + align_up(bp);
+    const size_t tables = s.t_factor + bf;
+    const size_t bsig = (size_t)pl.nout * es::kMaxT * 2 * sizeof(double), blt = (size_t)pl.nout * sizeof(double);
+    s.o_ltsum = align_up(bsig);
+    s.o_mout = s.o_ltsum + align_up(blt);
+    const size_t outs = s.o_mout + (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t);
+    if ((rc = ensure(c, s.tables, tables, "sample / pair tables"))) return rc;
+    if ((rc = ensure(c, s.outs, outs, "results"))) return rc;
     if ((rc = ensure(c, s.dpart, (size_t)pl.max_blocks * NQ * 2 * sizeof(double), "partials"))) return rc;
     if ((rc = ensure(c, s.mpart, (size_t)pl.max_blocks * es::kMetricSlots * sizeof(int64_t), "metric partials")))
         return rc;
-    if ((rc = ensure(c, s.sig, (size_t)pl.nout * es::kMaxT * 2 * sizeof(double), "sigma out"))) return rc;
-    if ((rc = ensure(c, s.ltsum, (size_t)pl.nout * sizeof(double), "lifetime sums"))) return rc;
-    if ((rc = ensure(c, s.mout, (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t), "metrics out"))) return rc;
     const size_t items = (size_t)std::max<long long>(pl.max_items, 1);
     if ((rc = ensure(c, s.lt_item, items * sizeof(double), "item lifetimes"))) return rc;
     if ((rc = ensure(c, s.m_item, items * sizeof(int4), "item counters"))) return rc;
     if ((rc = ensure(c, s.x_item, items * sizeof(int32_t), "item exact solves"))) return rc;
     if ((rc = ensure(c, s.f_item, items, "item flags"))) return rc;
+    size_t had = s.queue.cap;
     if ((rc = ensure(c, s.queue, sizeof(unsigned long long), "work queue"))) return rc;
+    if (s.queue.cap != had) s.queue_dirty = true;
     if (pl.precision == ES_PRECISION_MIXED) {
         if ((rc = ensure(c, s.list, items * sizeof(int32_t), "recompute list"))) return rc;
+        had = s.list_count.cap;
         if ((rc = ensure(c, s.list_count, sizeof(int32_t), "recompute count"))) return rc;
+        if (s.list_count.cap != had) s.queue_dirty = true;
     }
     if ((rc = ctx_events(c, 4 * pl.segs.size()))) return rc;
-    // The tables go up from pinned staging (asynchronous copies); the same
+    if ((rc = stages_reset_queue(c, s))) return rc;
+    // The tables go up from pinned staging in one asynchronous copy; the same
     // staging later takes the results (stages_collect), so it is sized for both.
-    const size_t bs = pl.samples.size() * sizeof(es::SampleDev), bp = npairs * sizeof(es::PairDev);
-    const size_t bf = pl.factor.size() * sizeof(double);
-    const size_t op = align_up(bs), of = op + align_up(bp);
-    if ((rc = ensure_host(c, std::max(of + bf, collect_bytes(pl))))) return rc;
+    if ((rc = ensure_host(c, std::max(tables, outs)))) return rc;
     char* h = static_cast<char*>(c.hbuf);
     std::memcpy(h, pl.samples.data(), bs);
-    std::memcpy(h + op, pl.pairs.data(), bp);
-    std::memcpy(h + of, pl.factor.data(), bf);
-    ES_HIP(c, hipMemcpyAsync(s.samples.p, h, bs, hipMemcpyHostToDevice, c.stream));
-    ES_HIP(c, hipMemcpyAsync(s.pairs.p, h + op, bp, hipMemcpyHostToDevice, c.stream));
-    ES_HIP(c, hipMemcpyAsync(s.factor.p, h + of, bf, hipMemcpyHostToDevice, c.stream));
+    std::memcpy(h + s.t_pairs, pl.pairs.data(), bp);
+    std::memcpy(h + s.t_factor, pl.factor.data(), bf);
+    ES_HIP(c, hipMemcpyAsync(s.tables.p, h, tables, hipMemcpyHostToDevice, c.stream));
     return ES_OK;
 }
 
@@ -670,18 +704,21 @@ int trace_blocks(Ctx& c, int cw, const Shape& sh, const Plan& pl, std::vector<in
 int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trace, int with_metrics,
                    double* lifetimes_dev, const std::vector<int>& nblocks, bool events = true, int nblocks64 = 0)
 {
+    s.queue_dirty = true;  // until stages_collect has seen the run through
     for (size_t si = 0; si < pl.segs.size(); si++) {
         const Segment& g = pl.segs[si];
-        es::PairDev* pairs = P<es::PairDev>(s.pairs) + g.p0;
+        es::PairDev* pairs = d_pairs(s) + g.p0;
         const int npairs = g.p1 - g.p0;
         if (trace) {
             es::TraceArgs a{};
-            a.samples = P<es::SampleDev>(s.samples);
+            a.samples = d_samples(s);
             a.pairs = pairs;
             a.npairs = npairs;
             a.total_items = g.items;
             a.queue = P<unsigned long long>(s.queue);
-            a.chunk = sh.chunk;
+            // A small launch gives every wave work: at most the items per wave of its grid.
+            const long long per_wave = (g.items + 4LL * nblocks[si] - 1) / (4LL * nblocks[si]);
+            a.chunk = (int)std::max<long long>(16, std::min<long long>(sh.chunk, per_wave));
             a.lt_item = P<double>(s.lt_item);
             a.m_item = P<int4>(s.m_item);
             a.x_item = P<int32_t>(s.x_item);
@@ -700,7 +737,7 @@ int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trac
             if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 1], c.stream));
         }
         es::SigmaArgs sa{};
-        sa.samples = P<es::SampleDev>(s.samples);
+        sa.samples = d_samples(s);
         sa.pairs = pairs;
         sa.npairs = npairs;
         sa.total_blocks = g.blocks;
@@ -712,11 +749,12 @@ int stages_enqueue(Ctx& c, Stages& s, const Plan& pl, const Shape& sh, bool trac
         sa.with_metrics = with_metrics;
         sa.dpart = P<double>(s.dpart);
         sa.mpart = P<int64_t>(s.mpart);
+        sa.queue = P<unsigned long long>(s.queue);
+        sa.list_count = (pl.precision == ES_PRECISION_MIXED) ? P<int32_t>(s.list_count) : nullptr;
         if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 2], c.stream));
         ES_HIP(c, es::launch_sigma(sa, pl.NT, c.stream));
         ES_HIP(c, es::launch_finalize(sa.samples, sa.pairs, npairs, pl.NT, with_metrics, sa.dpart, sa.mpart,
-                                      P<double>(s.factor), P<double>(s.sig), P<double>(s.ltsum),
-                                      P<int64_t>(s.mout), c.stream));
+                                      d_factor(s), d_sig(s), d_ltsum(s), d_mout(s), c.stream));
         if (events) ES_HIP(c, hipEventRecord(c.ev[4 * si + 3], c.stream));
     }
     return ES_OK;
@@ -734,22 +772,19 @@ int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, 
     o.sig.resize((size_t)pl.nout * es::kMaxT * 2);
     o.ltsum.resize(pl.nout);
     o.m.resize((size_t)pl.nout * es::kMetricSlots);
-    // Into pinned staging (sized by stages_prepare), then one wait.
+    // Into pinned staging (sized by stages_prepare) in one copy, then one wait.
     const size_t bsig = o.sig.size() * sizeof(double), blt = o.ltsum.size() * sizeof(double);
     const size_t bm = o.m.size() * sizeof(int64_t);
-    const size_t ol = align_up(bsig), om = ol + align_up(blt);
-    if (c.hbuf_cap < om + bm) return fail(c, ES_ERR_INVALID_ARG, "result staging smaller than the plan's results");
+    const size_t bytes = metrics ? s.o_mout + bm : bsig;
+    if (c.hbuf_cap < bytes) return fail(c, ES_ERR_INVALID_ARG, "result staging smaller than the plan's results");
     char* h = static_cast<char*>(c.hbuf);
-    ES_HIP(c, hipMemcpyAsync(h, s.sig.p, bsig, hipMemcpyDeviceToHost, c.stream));
-    if (metrics) {
-        ES_HIP(c, hipMemcpyAsync(h + ol, s.ltsum.p, blt, hipMemcpyDeviceToHost, c.stream));
-        ES_HIP(c, hipMemcpyAsync(h + om, s.mout.p, bm, hipMemcpyDeviceToHost, c.stream));
-    }
+    ES_HIP(c, hipMemcpyAsync(h, s.outs.p, bytes, hipMemcpyDeviceToHost, c.stream));
     ES_HIP(c, hipStreamSynchronize(c.stream));
+    s.queue_dirty = false;
     std::memcpy(o.sig.data(), h, bsig);
     if (metrics) {
-        std::memcpy(o.ltsum.data(), h + ol, blt);
-        std::memcpy(o.m.data(), h + om, bm);
+        std::memcpy(o.ltsum.data(), h + s.o_ltsum, blt);
+        std::memcpy(o.m.data(), h + s.o_mout, bm);
     }
     float ms = 0;
     o.t_trace = o.t_temp = 0;
@@ -1142,7 +1177,7 @@ int es_solver_compute_sigmas_with_images(es_solver* s, double B, const double* T
     if ((rc = ensure(c, s->img_aux, aux.size() * sizeof(double), "image taus"))) return rc;
     ES_HIP(c, hipMemcpyAsync(s->img_aux.p, aux.data(), aux.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     ES_HIP(c, hipMemsetAsync(s->img.p, 0, bytes, c.stream));
-    ES_HIP(c, es::launch_images(P<es::SampleDev>(s->s.samples), P<es::PairDev>(s->s.pairs),
+    ES_HIP(c, es::launch_images(d_samples(s->s), d_pairs(s->s),
                                 s->st.is_clockwise ? 1 : 0, P<double>(s->s.lt_item), P<double>(s->img_aux),
                                 P<double>(s->img_aux) + nT, nT, s->f.nfree, ppr, P<double>(s->img), c.stream));
     ES_HIP(c, hipMemcpyAsync(images, s->img.p, bytes, hipMemcpyDeviceToHost, c.stream));
@@ -1220,6 +1255,7 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
         if ((rc = batch_plan(bt))) return rc;
         if ((rc = stages_prepare(c, bt->s, bt->plan))) return rc;
     }
+    if ((rc = stages_reset_queue(c, bt->s))) return rc;
     std::vector<int> nblocks;
     int nblocks64 = 0;
     if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks, &nblocks64))) return rc;

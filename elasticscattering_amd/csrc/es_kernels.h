@@ -100,26 +100,25 @@ struct SigmaArgs {
     int with_metrics;
     double* dpart;                   // [block][2*NT+1][2] double-double partials
     int64_t* mpart;                  // [block][kMetricSlots]
+    // Cleared by the stage for the next trace (it runs after this run's trace).
+    unsigned long long* queue;
+    int32_t* list_count;             // or null
 };
-
-hipError_t launch_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y, hipStream_t st);
 
 // Binning (Grid.cpp:77-180) into cell-ordered (x, y) + inclusive ends.
 struct BinScratch {
     int32_t* targets;   // 9 per impurity
     uint8_t* ntarg;     // per impurity
-    int32_t* counts;    // per cell
+    int32_t* counts;    // per cell; zero on entry, left zero
     int32_t* fill;      // per cell
     int64_t* begin;     // per cell + 1
     int64_t* bsum;      // per scan block
     int32_t* order;     // per indexed impurity
 };
-hipError_t launch_bin_count(const double* x, const double* y, int64_t n, const GridDev& g, BinScratch& s,
-                            hipStream_t st);
-hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st);
-// The indexed count is read on the device (total_dev, at most max_total).
-hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s,
-                            const int64_t* total_dev, int64_t max_total, hipStream_t st);
+// Generates the n centres into x, y first when `generate` (es_rng.h, key, range
+// [lo, hi)); writes the indexed count to *total_out (device memory).
+hipError_t launch_bin(double* x, double* y, int64_t n, int generate, uint64_t key, double lo, double hi, GridDev& g,
+                      BinScratch& s, int64_t* total_out, hipStream_t st);
 
 // Start-cell test per position (lifetime.h:84-99) and ordered compaction.
 // bcount: one int per 256 positions.
@@ -131,7 +130,10 @@ hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx,
 // for wave-cooperative and screened (disks that cannot lower the minimum are
 // not solved).  The FP32 trace has coop 0 and 1 (2 counts as 1).
 hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* blocks);
-// Resets the queue (a kernel, so the sequence can be captured in a graph) and traces.
+// Traces; the work queue must be zero (the temperature stage clears it after
+// every run).  In list mode (the FP64 recompute of the mixed path, after the
+// FP32 trace has used the queue) the queue is reset by a kernel first, so the
+// sequence can be captured in a graph.
 hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st);
 // Mixed precision: FP32 trace of every item, then FP64 trace of the flagged ones.
 hipError_t trace32_blocks_per_cu(int clockwise, int coop, int* blocks);

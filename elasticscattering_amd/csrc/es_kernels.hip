@@ -1,14 +1,16 @@
 // es_kernels.hip — gfx950 kernels of the lifetime / sigma path.
 //
-//   k_generate      impurity centres from es_rng.h (Grid.cpp:61-75 analogue)
-//   k_bin_count     per impurity: target cells (Grid.cpp:112-180), per-cell counts
-//   k_scan_block /
-//   k_scan_add      two-level exclusive scan -> cell begin offsets + the
-//                   reference's inclusive ends
+//   k_bin_count     per impurity (drawn first from es_rng.h when generating,
+//                   the Grid.cpp:61-75 analogue): target cells (Grid.cpp:
+//                   112-180), per-cell counts
+//   k_scan_one /
+//   k_scan_block,
+//   k_scan_add      exclusive scan -> cell begin offsets + the reference's
+//                   inclusive ends (one block up to 32,768 cells)
 //   k_bin_fill      scatter impurity ids into their cells
-//   k_bin_sort      ascending impurity id inside each cell (the reference's
-//                   order), one wave per cell
-//   k_bin_gather    cell-ordered (x, y) pairs
+//   k_bin_sort_gather  ascending impurity id inside each cell (the
+//                   reference's order) and the cell-ordered (x, y) pairs, one
+//                   wave per cell
 //   k_positions     start cell + "inside an impurity" test per position
 //                   (lifetime.h:84-99), hoisted out of the phi and B loops
 //   k_compact       ordered list of free positions (deterministic)
@@ -112,17 +114,6 @@ __device__ __forceinline__ void dd_add_exact_product(DD& a, double x, double y)
     dd_add_d(a, lo);
 }
 
-// ------------------------------------------------------------------ generate
-
-__global__ void k_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double a, b;
-    esr_impurity(key, (uint64_t)i, lo, hi, &a, &b);
-    x[i] = a;
-    y[i] = b;
-}
 
 // ------------------------------------------------------------------- binning
 
@@ -182,13 +173,24 @@ __device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
     return n;
 }
 
-__global__ void k_bin_count(const double* x, const double* y, int64_t n, BinGeom g, int32_t* targets,
-                            uint8_t* ntarg, int32_t* counts)
+// Targets and per-cell counts of every impurity; with `generate` the centres
+// are drawn first (es_rng.h) and stored in x, y.  counts are zero on entry.
+__global__ void k_bin_count(double* x, double* y, int64_t n, BinGeom g, int generate, uint64_t key, double lo,
+                            double hi, int32_t* targets, uint8_t* ntarg, int32_t* counts)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    double px, py;
+    if (generate) {
+        esr_impurity(key, (uint64_t)i, lo, hi, &px, &py);
+        x[i] = px;
+        y[i] = py;
+    } else {
+        px = x[i];
+        py = y[i];
+    }
     int32_t t[9];
-    const int k = bin_targets(g, x[i], y[i], t);
+    const int k = bin_targets(g, px, py, t);
     ntarg[i] = (uint8_t)k;
     for (int m = 0; m < k; m++) {
         targets[9 * i + m] = t[m];
@@ -241,17 +243,74 @@ __global__ __launch_bounds__(256) void k_scan_block(const int32_t* counts, int64
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_add(const int32_t* counts, int64_t n, int64_t* begin,
-                                                  const int64_t* bsum, int32_t* incl_end)
+// Second level of the two-level scan; also clears the fill counters for
+// k_bin_fill and the counts for the next binning, and writes the total.
+__global__ __launch_bounds__(256) void k_scan_add(int32_t* counts, int64_t n, int64_t* begin, const int64_t* bsum,
+                                                  int32_t* incl_end, int32_t* fill, int64_t* total_out)
 {
     __shared__ int64_t sh[256];
     const int64_t off = block_prefix_of(bsum, blockIdx.x, sh);
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c < n) {
         const int64_t b = begin[c] + off;
+        const int32_t cnt = counts[c];
         begin[c] = b;
-        incl_end[c] = (int32_t)(b + counts[c]);
-        if (c == n - 1) begin[n] = b + counts[c];
+        incl_end[c] = (int32_t)(b + cnt);
+        fill[c] = 0;
+        counts[c] = 0;
+        if (c == n - 1) {
+            begin[n] = b + cnt;
+            *total_out = b + cnt;
+        }
+    }
+}
+
+// The same scan in one block of 1024 threads for cell grids up to
+// kScanOneCells cells, staged in LDS: coalesced loads of the counts, each
+// thread sums a contiguous run, the block scans the run sums, the runs are
+// turned into exclusive offsets in place, and coalesced stores write them out.
+constexpr int kScanOneCells = 32768;
+
+__global__ __launch_bounds__(1024) void k_scan_one(int32_t* counts, int64_t n, int64_t* begin, int32_t* incl_end,
+                                                   int32_t* fill, int64_t* total_out)
+{
+    extern __shared__ int32_t sh_cnt[];  // [n]
+    __shared__ int64_t sh[1024];
+    const int tid = threadIdx.x;
+    const int ni = (int)n;
+    for (int c = tid; c < ni; c += 1024) {
+        sh_cnt[c] = counts[c];
+        counts[c] = 0;
+        fill[c] = 0;
+    }
+    __syncthreads();
+    const int per = (ni + 1023) / 1024;
+    const int c0 = tid * per < ni ? tid * per : ni, c1 = (c0 + per < ni) ? c0 + per : ni;
+    int64_t run = 0;
+    for (int c = c0; c < c1; c++) run += sh_cnt[c];
+    sh[tid] = run;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t add = (tid >= off) ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    const int64_t total = sh[1023];
+    int64_t b = sh[tid] - run;
+    for (int c = c0; c < c1; c++) {
+        const int32_t cnt = sh_cnt[c];
+        sh_cnt[c] = (int32_t)b;  // exclusive offset (the total fits in int32: checked by the host)
+        b += cnt;
+    }
+    __syncthreads();
+    for (int c = tid; c < ni; c += 1024) {
+        begin[c] = sh_cnt[c];
+        incl_end[c] = (c + 1 < ni) ? sh_cnt[c + 1] : (int32_t)total;
+    }
+    if (tid == 0) {
+        begin[n] = total;
+        *total_out = total;
     }
 }
 
@@ -282,17 +341,21 @@ __device__ void sift_down(int32_t* a, int64_t root, int64_t n)
     }
 }
 
-// Ascending impurity ids inside each cell: the reference pushes impurities into
-// a cell in index order (Grid.cpp:87-89,129).  One wave per cell; ids in a
-// cell are distinct, so a lane's rank is the count of smaller ids.
-__global__ __launch_bounds__(256) void k_bin_sort(int64_t ncell, const int64_t* begin, int32_t* order)
+// Ascending impurity ids inside each cell — the reference pushes impurities
+// into a cell in index order (Grid.cpp:87-89,129) — and the cell's (x, y)
+// pairs in that order.  One wave per cell; ids in a cell are distinct, so a
+// lane's rank is the count of smaller ids (a heap sort by one lane for cells
+// of more than 64).
+__global__ __launch_bounds__(256) void k_bin_sort_gather(int64_t ncell, const int64_t* begin, int32_t* order,
+                                                         const double* x, const double* y, double2* ixy)
 {
     const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= ncell) return;
-    int32_t* a = order + begin[c];
-    const int64_t n = begin[c + 1] - begin[c];
-    if (n <= 1) return;
+    const int64_t b = begin[c];
+    int32_t* a = order + b;
+    const int64_t n = begin[c + 1] - b;
+    if (n <= 0) return;
     if (n <= 64) {
         const int32_t v = (lane < n) ? a[lane] : INT32_MAX;
         int rank = 0;
@@ -300,8 +363,13 @@ __global__ __launch_bounds__(256) void k_bin_sort(int64_t ncell, const int64_t* 
             const int32_t o = __shfl(v, k, 64);
             rank += (o < v) ? 1 : 0;
         }
-        if (lane < n) a[rank] = v;
-    } else if (lane == 0) {
+        if (lane < n) {
+            a[rank] = v;
+            ixy[b + rank] = make_double2(x[v], y[v]);
+        }
+        return;
+    }
+    if (lane == 0) {
         for (int64_t r = n / 2 - 1; r >= 0; r--) sift_down(a, r, n);
         for (int64_t e = n - 1; e > 0; e--) {
             const int32_t t = a[0];
@@ -310,16 +378,13 @@ __global__ __launch_bounds__(256) void k_bin_sort(int64_t ncell, const int64_t* 
             sift_down(a, 0, e);
         }
     }
-}
-
-// Grid over the largest possible count; the actual one is read on the device.
-__global__ void k_bin_gather(const int64_t* total, const int32_t* order, const double* x, const double* y,
-                             double2* ixy)
-{
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *total) return;
-    const int32_t i = order[k];
-    ixy[k] = make_double2(x[i], y[i]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int64_t k = lane; k < n; k += 64) {
+        const int32_t v = a[k];
+        ixy[b + k] = make_double2(x[v], y[v]);
+    }
 }
 
 // ----------------------------------------------------------------- positions
@@ -1446,6 +1511,10 @@ template <int NT>
 __global__ __launch_bounds__(kBlock) void k_sigma(SigmaArgs a)
 {
     constexpr int NQ = 2 * NT + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // ready for the next run's trace
+        *a.queue = 0ull;
+        if (a.list_count) *a.list_count = 0;
+    }
     const int pi = pair_of_block(a.pairs, a.npairs, blockIdx.x);
     const PairDev& pr = a.pairs[pi];
     const SampleDev& sd = a.samples[pr.sample];
@@ -1578,11 +1647,6 @@ __global__ __launch_bounds__(256) void k_images(const SampleDev* sample, const P
     }
 }
 
-__global__ void k_reset2(unsigned long long* q, int32_t* count)
-{
-    *q = 0ull;
-    *count = 0;
-}
 
 // ------------------------------------------------------------------ finalize
 
@@ -1660,56 +1724,44 @@ inline unsigned int blocks_for(int64_t n, int bs) { return (unsigned int)((n + b
 
 }  // namespace
 
-hipError_t launch_generate(uint64_t key, int64_t n, double lo, double hi, double* x, double* y, hipStream_t st)
-{
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, st, key, n, lo, hi, x, y);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_count(const double* x, const double* y, int64_t n, const GridDev& g, BinScratch& s,
-                            hipStream_t st)
+hipError_t launch_bin(double* x, double* y, int64_t n, int generate, uint64_t key, double lo, double hi, GridDev& g,
+                      BinScratch& s, int64_t* total_out, hipStream_t st)
 {
     const int64_t ncell = (int64_t)g.cpr * g.cpr;
-    hipError_t e = hipMemsetAsync(s.counts, 0, ncell * sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
-    if (n <= 0) return hipSuccess;
-    BinGeom bg{g.lo, g.size, g.cell_size, g.radius, g.cpr};
-    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, y, n, bg, s.targets, s.ntarg,
-                       s.counts);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_scan(const GridDev& g, BinScratch& s, hipStream_t st)
-{
-    const int64_t ncell = (int64_t)g.cpr * g.cpr;
-    const unsigned int nb = blocks_for(ncell, 256);
-    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum, g.cell_end);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_place(const double* x, const double* y, int64_t n, GridDev& g, BinScratch& s,
-                            const int64_t* total_dev, int64_t max_total, hipStream_t st)
-{
-    const int64_t ncell = (int64_t)g.cpr * g.cpr;
-    hipError_t e = hipMemsetAsync(s.fill, 0, ncell * sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if (n > 0) {
+        BinGeom bg{g.lo, g.size, g.cell_size, g.radius, g.cpr};
+        hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, y, n, bg, generate, key, lo,
+                           hi, s.targets, s.ntarg, s.counts);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (ncell <= kScanOneCells) {
+        static bool lds_set = false;  // allow the LDS staging beyond the 64 KB default (once per process)
+        if (!lds_set) {
+            if ((e = hipFuncSetAttribute((const void*)k_scan_one, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         kScanOneCells * (int)sizeof(int32_t))) != hipSuccess)
+                return e;
+            lds_set = true;
+        }
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), (size_t)ncell * sizeof(int32_t), st, s.counts, ncell,
+                           s.begin, g.cell_end, s.fill, total_out);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else {
+        const unsigned int nb = blocks_for(ncell, 256);
+        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum, g.cell_end,
+                           s.fill, total_out);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, s.targets, s.ntarg, s.begin,
                            s.fill, s.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(ncell, 4)), dim3(256), 0, st, ncell, s.begin, s.order);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (max_total > 0) {
-        hipLaunchKernelGGL(k_bin_gather, dim3(blocks_for(max_total, 256)), dim3(256), 0, st, total_dev, s.order, x, y,
-                           g.ixy);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    hipLaunchKernelGGL(k_bin_sort_gather, dim3(blocks_for(ncell, 4)), dim3(256), 0, st, ncell, s.begin, s.order, x, y,
+                       g.ixy);
+    return hipGetLastError();
 }
 
 hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
@@ -1754,9 +1806,13 @@ hipError_t trace_blocks_per_cu(int clockwise, int waves_per_eu, int coop, int* b
 hipError_t launch_trace(const TraceArgs& a, int clockwise, int waves_per_eu, int coop, int nblocks, hipStream_t st)
 {
     if (a.npairs <= 0 || nblocks <= 0 || a.chunk <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || (!a.list_mode && a.total_items <= 0)) return e;
+    if (a.list_mode) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, a.queue);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    } else if (a.total_items <= 0) {
+        return hipSuccess;
+    }
     TraceArgs arg = a;
     void* params[] = {&arg};
     return hipLaunchKernel(trace_kernel(clockwise, waves_per_eu, coop), dim3(nblocks), dim3(kBlock), params, 0, st);
@@ -1788,9 +1844,8 @@ hipError_t launch_trace_mixed(const TraceArgs& a, int clockwise, int waves_per_e
 {
     if (a.npairs <= 0 || nblocks32 <= 0 || nblocks64 <= 0 || a.chunk <= 0 || !a.list || !a.list_count)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset2, dim3(1), dim3(1), 0, st, a.queue, a.list_count);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || a.total_items <= 0) return e;
+    hipError_t e = hipSuccess;
+    if (a.total_items <= 0) return e;
     TraceArgs a32 = a;
     a32.list_mode = 0;
     void* params[] = {&a32};
