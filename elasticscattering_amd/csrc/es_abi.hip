@@ -20,6 +20,7 @@
 
 #include "../../include/es_abi.h"
 #include "es_kernels.h"
+#include "es_math.h"
 #include "es_rng.h"
 
 namespace {
@@ -614,7 +615,13 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     const size_t bf = (size_t)pl.nout * es::kMaxT * sizeof(double);
     s.t_pairs = align_up(bs);
     s.t_factor = s.t_pairs + align_up(bp);
-    const size_t tables = s.t_factor + bf;
+    // then each sample's phi table: (sin, cos) of its 4 ppq start angles
+    std::vector<size_t> t_phi(pl.samples.size());
+    size_t tables = align_up(s.t_factor + bf);
+    for (size_t i = 0; i < pl.samples.size(); i++) {
+        t_phi[i] = tables;
+        tables += align_up((size_t)4 * pl.samples[i].ppq * sizeof(double2));
+    }
     const size_t bsig = (size_t)pl.nout * es::kMaxT * 2 * sizeof(double), blt = (size_t)pl.nout * sizeof(double);
     s.o_ltsum = align_up(bsig);
     s.o_mout = s.o_ltsum + align_up(blt);
@@ -647,6 +654,19 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     std::memcpy(h, pl.samples.data(), bs);
     std::memcpy(h + s.t_pairs, pl.pairs.data(), bp);
     std::memcpy(h + s.t_factor, pl.factor.data(), bf);
+    es::SampleDev* hs = reinterpret_cast<es::SampleDev*>(h);
+    for (size_t i = 0; i < pl.samples.size(); i++) {
+        const es::SampleDev& sd = pl.samples[i];
+        hs[i].phi_sc = reinterpret_cast<const double2*>(P<char>(s.tables) + t_phi[i]);
+        double2* tab = reinterpret_cast<double2*>(h + t_phi[i]);
+        for (int q = 0; q < 4; q++)
+            for (int p = 0; p < sd.ppq; p++) {
+                const double phi = sd.phi_start + q * kHALF_PI + p * sd.phi_step;  // lifetime.h:36-58
+                double sn, cs;
+                esm_sincos(phi, &sn, &cs);
+                tab[q * sd.ppq + p] = make_double2(sn, cs);
+            }
+    }
     ES_HIP(c, hipMemcpyAsync(s.tables.p, h, tables, hipMemcpyHostToDevice, c.stream));
     return ES_OK;
 }

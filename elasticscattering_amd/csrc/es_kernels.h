@@ -46,6 +46,9 @@ struct SampleDev {
     double phi_integrand_factor;     // Simulation.h:116-120
     double inv_L, inv_size, inv_speed;  // RN(1 / cell size), RN(1 / size), RN(1 / speed)
     double cell_population;          // indexed impurities per cell (host: picks the trace variant)
+    // (sin, cos) of phi = phi_start + q pi/2 + p phi_step for k = q ppq + p,
+    // k < 4 ppq (esm_sincos on the host: the device's bits).
+    const double2* phi_sc;
     int coherent;
     int nT;
     double tau[kMaxT];

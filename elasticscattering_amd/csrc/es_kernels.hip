@@ -547,8 +547,8 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, const PairHint&
     const double py = (double)j * sd.dist + sd.offy;
     const double w = pr.w;
     const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
-    double sphi, cphi;
-    esm_sincos(phi, &sphi, &cphi);
+    const double2 sc = sd.phi_sc[q * sd.ppq + p];
+    const double sphi = sc.x, cphi = sc.y;
     const double vx = cphi * sd.speed, vy = sphi * sd.speed;
     const double R = pr.R;  // speed / w
     const double sx = esm_div_rcp(vy * R, sd.speed, sd.inv_speed);
@@ -1049,8 +1049,8 @@ __device__ __forceinline__ void t32_setup_item(const TraceArgs& a, const PairHin
     const double py = (double)j * sd.dist + sd.offy;
     w = pr.w;
     const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;  // lifetime.h:36-58
-    double sphi, cphi;
-    esm_sincos(phi, &sphi, &cphi);
+    const double2 sc = sd.phi_sc[q * sd.ppq + p];
+    const double sphi = sc.x, cphi = sc.y;
     const double vx = cphi * sd.speed, vy = sphi * sd.speed;
     const double R = pr.R;
     const double sx = esm_div_rcp(vy * R, sd.speed, sd.inv_speed);
@@ -1469,8 +1469,8 @@ __device__ __forceinline__ void sigma_unit(const SigmaArgs& a, const PairDev& pr
     // lifetime contributes exactly +-0 (util.h:34-46), so it is skipped.
     if (lt_raw == 0.0) return;  // every temperature's term is +-0
     const double phi = sd.phi_start + q * ES_HALF_PI + p0 * sd.phi_step;
-    double sphi, cphi;
-    esm_sincos(phi, &sphi, &cphi);
+    const double2 sc = sd.phi_sc[q * ppq + p0];
+    const double sphi = sc.x, cphi = sc.y;
     const double wji = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr);
     // sin, cos of the end angle for the untruncated lifetime, shared by every
     // temperature whose cap does not cut it (the same argument, so the same bits).
@@ -1628,8 +1628,9 @@ __global__ __launch_bounds__(256) void k_images(const SampleDev* sample, const P
                 lt_tot += lt * w;
                 if (lt != 0.0) {
                     const double phi = sd.phi_start + q * ES_HALF_PI + p * sd.phi_step;
-                    double sphi, cphi, sa, ca;
-                    esm_sincos(phi, &sphi, &cphi);
+                    const double2 sc = sd.phi_sc[q * ppq + p];
+                    const double sphi = sc.x, cphi = sc.y;
+                    double sa, ca;
                     const double z = esm_exp(-lt / tt);
                     esm_sincos(phi + ws * lt, &sa, &ca);
                     double r = cphi - ca * z;
