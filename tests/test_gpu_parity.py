@@ -68,8 +68,10 @@ def test_binning_file_fields_bit_exact(es, oracle_esm, name, rs, ext, r, target,
         _gpu_grid_matches(sim, info, og)
 
 
+# cell grids: 1, 32^2, 128^2 (one-block scan), 181^2 = 32,761 (its largest grid, 128 KB of
+# LDS), 200^2 (two-level scan) and the reference rule (cpr 0)
 @pytest.mark.parametrize("seed,n,r,cpr", [(101, 2000, 1.1e-8, 32), (7, 60000, 1.1e-8, 128), (3, 400, 2e-8, 0),
-                                          (11, 5000, 5e-9, 1)])
+                                          (11, 5000, 5e-9, 1), (13, 120000, 6e-9, 181), (17, 90000, 5e-9, 200)])
 def test_generated_fields_bit_exact(es, oracle_esm, seed, n, r, cpr):
     announce(f"generate+bin seed={seed} n={n} r={r} cpr={cpr}")
     rs, ext = 1e-6, 5e-7
