@@ -156,6 +156,12 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     const double lrx = llx + L, lry = lly + 0.0;
     const double trx = llx + L, try_ = lly + L;
     const double tlx = llx + 0.0, tly = lly + L;
+    // The edge directions (p2 - p1) / L of cell_grid.h:98-99: the horizontal
+    // edges share trx - tlx = lrx - llx = (llx + L) - llx, the vertical ones
+    // try - lry = tly - lly = (lly + L) - lly, and the other components are
+    // exactly 0 (0 / L = +0); two quotients per cell.
+    const double qx = esm_div_rcp(trx - tlx, L, g.inv_L);
+    const double qy = esm_div_rcp(try_ - lry, L, g.inv_L);
 
     // Edges in the reference order: top (0,+1), bottom (0,-1), right (+1,0), left (-1,0).
 #pragma unroll 1
@@ -167,7 +173,7 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
         const int ncx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
         const int ncy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
 
-        const double ux = esm_div_rcp(p2x - p1x, L, g.inv_L), uy = esm_div_rcp(p2y - p1y, L, g.inv_L);
+        const double ux = (e < 2) ? qx : 0.0, uy = (e < 2) ? 0.0 : qy;
         const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
         const double projx = p1x + ux * pd, projy = p1y + uy * pd;
         const double dx = projx - o.cx, dy = projy - o.cy;
