@@ -328,6 +328,9 @@ def sweep_comparison(n_materials, train_runs):
                         "curves_with_jitter_above_half_tolerance": int((jitter > 5e-4).sum())}
     for name, sg, prec in (("adaptive_mixed_surrogate", sur, "mixed"), ("adaptive_mixed", None, "mixed"),
                            ("adaptive_fp64", None, "fp64")):
+        # one untimed run first: the first use of a precision loads its kernels
+        # and sizes the batch buffers, which would land on whichever variant ran first
+        S.adaptive_sweep(mats, B, surrogate=sg, precision=prec)
         t0 = time.perf_counter()
         est, rep = S.adaptive_sweep(mats, B, surrogate=sg, precision=prec)
         wall = time.perf_counter() - t0

@@ -171,6 +171,7 @@ _SIGNATURES = {
     "es_batch_destroy": (None, [_P]),
     "es_batch_last_error": (C.c_char_p, [_P]),
     "es_batch_add_sample": (C.c_int, [_P, C.POINTER(BatchSample), C.POINTER(C.c_int32)]),
+    "es_batch_clear": (C.c_int, [_P]),
     "es_batch_set_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_get_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
@@ -427,6 +428,12 @@ class Batch:
         self._check(self.lib.es_batch_add_sample(self.h, C.byref(s), C.byref(idx)))
         self.entries.append((len(Bs), len(T)))
         return idx.value
+
+    def clear(self):
+        """Removes every entry; keeps the drawn fields for later entries with the same recipe
+        (seed, count, grid description), which then skip drawing and binning."""
+        self._check(self.lib.es_batch_clear(self.h))
+        self.entries = []
 
     def counts(self):
         ns, npr, no = C.c_int32(), C.c_int32(), C.c_int32()

@@ -912,7 +912,15 @@ struct es_batch {
         PassSettings ps;
         std::vector<double> B, T, taus;
     };
+    // What drew each field: es_batch_clear keeps the fields, and a sample added
+    // later with the same recipe takes one that no current entry uses.
+    struct Recipe {
+        uint64_t seed;
+        int64_t count;
+        es_grid_desc grid;
+    };
     std::vector<Field*> fields;
+    std::vector<Recipe> recipes;     // per field
     std::vector<Entry> entries;
     Stages s;
     Shape shape;
@@ -1211,6 +1219,15 @@ int es_solver_compute_sigmas_with_images(es_solver* s, double B, const double* T
 
 namespace {
 
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+bool same_grid_desc(const es_grid_desc& a, const es_grid_desc& b)
+{
+    return same_bits(a.region_size, b.region_size) && same_bits(a.region_extends, b.region_extends) &&
+           same_bits(a.impurity_radius, b.impurity_radius) && same_bits(a.impurity_density, b.impurity_density) &&
+           a.target_cell_population == b.target_cell_population && a.cells_per_row == b.cells_per_row;
+}
+
 int batch_plan(es_batch* bt)
 {
     Ctx& c = bt->c;
@@ -1393,16 +1410,27 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
             return fail(c, ES_ERR_INVALID_ARG, "share_field_of names no earlier sample");
         en.field = bt->entries[smp->share_field_of].field;
     } else {
-        Field* f = new Field();
-        es_grid_info info{};
-        rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, &info);
-        if (rc) {
-            field_release(c, *f);
-            delete f;
-            return rc;
+        en.field = -1;
+        for (size_t i = 0; i < bt->fields.size() && en.field < 0; i++) {
+            const es_batch::Recipe& r = bt->recipes[i];
+            if (r.seed != smp->seed || r.count != smp->impurity_count || !same_grid_desc(r.grid, smp->grid)) continue;
+            bool used = false;
+            for (const es_batch::Entry& other : bt->entries) used = used || other.field == (int)i;
+            if (!used) en.field = (int)i;
         }
-        bt->fields.push_back(f);
-        en.field = (int)bt->fields.size() - 1;
+        if (en.field < 0) {
+            Field* f = new Field();
+            es_grid_info info{};
+            rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, &info);
+            if (rc) {
+                field_release(c, *f);
+                delete f;
+                return rc;
+            }
+            bt->fields.push_back(f);
+            bt->recipes.push_back({smp->seed, smp->impurity_count, smp->grid});
+            en.field = (int)bt->fields.size() - 1;
+        }
     }
     Field& f = *bt->fields[en.field];
     const double cell = smp->grid_cell_size > 0 ? smp->grid_cell_size : f.g.cell_size;
@@ -1422,6 +1450,15 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
     bt->entries.push_back(en);
     bt->plan_ready = false;
     if (index) *index = (int32_t)bt->entries.size() - 1;
+    return ES_OK;
+}
+
+int es_batch_clear(es_batch* bt)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    bt->entries.clear();
+    bt->plan_ready = false;
+    batch_release_graph(bt);
     return ES_OK;
 }
 

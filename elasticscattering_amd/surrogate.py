@@ -179,26 +179,42 @@ def draw_materials(n, seed, alpha=None):
 
 
 def solver_runs(materials, fields, temperatures=None, positions_per_row=128, particles_per_quadrant=65,
-                side=2.0e-6, clockwise=1, speed=7e5, device=0, with_free=False, precision="fp64"):
+                side=2.0e-6, clockwise=1, speed=7e5, device=0, with_free=False, precision="fp64", reuse=None):
     """sigma [n_mat][n_B][n_T][2] = coherent + incoherent pass, all materials in one batch.
 
     ``fields[i]`` / ``temperatures[i]`` are material i's B values and
     temperatures (default: its own T).  Returns (sigma list, device seconds,
     traced work items[, start positions outside every impurity per material]).
+    With a dict ``reuse``, the batch is kept in it and cleared for the next
+    call (its drawn fields and buffers stay; the caller closes
+    ``reuse["batch"]``); otherwise a batch is made and closed per call.
     """
     rs, ext = side / 2.0, side / 4.0
-    with Batch(device) as b:
-        b.set_precision(precision)
-        for i, m in enumerate(materials):
-            Ts = [m["T"]] if temperatures is None else list(temperatures[i])
-            kw = dict(seed=m["seed"], region_size=rs, region_extends=ext, impurity_radius=m["radius"],
-                      density=m["density"], target_cell_population=4, positions_per_row=positions_per_row,
-                      particles_per_quadrant=particles_per_quadrant, fields=list(fields[i]), temperatures=Ts,
-                      tau=m["tau"], alpha=m["alpha"], is_clockwise=clockwise, particle_speed=speed)
-            j = b.add(coherent=True, **kw)
-            b.add(coherent=False, share_field_of=j, **kw)
-        items = b.counts()["traced_items"]
-        sig, mets, secs = b.run(metrics=with_free)
+    if reuse is None:
+        with Batch(device) as b:
+            return _solver_batch(b, materials, fields, temperatures, positions_per_row, particles_per_quadrant,
+                                 rs, ext, clockwise, speed, with_free, precision)
+    b = reuse.get("batch")
+    if b is None:
+        b = reuse["batch"] = Batch(device)
+    b.clear()
+    return _solver_batch(b, materials, fields, temperatures, positions_per_row, particles_per_quadrant, rs, ext,
+                         clockwise, speed, with_free, precision)
+
+
+def _solver_batch(b, materials, fields, temperatures, positions_per_row, particles_per_quadrant, rs, ext, clockwise,
+                  speed, with_free, precision):
+    b.set_precision(precision)
+    for i, m in enumerate(materials):
+        Ts = [m["T"]] if temperatures is None else list(temperatures[i])
+        kw = dict(seed=m["seed"], region_size=rs, region_extends=ext, impurity_radius=m["radius"],
+                  density=m["density"], target_cell_population=4, positions_per_row=positions_per_row,
+                  particles_per_quadrant=particles_per_quadrant, fields=list(fields[i]), temperatures=Ts,
+                  tau=m["tau"], alpha=m["alpha"], is_clockwise=clockwise, particle_speed=speed)
+        j = b.add(coherent=True, **kw)
+        b.add(coherent=False, share_field_of=j, **kw)
+    items = b.counts()["traced_items"]
+    sig, mets, secs = b.run(metrics=with_free)
     out = [sig[2 * i] + sig[2 * i + 1] for i in range(len(materials))]
     if with_free:
         free = [mets[2 * i + 1][0]["real_particles"] // (4 * particles_per_quadrant) for i in range(len(materials))]
@@ -268,18 +284,68 @@ def tensor_error(approx, exact):
 
 # ---------------------------------------------------------- adaptive sweep
 
-def _curve_from(u, idx, vals):
-    """PCHIP through the solved points in (log B, asinh(sigma / S0))."""
-    from scipy.interpolate import PchipInterpolator
+def _pchip_end(h0, h1, m0, m1):
+    """End slope: the three-point formula, set to 0 against the secant's sign
+    and capped at 3 m0 where the secants change sign."""
+    d = ((2 * h0 + h1) * m0 - h0 * m1) / (h0 + h1)
+    flip = np.sign(d) != np.sign(m0)
+    cap = (np.sign(m0) != np.sign(m1)) & (np.abs(d) > 3 * np.abs(m0))
+    return np.where(flip, 0.0, np.where(cap, 3 * m0, d))
 
+
+def pchip(x, y, xe):
+    """Piecewise cubic Hermite interpolation with Fritsch-Butland slopes, the
+    method of scipy.interpolate.PchipInterpolator: at each interior knot the
+    weighted harmonic mean of the two secants (0 where they differ in sign or
+    one is 0), at the ends the three-point rule of _pchip_end; cubics
+    extended past the ends.  x [k] ascending (k >= 3), y [k][...], xe [n]."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    sh = (-1,) + (1,) * (y.ndim - 1)
+    h = np.diff(x).reshape(sh)
+    m = np.diff(y, axis=0) / h
+    d = np.empty_like(y)
+    w1 = 2 * h[1:] + h[:-1]
+    w2 = h[1:] + 2 * h[:-1]
+    flat = (np.sign(m[1:]) != np.sign(m[:-1])) | (m[1:] == 0) | (m[:-1] == 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hm = (w1 / m[:-1] + w2 / m[1:]) / (w1 + w2)
+        d[1:-1] = np.where(flat, 0.0, 1.0 / hm)
+    d[0] = _pchip_end(h[0], h[1], m[0], m[1])
+    d[-1] = _pchip_end(h[-1], h[-2], m[-1], m[-2])
+    i = np.clip(np.searchsorted(x, xe, side="right") - 1, 0, len(x) - 2)
+    t = (np.asarray(xe, dtype=np.float64) - x[i]).reshape(sh)
+    hi, mi, d0, d1 = h[i], m[i], d[i], d[i + 1]
+    c2 = (3 * mi - 2 * d0 - d1) / hi
+    c3 = (d0 + d1 - 2 * mi) / (hi * hi)
+    return y[i] + t * (d0 + t * (c2 + t * c3))
+
+
+def _curve_from(u, idx, vals, at=None):
+    """PCHIP through the solved points in (log B, asinh(sigma / S0)) at the
+    grid indices `at` (all by default); vals [len(idx)][...][2]."""
+    ue = u if at is None else u[at]
     y = targets(vals)
     if len(idx) == 1:
-        return np.repeat(vals[:1], len(u), axis=0)
+        return np.repeat(vals[:1], len(ue), axis=0)
     if len(idx) == 2:
-        est = np.stack([np.interp(u, u[idx], y[:, c]) for c in range(2)], axis=-1)
+        t = (ue - u[idx[0]]) / (u[idx[1]] - u[idx[0]])
+        t = np.clip(t, 0.0, 1.0).reshape((-1,) + (1,) * (y.ndim - 1))
+        est = y[0] + t * (y[1] - y[0])
     else:
-        est = PchipInterpolator(u[idx], y, axis=0)(u)
+        est = pchip(u[idx], y, ue)
     return from_targets(est)
+
+
+def _prior_errors(u, idx, prior, gaps):
+    """Per gap (a, b) of consecutive solved indices, the largest error over its
+    points and the ensemble members of interpolating each member's own curve
+    through the solved indices (one interpolation for all members and gaps)."""
+    at = np.concatenate([np.arange(a, b + 1) for a, b in gaps])
+    starts = np.cumsum([0] + [b - a + 1 for a, b in gaps[:-1]])
+    p = np.moveaxis(prior, 0, 1)  # [n][members][2]
+    err = tensor_error(_curve_from(u, idx, p[idx], at), p[at]).max(axis=1)
+    return np.maximum.reduceat(err, starts)
 
 
 class _Curve:
@@ -327,15 +393,28 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     B = np.asarray(B, dtype=np.float64)
     u = np.log(B)
     n = len(B)
-    curves = []
-    for m in materials:
-        prior = None
-        if surrogate is not None:
-            X = np.stack([B, np.full(n, m["T"]), np.full(n, m["density"]), np.full(n, m["radius"]),
-                          np.full(n, m["tau"]), np.full(n, m["alpha"])], axis=-1)
-            prior = from_targets(surrogate.predict_targets(X))
-        curves.append(_Curve(m, u, prior))
+    priors = [None] * len(materials)
+    if surrogate is not None and len(materials):
+        # every material's curve from every member, in one inference launch
+        X = np.concatenate([np.stack([B, np.full(n, m["T"]), np.full(n, m["density"]), np.full(n, m["radius"]),
+                                      np.full(n, m["tau"]), np.full(n, m["alpha"])], axis=-1) for m in materials])
+        P = from_targets(surrogate.predict_targets(X))  # [members][n_mat * n][2]
+        priors = [P[:, k * n:(k + 1) * n] for k in range(len(materials))]
+    curves = [_Curve(m, u, pr) for m, pr in zip(materials, priors)]
     init = [int(i) for i in np.unique(np.round(np.linspace(0, n - 1, max(2, n_initial))).astype(int))]
+    reuse = solver_kw.pop("reuse", None)
+    own = reuse is None
+    reuse = {} if own else reuse
+    try:
+        return _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_rounds,
+                             dict(solver_kw, reuse=reuse))
+    finally:
+        if own and reuse.get("batch") is not None:
+            reuse["batch"].close()
+
+
+def _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_rounds, solver_kw):
+    """The rounds of adaptive_sweep; every solver call shares one batch (solver_kw["reuse"])."""
     solver_s = 0.0
     items = 0
     rounds = 0
@@ -350,15 +429,13 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
             if c.phase == "init":
                 c.pending = [(i, None) for i in init]
             elif c.phase == "refine":
-                idx = np.array(sorted(c.solved))
-                apri = None
-                if c.prior is not None:
-                    apri = np.max([tensor_error(_curve_from(u, idx, p[idx]), p) for p in c.prior], axis=0)
-                for (a, b) in sorted(c.active):
-                    if apri is not None and apri[a:b + 1].max() <= kappa * tol:
-                        c.skipped += 1
-                        continue
-                    c.pending.append(((a + b) // 2, (a, b)))
+                gaps = sorted(c.active)
+                if c.prior is not None and gaps:
+                    idx = np.array(sorted(c.solved))
+                    keep = _prior_errors(u, idx, c.prior, gaps) > kappa * tol
+                    c.skipped += int(np.count_nonzero(~keep))
+                    gaps = [g for g, k in zip(gaps, keep) if k]
+                c.pending = [((a + b) // 2, (a, b)) for a, b in gaps]
                 c.active = set()
                 if not c.pending:
                     c.phase = "check"

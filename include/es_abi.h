@@ -214,6 +214,13 @@ void es_batch_destroy(es_batch* b);
 const char* es_batch_last_error(const es_batch* b);
 /* Draws and bins the entry's field on the GPU (unless shared) and runs its start-cell test. */
 int es_batch_add_sample(es_batch* b, const es_batch_sample* sample, int32_t* index);
+/*
+ * Removes every entry and keeps the drawn fields: a sample added afterwards
+ * with the same seed, impurity count and grid description takes such a field
+ * (not yet used by another entry) instead of drawing and binning it again.
+ * Fields are freed by es_batch_destroy.
+ */
+int es_batch_clear(es_batch* b);
 int es_batch_set_shape(es_batch* b, const es_batch_shape* shape);
 int es_batch_get_shape(es_batch* b, es_batch_shape* shape);
 /* enable != 0: record the stage launches once in a hipGraph and replay it. */

@@ -215,6 +215,30 @@ def test_tuner_shapes_give_same_results(es, batch_eager):
     b.set_shape(3, 128, 1, -1)
 
 
+def test_clear_reuses_fields_with_same_bits(es, batch_eager):
+    """After es_batch_clear, entries with a known recipe take the kept field
+    (its positions recomputed when they differ); results equal a fresh batch."""
+    entries, _, _ = batch_eager
+    announce("batch clear: kept fields against a fresh batch")
+    later = [dict(entries[0], fields=[4.0, 20.0]), dict(entries[3], positions_per_row=24, temperatures=[7.0]),
+             dict(entries[0], temperatures=[3.0]), dict(entries[4], seed=18)]
+    with es.Batch() as b:
+        for e in entries:
+            _add(b, e)
+        b.run()
+        b.clear()
+        for e in later:
+            _add(b, e)
+        got = b.run()
+        b.clear()
+        with pytest.raises(es.EsError):
+            b.run()  # empty again
+    with es.Batch() as fresh:
+        for e in later:
+            _add(fresh, e)
+        _same_results(fresh.run(), got)
+
+
 def test_batch_errors(es):
     announce("batch argument errors")
     with es.Batch() as b:

@@ -66,8 +66,12 @@ class _ExactSurrogate:
         self.mats = mats
 
     def predict_targets(self, X):
-        m = [m for m in self.mats if m["tau"] == X[0, 4]][0]
-        return S.targets(_drude(m, X[:, 0]))[None]
+        """Rows of any materials (matched by tau): [1 member][rows][2]."""
+        out = np.empty((len(X), 2))
+        for m in self.mats:
+            rows = X[:, 4] == m["tau"]
+            out[rows] = S.targets(_drude(m, X[rows, 0]))
+        return out[None]
 
 
 def test_exact_surrogate_saves_solver_points(fake_solver):
@@ -108,3 +112,29 @@ def test_tensor_error_conventions():
 def test_targets_round_trip():
     sig = np.array([[0.5, 1e-3], [0.0, 0.0], [1e-9, -1e-12]])
     np.testing.assert_allclose(S.from_targets(S.targets(sig)), sig, rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("kind", ["random", "monotone", "flat_steps", "smooth"])
+def test_pchip_matches_scipy(kind):
+    """The sweep's vectorized PCHIP against scipy.interpolate.PchipInterpolator
+    (same slopes and end rule): knots hit exactly, every point within 1e-12,
+    extrapolation past both ends included."""
+    from scipy.interpolate import PchipInterpolator
+
+    rng = np.random.default_rng(hash(kind) % 2**32)
+    for _ in range(200):
+        k = int(rng.integers(3, 40))
+        x = np.sort(rng.choice(400, k, replace=False)).astype(float) * 0.01
+        y = rng.normal(size=(k, 3, 2))
+        if kind == "monotone":
+            y = np.cumsum(np.abs(y), axis=0)
+        elif kind == "flat_steps":
+            y[rng.random(y.shape) < 0.3] = 0
+            y = np.round(y, 1)
+        elif kind == "smooth":
+            y = np.sin(x)[:, None, None] * np.ones((1, 3, 2))
+        xe = np.linspace(x[0] - 0.2, x[-1] + 0.2, 300)
+        ref = PchipInterpolator(x, y, axis=0)(xe)
+        got = S.pchip(x, y, xe)
+        assert np.max(np.abs(got - ref) / (1 + np.abs(ref))) <= 1e-12
+        assert np.allclose(S.pchip(x, y, x), y, rtol=0, atol=1e-14)
