@@ -232,7 +232,7 @@ def main():
         "roofline": {
             "bound": "mfma",
             "bound_detail": "FP64 vector ALU work, priced at MI355X's 78.6 TFLOP/s FP64 peak (vector and matrix FP64 "
-                            "peaks are equal); HBM traffic is ~1 % of peak (profiles/k_trace_production_fp64.md)",
+                            "peaks are equal); HBM traffic is ~2 % of peak (profiles/k_trace_production_fp64.md)",
             "kernel": "k_trace (CreateParticle + TraceOrbit, one launch per sample and pass, all fields)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
