@@ -86,7 +86,8 @@ def cpu_baseline(w, seconds_target=15.0):
         "cores": 1,
         "kind": "port",
         "sample": (f"CPU oracle (restated reference CPU path, glibc libm, -O2) on sample seed {w['seeds'][0]}, "
-                   f"incoherent pass, B in {fields[:done]} T, {len(w['temperatures'])} temperatures each: "
+                   f"incoherent pass, B in {[round(float(f), 6) for f in fields[:done]]} T, "
+                   f"{len(w['temperatures'])} temperatures each: "
                    f"{done * P} lifetimes + sigma stage in {dt:.1f} s"),
         "seconds": dt,
     }

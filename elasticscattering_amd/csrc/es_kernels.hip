@@ -692,13 +692,15 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 // solves which crossing disk, and which disks need solving at all.  Per cell
 // step of a wave:
 //   1. each lane runs CirclesCross (details.h:141-151) on its cell's disks,
-//      gives each crossing disk a pseudo-angle of its centre along the orbit
-//      (d_disk_proxy), keeps the earliest one and queues the others on the
-//      wave's stack in LDS;
-//   2. each lane solves its earliest disk (GetFirstCrossTime, details.h:153-206)
-//      and turns the time into a pseudo-angle threshold: a disk whose centre
-//      lies at or beyond it cannot cross the orbit before that time
-//      (d_prune_threshold);
+//      drops the crossing disks whose centre lies outside the window of the
+//      cell's angle range and gives the others a lower bound on the travel
+//      angle of their meeting points (d_range_window, d_disk_screen); it keeps
+//      the disk with the lowest bound and queues the others on the wave's
+//      stack in LDS;
+//   2. each lane solves its lowest-bound disk (GetFirstCrossTime,
+//      details.h:153-206) and turns the time into a travel-angle threshold: a
+//      disk whose bound reaches it cannot meet the orbit before that time
+//      (d_time_threshold);
 //   3. a pass over the stack drops those disks and compacts the rest;
 //   4. the rest are solved 64 at a time, any lane taking any lane's disk, with
 //      the owner's orbit and angle range read from LDS and the time folded
@@ -719,10 +721,10 @@ struct CoopWave {
     double orb[12][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry P0x P0y
     unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
     const double2* ixy[64];          // per lane: its sample's binned field
-    float thr[64];                   // per lane: pseudo-angle threshold of step 2
+    float thr[64];                   // per lane: travel-angle threshold of step 2
     unsigned int nexact[64];         // per lane: cooperative exact solves this cell
     int32_t qidx[kCoopCap];          // queued disk: index in its owner's field
-    float qpa[kCoopCap];             // queued disk: pseudo-angle
+    float qlb[kCoopCap];             // queued disk: lower bound of its travel angles
     uint8_t qown[kCoopCap];          // queued disk: owner lane
 };
 
@@ -758,40 +760,86 @@ __device__ __forceinline__ float pseudo_angle(float x, float y)
     return (x < 0.0f) ? 2.0f + r : 4.0f - r;
 }
 
-// Pseudo-angle of the direction of a disk centre seen from the orbit centre,
-// measured from the start in the direction of travel, from q = centre - disk
-// (the CirclesCross operands); -1 when the start lies within about
-// r + 1e-6 R of the disk (no bound is used for it: its crossing angles may sit
-// at 0 / 2 pi within the error of their FP64 computation).  u0 = start -
-// centre; gmin2 = (1.001 r + 1.001e-6 R)^2.
+// Screening of one lane's crossing disks in one cell.  Angles are travel
+// angles: from the start, in the direction of travel, in [0, 2 pi).  Two facts
+// bound where the orbit can meet a disk of radius r centred at P:
+//  * the arc from the start S to a point X is no shorter than the chord, and
+//    |S - X| >= |S - P| - r for X on the disk, so every meeting point lies at
+//    a travel angle >= (|S - P| - r) / R;
+//  * a disk whose centre is at distance d from the orbit centre meets the
+//    orbit within asin(r / d) <= 1.0473 r / d (r / d <= 1/2) of the centre's
+//    direction, and a crossing disk has d > R - r: within dmax = 1.0474 r /
+//    (R - r).  When the start lies outside the disk, those points do not wrap
+//    round the start, so they lie within dmax of the centre's travel angle
+//    th_c; the pseudo-angle p_c <= th_c of the centre (slope <= 1) gives the
+//    lower bound p_c - dmax.
+// "Near" disks, whose centre is within 1.001 r + 1.001e-6 R of the start, get
+// only the first bound.  A disk is dropped when its lower bound reaches the
+// travel angle of the minimum time so far plus 1e-5 rad, or when its centre
+// lies outside the cell's angle range widened by dmax + 1e-5 rad: the margin
+// covers the FP32 bounds (< 1e-6 rad) and the FP64 crossing angles (~3e-8
+// rad), so a dropped disk's time, as the reference computes it, could not
+// have lowered the minimum or passed AngleInRange.
+struct DiskScreen {
+    double u0x, u0y;        // start - orbit centre
+    float gmin2;            // (1.001 r + 1.001e-6 R)^2
+    float r, inv_R, dmax;   // dmax = +inf when r / (R - r) is not in (0, 1/2]
+    float lo_pa, hi_pa;     // centre pseudo-angles kept: (lo_pa, hi_pa)
+};
+
+// The window of centre pseudo-angles whose crossings can fall in the cell's
+// range (entry rx, exit ry, as AngleInRange takes them, details.h:119-136):
+// its accepted angles lie on the arc travelled from rx to ry.  No window when
+// every angle is accepted (|rx - ry| < 1e-10), the arc is a whole turn, or it
+// ends within dmax + 1e-4 of a full turn.
 template <bool CW>
-__device__ __forceinline__ float d_disk_proxy(double u0x, double u0y, float gmin2, double qx, double qy)
+__device__ __forceinline__ void d_range_window(double pa, double rx, double ry, DiskScreen& ds)
 {
-    const float gx = (float)(u0x + qx), gy = (float)(u0y + qy);  // start - disk
-    const float ux = (float)u0x, uy = (float)u0y;
-    const float vx = (float)(-qx), vy = (float)(-qy);             // disk - centre
-    const float dot = ux * vx + uy * vy;
-    const float crs = ux * vy - uy * vx;
-    const float pa = pseudo_angle(dot, CW ? -crs : crs);
-    return (gx * gx + gy * gy < gmin2) ? -1.0f : pa;
+    ds.lo_pa = -2.0f;
+    ds.hi_pa = 5.0f;
+    if (fabs(rx - ry) < 1e-10 || !(ds.dmax < 1.0f)) return;
+    const double low = CW ? ry : rx;
+    double high = CW ? rx : ry;
+    high = (high < low) ? (high + ES_PI2) : high;
+    const double span = high - low;
+    if (!(span < ES_PI2)) return;
+    const double tin = d_cross_angle<CW>(pa, rx);  // travel angle of the entry
+    const double th = tin + span + (double)ds.dmax + 1e-5;
+    if (!(th < ES_PI2 - 1e-4)) return;
+    float s, c;
+    sincosf((float)th, &s, &c);
+    ds.hi_pa = pseudo_angle(c, s) - 1e-6f;
+    const double tl = tin - (double)ds.dmax - 1e-5;
+    if (tl > 0.0) {
+        sincosf((float)tl, &s, &c);
+        ds.lo_pa = pseudo_angle(c, s) + 1e-6f;
+    }
 }
 
-// Pseudo-angle at and beyond which a crossing disk's centre lies too far
-// along the orbit to cross it before time t, or 5 (nothing is dropped).
-// A disk of radius r whose centre is at distance d from the orbit centre
-// meets the orbit within asin(r / d) <= 1.0473 r / d (r / d <= 1/2) of the
-// centre's direction, and a crossing disk has d > R - r.  The 1e-5 rad margin
-// covers the FP32 pseudo-angles (< 5e-6 rad) and the FP64 crossing angles.
-__device__ __forceinline__ float d_prune_threshold(double R, double w, double radius, double t)
+// A crossing disk (q = orbit centre - disk, the CirclesCross operands): false
+// when its centre is outside the range window, else lb = the lower bound on
+// its meeting points' travel angles.
+template <bool CW>
+__device__ __forceinline__ bool d_disk_screen(const DiskScreen& ds, double qx, double qy, float& lb)
 {
-    if (!(t < ES_INF)) return 5.0f;
-    const float q = (float)radius / (float)(R - radius);
-    if (!(q > 0.0f && q <= 0.5f)) return 5.0f;
-    const double theta = w * t + (double)(1.0474f * q) + 1e-5;
-    if (!(theta < ES_PI2 - 1e-4)) return 5.0f;
-    float sc, cc;
-    sincosf((float)theta, &sc, &cc);
-    return pseudo_angle(cc, sc) - 1e-6f;
+    const float gx = (float)(ds.u0x + qx), gy = (float)(ds.u0y + qy);  // start - disk
+    const float ux = (float)ds.u0x, uy = (float)ds.u0y;
+    const float vx = (float)(-qx), vy = (float)(-qy);                   // disk - centre
+    const float dot = ux * vx + uy * vy;
+    const float crs = ux * vy - uy * vx;
+    const float pc = pseudo_angle(dot, CW ? -crs : crs);
+    const float g2 = gx * gx + gy * gy;
+    const bool near = g2 < ds.gmin2;
+    const float ld = (__builtin_sqrtf(g2) - ds.r) * ds.inv_R;
+    lb = near ? ld : fmaxf(ld, pc - ds.dmax);
+    return near || (pc > ds.lo_pa && pc < ds.hi_pa);
+}
+
+// Travel angle below which a disk's bound must lie for it to be solved, given
+// the minimum time t so far.
+__device__ __forceinline__ float d_time_threshold(double w, double t)
+{
+    return (t < ES_INF) ? (float)(w * t) + 1e-5f : 3.0e38f;
 }
 
 // GetFirstCrossTime of disk gi of owner `own`'s field, folded into the
@@ -887,10 +935,18 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         o.u0y = L.orb[11][lane] - o.cy;
         o.r = (float)g.radius;
         const double phi0 = o.pa;
-        float gmin2;
-        {
+        DiskScreen ds;
+        ds.lo_pa = -2.0f;
+        ds.hi_pa = 5.0f;
+        if (SCREEN) {
+            ds.u0x = o.u0x;
+            ds.u0y = o.u0y;
             const float gm = 1.001f * (float)g.radius + 1.001e-6f * (float)o.R;
-            gmin2 = gm * gm;
+            ds.gmin2 = gm * gm;
+            ds.r = (float)g.radius;
+            ds.inv_R = (float)o.inv_R;
+            const float q = (float)g.radius / (float)(o.R - g.radius);
+            ds.dmax = (q > 0.0f && q <= 0.5f) ? 1.0474f * q : __builtin_inff();
         }
 
         // This lane's cell: the next cell and angle range (lifetime.h:106-115).
@@ -901,6 +957,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
             avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
             L.orb[8][lane] = entry.ang;
             L.orb[9][lane] = avail ? st.nx.ang : phi0;
+            if (SCREEN) d_range_window<CW>(phi0, entry.ang, avail ? st.nx.ang : phi0, ds);
             const int ci = entry.cy * g.cpr + entry.cx;
             s = (ci > 0) ? g.cell_end[ci - 1] : 0;
             n = g.cell_end[ci] - s;
@@ -911,45 +968,48 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         int i = 0;
         while (true) {
             // 1. CirclesCross, G disks at a time (fewer registers when
-            // screening); the earliest crossing disk stays with its lane, the
-            // others are queued.
+            // screening); with SCREEN, disks outside the range window are
+            // dropped, the one with the lowest bound stays with its lane and
+            // the others are queued with their bounds.
             constexpr int G = SCREEN ? 2 : 4;
             int top = 0;  // wave-uniform
-            float first_pa = 3.0e38f;
+            float first_lb = 3.0e38f;
             int first_i = -1;
             bool more;
             while ((more = (__ballot(i < n) != 0ull)) && top <= kCoopCap - 64 * G) {
                 double2 p[G];
 #pragma unroll
                 for (int u = 0; u < G; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
-                int hits = 0;
-                float pa[G];
+                int hits = 0, kept = 0;
+                float lbs[G];
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
                     const double d2 = qx * qx + qy * qy;
                     const bool h = (i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2);
                     hits |= h ? (1 << u) : 0;
-                    pa[u] = SCREEN ? d_disk_proxy<CW>(o.u0x, o.u0y, gmin2, qx, qy) : 0.0f;
+                    lbs[u] = 0.0f;
+                    const bool in_window = SCREEN ? d_disk_screen<CW>(ds, qx, qy, lbs[u]) : true;
+                    kept |= (h && in_window) ? (1 << u) : 0;
                 }
                 st.cross_calls += __popc(hits);
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    bool h = (hits >> u) & 1;
+                    bool h = (kept >> u) & 1;
                     int qi = s + i + u;
-                    float qp = pa[u];
-                    if (SCREEN && h && pa[u] < first_pa) {  // a new earliest: the previous one is queued
+                    float ql = lbs[u];
+                    if (SCREEN && h && lbs[u] < first_lb) {  // a new lowest bound: the previous one is queued
                         qi = first_i;
-                        qp = first_pa;
+                        ql = first_lb;
                         h = first_i >= 0;
-                        first_pa = pa[u];
+                        first_lb = lbs[u];
                         first_i = s + i + u;
                     }
                     const unsigned long long m = __ballot(h);
                     if (h) {
                         const int pos = top + __popcll(m & below);
                         L.qidx[pos] = qi;
-                        if (SCREEN) L.qpa[pos] = qp;
+                        if (SCREEN) L.qlb[pos] = ql;
                         L.qown[pos] = (uint8_t)lane;
                     }
                     top += __popcll(m);
@@ -957,21 +1017,21 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
                 i += G;
             }
 
-            // 2. Each lane solves its earliest disk, unless the minimum so far
-            // already rules it out; the threshold that results.
+            // 2. Each lane solves its lowest-bound disk, unless the minimum so
+            // far already rules it out; the threshold that results.
             // (The orbit is read back from LDS, as for a shared solve.)
             if (SCREEN && active) {
-                const double R = L.orb[2][lane], w = L.orb[5][lane];
-                float thr = d_prune_threshold(R, w, g.radius, d_from_order_key(L.tmin[lane]));
-                if (first_i >= 0 && !(first_pa >= thr)) {
+                const double w = L.orb[5][lane];
+                float thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
+                if (first_i >= 0 && !(first_lb >= thr)) {
                     d_coop_solve<CW>(L, lane, first_i);
-                    thr = d_prune_threshold(R, w, g.radius, d_from_order_key(L.tmin[lane]));
+                    thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
                 }
                 L.thr[lane] = thr;
             }
             wave_lds_sync();
 
-            // 3. Drop the queued disks at or past their owner's threshold; compact.
+            // 3. Drop the queued disks whose bound reaches their owner's threshold; compact.
             int nsurv = SCREEN ? 0 : top;  // wave-uniform
             for (int k0 = 0; SCREEN && k0 < top; k0 += 64) {
                 const int k = k0 + lane;
@@ -980,7 +1040,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
                 if (k < top) {
                     own = L.qown[k];
                     gi = L.qidx[k];
-                    keep = !(L.qpa[k] >= L.thr[own]);
+                    keep = !(L.qlb[k] >= L.thr[own]);
                 }
                 const unsigned long long m = __ballot(keep);
                 if (keep) {
