@@ -1,9 +1,10 @@
 #!/usr/bin/env python3
 """Throughput of the lifetime / sigma path on one MI355X.
 
-A step is one impurity sample of the production sweep: the field is drawn
-and binned on the GPU, then the incoherent and the coherent pass each run the
-40-field sweep with the 3 temperatures (SimulationRunner.cpp:88-98 order).
+A step is the whole production sweep: 8 impurity samples, each field drawn
+and binned on the GPU, and for each sample the incoherent and the coherent
+pass over the 40 fields with the 3 temperatures (SimulationRunner.cpp:88-98),
+run as one batch: one launch per stage for all samples, passes and fields.
 
 Prints one JSON line: lifetimes/s of the FP64 path (incoherent lifetimes, the
 coherent pass is run but not counted; `value`), the sweep wall time, the
@@ -130,6 +131,8 @@ def main():
               "see DESIGN.md", file=sys.stderr)
         return 2
 
+    import numpy as np
+
     import elasticscattering_amd as es
     from elasticscattering_amd.workloads import PRODUCTION_FP64, SMALL_FP64, region, lifetimes_per_pass
 
@@ -148,57 +151,73 @@ def main():
     fields, temps = w["fields"], w["temperatures"]
     P = lifetimes_per_pass(w)
 
-    sim = es.Simulation(ppr, ppq)
-    sim_mixed = es.Simulation(ppr, ppq, precision="mixed")
+    def production_entries(b):
+        """The 8 samples as batch entries: per sample its incoherent pass (drawn and binned
+        on the GPU) and its coherent pass on the same field (SimulationRunner.cpp:88-98)."""
+        b.reset()  # every step draws and bins its fields again
+        for seed in w["seeds"]:
+            kw = dict(seed=seed, region_size=rs, region_extends=ext, impurity_radius=w["impurity_radius"],
+                      positions_per_row=ppr, particles_per_quadrant=ppq, fields=fields, temperatures=temps,
+                      tau=w["tau"], alpha=w["alpha"], is_clockwise=w["clockwise"],
+                      particle_speed=w["particle_speed"], incoherent_tau=w["incoherent_tau"],
+                      count=w["impurity_count"], cells_per_row=w["cells_per_row"])
+            i = b.add(coherent=False, **kw)
+            b.add(coherent=True, share_field_of=i, **kw)
 
-    def step(s, stats, sim=sim):
-        seed = w["seeds"][s % len(w["seeds"])]
-        sim.generate_impurities(seed, rs, ext, w["impurity_radius"], count=w["impurity_count"],
-                                cells_per_row=w["cells_per_row"])
-        for coherent in (0, 1):
-            sim.init_sample(w["tau"], w["alpha"], w["clockwise"], w["particle_speed"], coherent, w["incoherent_tau"])
-            sig, ms = sim.compute_sweep(fields, temps)
-            if stats is not None and coherent == 0:
-                stats["ms"].append(ms)  # reduced after the timed loop
-                stats["sigma"].append(sig)
+    def step(b, stats):
+        production_entries(b)
+        sig, mets, secs = b.run()
+        if stats is not None:
+            stats["runs"].append((sig, mets, secs))  # reduced after the timed loop
 
-    def timed(sim_):
-        for s in range(args.warmup):
-            step(s, None, sim_)
-        st = {"ms": [], "sigma": []}
-        t0 = time.perf_counter()
-        for s in range(args.steps):
-            step(s, st, sim_)
-        dt_ = time.perf_counter() - t0
-        st["launch_s"] = [float(ms["time_elapsed_lifetimes"].sum()) for ms in st["ms"]]
-        st["flops"] = [algorithmic_flops(ms, TRACE_WORK) for ms in st["ms"]]
-        st["ref_flops"] = [algorithmic_flops(ms, REFERENCE_TRACE_WORK) for ms in st["ms"]]
-        st["sigma_flops"] = [algorithmic_flops(ms, SIGMA_WORK) for ms in st["ms"]]
-        st["sigma_s"] = [float(ms["time_elapsed_temperatures"].sum()) for ms in st["ms"]]
-        for key, col in (("traced", "traced_particles"), ("cross", "cross_calls"), ("exact", "cross_exact"),
-                         ("recomputed", "fp64_recomputed")):
-            st[key] = int(sum(int(ms[col].sum()) for ms in st["ms"]))
+    def timed(precision):
+        with es.Batch() as b:
+            b.set_precision(precision)
+            for _ in range(args.warmup):
+                step(b, None)
+            st = {"runs": []}
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step(b, st)
+            dt_ = time.perf_counter() - t0
+        st["launch_s"], st["flops"], st["ref_flops"], st["sigma_flops"], st["sigma_s"] = [], [], [], [], []
+        st["sigma"] = []
+        tot = dict(traced=0, cross=0, exact=0, recomputed=0)
+        for sig, mets, _ in st["runs"]:
+            # Every (entry, field) pair of the one launch, with the share of its counters
+            # that was executed: the coherent window at alpha = pi/4 has zero width, so its
+            # ppq phi points are one particle, traced once, and its counters (which count
+            # all ppq lifetimes, as the reference's metrics do) are ppq times the work done.
+            ms = [(m, 1.0 if e % 2 == 0 else 1.0 / ppq) for e, per in enumerate(mets) for m in per]
+            st["launch_s"].append(sum(m["time_elapsed_lifetimes"] for m, _ in ms))
+            st["sigma_s"].append(sum(m["time_elapsed_temperatures"] for m, _ in ms))
+            st["flops"].append(sum(x * algorithmic_flops(m, TRACE_WORK) for m, x in ms))
+            st["ref_flops"].append(sum(x * algorithmic_flops(m, REFERENCE_TRACE_WORK) for m, x in ms))
+            st["sigma_flops"].append(sum(x * algorithmic_flops(m, SIGMA_WORK) for m, x in ms))
+            st["sigma"].append(np.stack(sig[0::2]))  # incoherent passes [sample, field, T, 2]
+            for key, col in (("traced", "traced_particles"), ("cross", "cross_calls"), ("exact", "cross_exact"),
+                             ("recomputed", "fp64_recomputed")):
+                tot[key] += sum(round(m[col] * x) for m, x in ms)
+        st.update(tot)
         return st, dt_
 
-    stats, dt = timed(sim)
+    stats, dt = timed("fp64")
     mixed = None
     if not args.no_extras:
-        import numpy as np
-
-        mstats, mdt = timed(sim_mixed)
+        mstats, mdt = timed("mixed")
         err = 0.0
         for a, b in zip(mstats["sigma"], stats["sigma"]):
             n = np.hypot(b[..., 0], b[..., 1])
             err = max(err, float((np.abs(a - b).max(axis=-1) / n).max()))
         mixed = {
-            "value": args.steps * len(fields) * P / mdt, "unit": "lifetimes/s",
-            "ms_per_step": mdt / args.steps * 1e3, "sweep_wall_s": mdt * len(w["seeds"]) / args.steps,
+            "value": args.steps * len(w["seeds"]) * len(fields) * P / mdt, "unit": "lifetimes/s",
+            "ms_per_step": mdt / args.steps * 1e3, "sweep_wall_s": mdt / args.steps,
             "trace_ms_per_launch": sum(mstats["launch_s"]) / len(mstats["launch_s"]) * 1e3,
             "max_sigma_error_vs_fp64": err, "fp64_recomputed_fraction": mstats["recomputed"] / max(mstats["traced"], 1),
             "dtype": "f32 trace + f64 recompute, f64 start data / Simpson / sigma sums",
         }
 
-    lifetimes = args.steps * len(fields) * P
+    lifetimes = args.steps * len(w["seeds"]) * len(fields) * P
     avg_launch = sum(stats["launch_s"]) / len(stats["launch_s"])
     avg_flops = sum(stats["flops"]) / len(stats["flops"])
     achieved = avg_flops / avg_launch / 1e12
@@ -224,17 +243,17 @@ def main():
             "workload": w["name"],
             "impurities": w["impurity_count"], "side_m": w["side"], "impurity_radius_m": w["impurity_radius"],
             "cells": f"{w['cells_per_row']}x{w['cells_per_row']}", "positions": f"{ppr}x{ppr}",
-            "phi_points": 4 * ppq, "fields": len(fields), "temperatures": temps, "samples_timed": args.steps,
+            "phi_points": 4 * ppq, "fields": len(fields), "temperatures": temps, "samples_per_step": len(w["seeds"]),
             "passes": "incoherent (counted) + coherent (run, alpha=pi/4 gives zero weight)",
         },
-        "sweep_wall_s": dt * len(w["seeds"]) / args.steps,
+        "sweep_wall_s": dt / args.steps,
         "traced_lifetimes_per_s": stats["traced"] / dt,
         "exact_cross_solves_fraction": stats["exact"] / max(stats["cross"], 1),
         "roofline": {
             "bound": "mfma",
             "bound_detail": "FP64 vector ALU work, priced at MI355X's 78.6 TFLOP/s FP64 peak (vector and matrix FP64 "
                             "peaks are equal); HBM traffic is ~2 % of peak (profiles/k_trace_production_fp64.md)",
-            "kernel": "k_trace (CreateParticle + TraceOrbit, one launch per sample and pass, all fields)",
+            "kernel": "k_trace (CreateParticle + TraceOrbit; one launch per step: 8 samples, both passes, 40 fields)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -256,8 +275,6 @@ def main():
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     print(json.dumps(line), flush=True)
-    sim.close()
-    sim_mixed.close()
     return 0
 
 

@@ -172,6 +172,7 @@ _SIGNATURES = {
     "es_batch_last_error": (C.c_char_p, [_P]),
     "es_batch_add_sample": (C.c_int, [_P, C.POINTER(BatchSample), C.POINTER(C.c_int32)]),
     "es_batch_clear": (C.c_int, [_P]),
+    "es_batch_reset": (C.c_int, [_P]),
     "es_batch_set_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_get_shape": (C.c_int, [_P, C.POINTER(BatchShape)]),
     "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
@@ -433,6 +434,12 @@ class Batch:
         """Removes every entry; keeps the drawn fields for later entries with the same recipe
         (seed, count, grid description), which then skip drawing and binning."""
         self._check(self.lib.es_batch_clear(self.h))
+        self.entries = []
+
+    def reset(self):
+        """Removes every entry and forgets the drawn fields: later entries draw and bin
+        their fields again (into the memory the batch already holds)."""
+        self._check(self.lib.es_batch_reset(self.h))
         self.entries = []
 
     def counts(self):

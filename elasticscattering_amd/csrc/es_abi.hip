@@ -219,6 +219,7 @@ struct Field {
     int64_t* hsmall = nullptr;
     bool total_pending = false;
     bool pos_pending = false;
+    bool copy_queued = false;   // the copy into hsmall is enqueued behind the last launch
     bool counts_clean = false;  // b_counts all zero (left so by every binning's scan)
 };
 
@@ -254,15 +255,23 @@ int field_small(Ctx& c, Field& f)
     return field_host_small(c, f);
 }
 
-// Takes the field's read-backs in flight (the indexed count of a binning, the
-// start test of the positions) with one copy and one wait.  With
-// pos_errors = false a start position outside the grid only leaves the
-// positions invalid (the next field_positions reports it).
-int field_sync(Ctx& c, Field& f, bool pos_errors = true)
+// Enqueues the copy of the field's read-backs in flight (the indexed count of
+// a binning, the start test of the positions) behind the launches that
+// produce them; field_sync_finish takes them once the stream has reached it.
+int field_sync_enqueue(Ctx& c, Field& f)
 {
-    if (f.total_pending || f.pos_pending)
+    if ((f.total_pending || f.pos_pending) && !f.copy_queued) {
         ES_HIP(c, hipMemcpyAsync(f.hsmall, f.small.p, 16, hipMemcpyDeviceToHost, c.stream));
-    ES_HIP(c, hipStreamSynchronize(c.stream));
+        f.copy_queued = true;
+    }
+    return ES_OK;
+}
+
+// With pos_errors = false a start position outside the grid only leaves the
+// positions invalid (the next field_positions reports it).
+int field_sync_finish(Ctx& c, Field& f, bool pos_errors = true)
+{
+    f.copy_queued = false;
     const int32_t* h = reinterpret_cast<const int32_t*>(f.hsmall);
     if (f.total_pending) {
         f.total_pending = false;
@@ -283,6 +292,15 @@ int field_sync(Ctx& c, Field& f, bool pos_errors = true)
         f.pos_valid = true;
     }
     return ES_OK;
+}
+
+// The read-backs in flight with one copy and one wait.
+int field_sync(Ctx& c, Field& f, bool pos_errors = true)
+{
+    int rc;
+    if ((rc = field_sync_enqueue(c, f))) return rc;
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    return field_sync_finish(c, f, pos_errors);
 }
 
 // Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180), drawing them
@@ -333,6 +351,7 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* 
                              c.stream));
     f.counts_clean = true;
     f.total_pending = true;
+    f.copy_queued = false;
     f.unique = n;
     f.has_grid = true;
     f.pos_valid = false;
@@ -399,6 +418,7 @@ int field_positions_launch(Ctx& c, Field& f, int ppr, double dist, double offx, 
                                    P<int32_t>(f.free_pos), small + 1, c.stream));
     f.pos_valid = false;
     f.pos_pending = true;
+    f.copy_queued = false;
     f.pos_ppr = ppr;
     f.pos_dist = dist;
     f.pos_offx = offx;
@@ -415,6 +435,18 @@ int field_positions(Ctx& c, Field& f, int ppr, double dist, double offx, double 
     int rc;
     if ((rc = field_positions_launch(c, f, ppr, dist, offx, offy))) return rc;
     return field_sync(c, f);
+}
+
+// The same without the wait: the start test (when the positions are new) and
+// the copy of the read-backs are enqueued; field_sync_finish takes them after
+// the next stream synchronisation.
+int field_positions_deferred(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
+{
+    const bool same = f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy;
+    int rc;
+    if (!(same && (f.pos_valid || f.pos_pending)))
+        if ((rc = field_positions_launch(c, f, ppr, dist, offx, offy))) return rc;
+    return field_sync_enqueue(c, f);
 }
 
 // ----------------------------------------------------------------- stages
@@ -918,6 +950,7 @@ struct es_batch {
         uint64_t seed;
         int64_t count;
         es_grid_desc grid;
+        bool valid;              // false after es_batch_reset: the memory is reused by the next draw
     };
     std::vector<Field*> fields;
     std::vector<Recipe> recipes;     // per field
@@ -1234,6 +1267,25 @@ int batch_plan(es_batch* bt)
     Plan& pl = bt->plan;
     pl = Plan();
     if (bt->entries.empty()) return fail(c, ES_ERR_INVALID_ARG, "empty batch");
+    // The read-backs of the entries' fields still in flight: one wait for all.
+    std::vector<int> pending;
+    for (const es_batch::Entry& en : bt->entries) {
+        Field& f = *bt->fields[en.field];
+        if (!(f.total_pending || f.pos_pending) || std::find(pending.begin(), pending.end(), en.field) != pending.end())
+            continue;
+        int rc;
+        if ((rc = field_sync_enqueue(c, f))) return rc;
+        pending.push_back(en.field);
+    }
+    if (!pending.empty()) {
+        ES_HIP(c, hipStreamSynchronize(c.stream));
+        int first_rc = ES_OK;
+        for (int fi : pending) {
+            const int rc = field_sync_finish(c, *bt->fields[fi]);
+            if (rc && !first_rc) first_rc = rc;
+        }
+        if (first_rc) return first_rc;
+    }
     pl.clockwise = bt->entries[0].st.is_clockwise ? 1 : 0;
     pl.precision = bt->precision;
     int out = 0;
@@ -1242,6 +1294,10 @@ int batch_plan(es_batch* bt)
         if ((en.st.is_clockwise ? 1 : 0) != pl.clockwise)
             return fail(c, ES_ERR_INVALID_ARG, "a batch holds one rotation sense (is_clockwise)");
         Field& f = *bt->fields[en.field];
+        if (!f.pos_valid)
+            return fail(c, ES_ERR_OUT_OF_GRID,
+                        "a start position lies outside the impurity cell grid (region_extends smaller than the "
+                        "position offset); the reference would read out of bounds (lifetime.h:88-91)");
         es::SampleDev sd = sample_dev(f, en.ppr, en.ppq, en.dist, en.offx, en.offy, en.ps, en.st, en.coherent);
         sd.nT = (int)en.T.size();
         for (int t = 0; t < es::kMaxT; t++) {
@@ -1411,25 +1467,40 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
         en.field = bt->entries[smp->share_field_of].field;
     } else {
         en.field = -1;
+        int spare = -1;  // a forgotten field no entry uses: its memory takes the new draw
         for (size_t i = 0; i < bt->fields.size() && en.field < 0; i++) {
-            const es_batch::Recipe& r = bt->recipes[i];
-            if (r.seed != smp->seed || r.count != smp->impurity_count || !same_grid_desc(r.grid, smp->grid)) continue;
             bool used = false;
             for (const es_batch::Entry& other : bt->entries) used = used || other.field == (int)i;
-            if (!used) en.field = (int)i;
+            if (used) continue;
+            const es_batch::Recipe& r = bt->recipes[i];
+            if (!r.valid) {
+                if (spare < 0) spare = (int)i;
+                continue;
+            }
+            if (r.seed == smp->seed && r.count == smp->impurity_count && same_grid_desc(r.grid, smp->grid))
+                en.field = (int)i;
         }
         if (en.field < 0) {
-            Field* f = new Field();
-            es_grid_info info{};
-            rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, &info);
+            // Drawn, binned and start-tested without a wait: es_batch_run (or
+            // es_batch_counts) collects the read-backs of every new field at once.
+            Field* f = spare >= 0 ? bt->fields[spare] : new Field();
+            if (spare >= 0) bt->recipes[spare].valid = false;
+            rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, nullptr);
             if (rc) {
-                field_release(c, *f);
-                delete f;
+                if (spare < 0) {
+                    field_release(c, *f);
+                    delete f;
+                }
                 return rc;
             }
-            bt->fields.push_back(f);
-            bt->recipes.push_back({smp->seed, smp->impurity_count, smp->grid});
-            en.field = (int)bt->fields.size() - 1;
+            if (spare < 0) {
+                bt->fields.push_back(f);
+                bt->recipes.push_back({smp->seed, smp->impurity_count, smp->grid, true});
+                en.field = (int)bt->fields.size() - 1;
+            } else {
+                bt->recipes[spare] = {smp->seed, smp->impurity_count, smp->grid, true};
+                en.field = spare;
+            }
         }
     }
     Field& f = *bt->fields[en.field];
@@ -1446,7 +1517,7 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
     en.B.assign(smp->fields, smp->fields + smp->n_fields);
     en.T.assign(smp->temperatures, smp->temperatures + smp->n_temperatures);
     if ((rc = taus_for(c, en.st, en.coherent, en.T.data(), (int)en.T.size(), en.taus))) return rc;
-    if ((rc = field_positions(c, f, en.ppr, en.dist, en.offx, en.offy))) return rc;
+    if ((rc = field_positions_deferred(c, f, en.ppr, en.dist, en.offx, en.offy))) return rc;
     bt->entries.push_back(en);
     bt->plan_ready = false;
     if (index) *index = (int32_t)bt->entries.size() - 1;
@@ -1459,6 +1530,16 @@ int es_batch_clear(es_batch* bt)
     bt->entries.clear();
     bt->plan_ready = false;
     batch_release_graph(bt);
+    return ES_OK;
+}
+
+int es_batch_reset(es_batch* bt)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    bt->entries.clear();
+    bt->plan_ready = false;
+    batch_release_graph(bt);
+    for (es_batch::Recipe& r : bt->recipes) r.valid = false;
     return ES_OK;
 }
 

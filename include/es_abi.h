@@ -212,7 +212,12 @@ typedef struct es_batch_shape {
 int es_batch_create(int32_t device, es_batch** out);
 void es_batch_destroy(es_batch* b);
 const char* es_batch_last_error(const es_batch* b);
-/* Draws and bins the entry's field on the GPU (unless shared) and runs its start-cell test. */
+/*
+ * Draws and bins the entry's field on the GPU (unless shared) and runs its
+ * start-cell test, without waiting: the results of every new field are read
+ * back together by the next es_batch_run or es_batch_counts, which also report
+ * a start position outside a field's cell grid (ES_ERR_OUT_OF_GRID).
+ */
 int es_batch_add_sample(es_batch* b, const es_batch_sample* sample, int32_t* index);
 /*
  * Removes every entry and keeps the drawn fields: a sample added afterwards
@@ -221,6 +226,14 @@ int es_batch_add_sample(es_batch* b, const es_batch_sample* sample, int32_t* ind
  * Fields are freed by es_batch_destroy.
  */
 int es_batch_clear(es_batch* b);
+/*
+ * Removes every entry and forgets what drew the kept fields: later entries
+ * draw and bin their fields again, into the device memory the batch holds.
+ * (No counterpart in the reference, whose runner draws each sample's field
+ * once, SimulationRunner.cpp:88-98; for callers that run many batches of fresh
+ * samples without freeing and allocating per batch.)
+ */
+int es_batch_reset(es_batch* b);
 int es_batch_set_shape(es_batch* b, const es_batch_shape* shape);
 int es_batch_get_shape(es_batch* b, es_batch_shape* shape);
 /* enable != 0: record the stage launches once in a hipGraph and replay it. */

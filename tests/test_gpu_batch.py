@@ -239,6 +239,34 @@ def test_clear_reuses_fields_with_same_bits(es, batch_eager):
         _same_results(fresh.run(), got)
 
 
+def test_reset_redraws_into_kept_memory_with_same_bits(es, batch_eager):
+    """After es_batch_reset every entry draws and bins its field again (a changed
+    seed, count or radius lands in the memory of a forgotten field); results
+    equal a fresh batch, and a second reset + refill gives the same bits again."""
+    entries, _, _ = batch_eager
+    announce("batch reset: redrawn fields against a fresh batch")
+    later = [dict(entries[1], seed=77), dict(entries[0], temperatures=[3.0]),
+             dict(entries[3], impurity_radius=entries[3]["impurity_radius"] * 1.5)]
+    with es.Batch() as b:
+        for e in entries:
+            _add(b, e)
+        b.run()
+        b.reset()
+        for e in later:
+            _add(b, e)
+        got = b.run()
+        b.reset()
+        for e in later:
+            _add(b, e)
+        again = b.run()
+    with es.Batch() as fresh:
+        for e in later:
+            _add(fresh, e)
+        want = fresh.run()
+    _same_results(want, got)
+    _same_results(want, again)
+
+
 def test_batch_errors(es):
     announce("batch argument errors")
     with es.Batch() as b:
