@@ -52,6 +52,11 @@ SIGMA_WORK = {
 }
 
 
+def _json(obj):
+    """One JSON line; numpy scalars (metrics records) as plain numbers."""
+    return json.dumps(obj, default=lambda o: o.item() if hasattr(o, "item") else str(o))
+
+
 def algorithmic_flops(m, model):
     """Work of one metrics record, or of a record array (summed)."""
     return float(sum(model[k] * m[k].sum() if hasattr(m[k], "sum") else model[k] * m[k] for k in model))
@@ -119,7 +124,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 production line (profiling runs)")
     ap.add_argument("--part", default="all", choices=("all", "production", "batched", "sweep"),
                     help="batched / sweep: print only that part's JSON (profiling runs)")
-    ap.add_argument("--sweep-materials", type=int, default=24)
+    ap.add_argument("--sweep-materials", type=int, default=200)
     ap.add_argument("--train-runs", type=int, default=20000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 4 times (HBM counter calibration) and exit")
@@ -137,10 +142,10 @@ def main():
     from elasticscattering_amd.workloads import PRODUCTION_FP64, SMALL_FP64, region, lifetimes_per_pass
 
     if args.part == "batched":
-        print(json.dumps({"batched_uneven": batched_uneven(es)}), flush=True)
+        print(_json({"batched_uneven": batched_uneven(es)}), flush=True)
         return 0
     if args.part == "sweep":
-        print(json.dumps({"sweep": sweep_comparison(args.sweep_materials, args.train_runs)}), flush=True)
+        print(_json({"sweep": sweep_comparison(args.sweep_materials, args.train_runs)}), flush=True)
         return 0
     if args.part == "production":
         args.no_extras = True
@@ -274,7 +279,7 @@ def main():
         cb = cpu_baseline(w)
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-    print(json.dumps(line), flush=True)
+    print(_json(line), flush=True)
     return 0
 
 
@@ -356,6 +361,8 @@ def sweep_comparison(n_materials, train_runs):
         cmax = err.max(axis=1)
         out[name] = {"device_s": rep["solver_seconds"], "wall_s": wall, "solver_points": rep["solver_points"],
                      "points_per_curve": rep["points_per_curve"], "rounds": rep["rounds"],
+                     "traced_items": rep["traced_items"], "work_vs_full": full_items / max(rep["traced_items"], 1),
+                     "rough_points": rep["rough_points"],
                      "accepted": rep["accepted"], "true_max_error": float(cmax.max()),
                      "curves_within_tolerance": int((cmax <= 1e-3).sum()),
                      "over_tolerance": [{"material": int(i), "error": float(cmax[i]), "solver_jitter": float(jitter[i])}

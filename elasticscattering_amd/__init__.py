@@ -475,7 +475,10 @@ class Batch:
         return sh.as_dict(), t.value
 
     def run(self, metrics=True):
-        """([sigma [nB, nT, 2] per entry], [[metrics dict per field] per entry] | None, device seconds)."""
+        """([sigma [nB, nT, 2] per entry], [metrics record array [nB] per entry] | None, device seconds).
+
+        The metrics of an entry are a numpy record array with the es_metrics fields
+        (``mets[e][b]["cells_passed"]``)."""
         nout = sum(b * t for b, t in self.entries)
         npairs = sum(b for b, _ in self.entries)
         out = (Sigma * max(nout, 1))()
@@ -483,12 +486,13 @@ class Batch:
         secs = C.c_double()
         self._check(self.lib.es_batch_run(self.h, out, ms, C.byref(secs)))
         flat = np.frombuffer(out, dtype=np.float64, count=2 * nout) if nout else np.empty(0)
+        recs = np.frombuffer(ms, dtype=_METRICS_DTYPE, count=npairs).copy() if metrics else None
         sig, mets, o, p = [], [] if metrics else None, 0, 0
         for nB, nT in self.entries:
             sig.append(flat[2 * o: 2 * (o + nB * nT)].reshape(nB, nT, 2).copy())
             o += nB * nT
             if metrics:
-                mets.append([ms[p + b].as_dict() for b in range(nB)])
+                mets.append(recs[p:p + nB])
             p += nB
         return sig, mets, secs.value
 

@@ -638,6 +638,8 @@ void plan_offsets(Plan& pl, long long seg_items)
 
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
 
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
 int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
 {
     const int NQ = 2 * pl.NT + 1;
@@ -647,12 +649,28 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     const size_t bf = (size_t)pl.nout * es::kMaxT * sizeof(double);
     s.t_pairs = align_up(bs);
     s.t_factor = s.t_pairs + align_up(bp);
-    // then each sample's phi table: (sin, cos) of its 4 ppq start angles
+    // then the phi tables: (sin, cos) of the 4 ppq start angles, one per
+    // distinct (ppq, phi_start, phi_step) of the samples
     std::vector<size_t> t_phi(pl.samples.size());
+    std::vector<int> phi_owner(pl.samples.size());  // the first sample with the same start angles
     size_t tables = align_up(s.t_factor + bf);
     for (size_t i = 0; i < pl.samples.size(); i++) {
+        const es::SampleDev& sd = pl.samples[i];
+        phi_owner[i] = (int)i;
+        for (size_t k = 0; k < i; k++) {
+            const es::SampleDev& o = pl.samples[k];
+            if (phi_owner[k] == (int)k && o.ppq == sd.ppq && same_bits(o.phi_start, sd.phi_start) &&
+                same_bits(o.phi_step, sd.phi_step)) {
+                phi_owner[i] = (int)k;
+                break;
+            }
+        }
+        if (phi_owner[i] != (int)i) {
+            t_phi[i] = t_phi[phi_owner[i]];
+            continue;
+        }
         t_phi[i] = tables;
-        tables += align_up((size_t)4 * pl.samples[i].ppq * sizeof(double2));
+        tables += align_up((size_t)4 * sd.ppq * sizeof(double2));
     }
     const size_t bsig = (size_t)pl.nout * es::kMaxT * 2 * sizeof(double), blt = (size_t)pl.nout * sizeof(double);
     s.o_ltsum = align_up(bsig);
@@ -690,6 +708,7 @@ int stages_prepare(Ctx& c, Stages& s, const Plan& pl)
     for (size_t i = 0; i < pl.samples.size(); i++) {
         const es::SampleDev& sd = pl.samples[i];
         hs[i].phi_sc = reinterpret_cast<const double2*>(P<char>(s.tables) + t_phi[i]);
+        if (phi_owner[i] != (int)i) continue;
         double2* tab = reinterpret_cast<double2*>(h + t_phi[i]);
         for (int q = 0; q < 4; q++)
             for (int p = 0; p < sd.ppq; p++) {
@@ -1252,8 +1271,6 @@ int es_solver_compute_sigmas_with_images(es_solver* s, double B, const double* T
 
 namespace {
 
-bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
-
 bool same_grid_desc(const es_grid_desc& a, const es_grid_desc& b)
 {
     return same_bits(a.region_size, b.region_size) && same_bits(a.region_extends, b.region_extends) &&
@@ -1352,7 +1369,6 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
     std::vector<int> nblocks;
     int nblocks64 = 0;
     if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks, &nblocks64))) return rc;
-    ES_HIP(c, hipStreamSynchronize(c.stream));
     hipEvent_t t0 = nullptr, t1 = nullptr;
     ES_HIP(c, hipEventCreate(&t0));
     ES_HIP(c, hipEventCreate(&t1));

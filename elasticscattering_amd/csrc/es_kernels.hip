@@ -1423,7 +1423,7 @@ __device__ __forceinline__ void trace32_coop_body(const TraceArgs& a)
             }
             if (ae != 0xFFFFFFFFu) {
                 const float ak = __uint_as_float(ae);
-                if (ak < best_k + (2e-6f * ak + T32_ABS)) bad = true;
+                if (ak < best_k + (2e-6f * ak + T32_END)) bad = true;
             }
             int flags = 0;
             double theta = 0.0;

@@ -44,7 +44,9 @@
 // traced again in FP64.  They are: a disk within FP32 error of tangency to the
 // orbit, an edge line within error of tangency, an exit within error of a
 // cell corner (the corner snap of cell_grid.h:162-175) or of the entry point
-// (DifferentPoint, :74-77), a start point on a cell edge, a crossing within
+// (DifferentPoint, :74-77), an edge point of the first cell within the FP64
+// angle floor of the start (ahead or behind), a start point on a cell edge, a
+// crossing within
 // error of its cell's range ends, an exit or a hit within error of a full
 // revolution or of the angles m, 2 pi - m above, a hit within the FP64
 // path's angle floor of the start, and a lifetime within 1e-5 of the bound
@@ -150,6 +152,10 @@ __device__ __forceinline__ void t32_begin(Trace32& st, int c0x, int c0y)
 // (details.h:93-107), so it may decide either way within that distance.
 #define T32_REL 2e-5f
 #define T32_ABS 1e-8f
+// Where the reference compares two of its angles (a crossing against a range
+// end, an edge point against the start), each may be off by ~3e-8 rad near
+// y = +-R (asin of a value within a few ulps of 1): a band of 8e-8 rad in keys.
+#define T32_END (4.0f * T32_ABS)
 
 __device__ __forceinline__ bool t32_near(float a, float b, float band) { return fabsf(a - b) <= band; }
 
@@ -217,13 +223,25 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
             const float a = t32_coord(rt[k], n_o, t_o);
             if (!(a > lo - cband && a < hi + cband)) continue;
             const float kk = t32_key(rt[k]);
-            if (kk <= k_in - band_in) continue;  // behind the entry: after a full revolution
+            // An edge point within the FP64 floor of the start, ahead or just
+            // behind: the reference's angle of it (GetAngle, details.h:93-107,
+            // off by up to ~2e-8 rad near y = +-R) may equal phi, which makes it
+            // the exit (dphi = 0, cell_grid.h:80-89) and leaves no next cell
+            // (AngleInRange of a zero-width range, cell_grid.h:177-180).
+            if (first && (kk <= T32_END || kk >= 2.0f - T32_END)) {
+                bad = true;
+                continue;
+            }
+            // Near the entry the reference's two angles (each off by up to ~2e-8
+            // rad near y = +-R) may order the points either way.
+            const float band_e = band_in + T32_END;
+            if (kk <= k_in - band_e) continue;  // behind the entry: after a full revolution
             if (a < lo + cband || a > hi - cband) {  // within error of a corner
                 k_flag = fminf(k_flag, kk);
                 continue;
             }
             st.edge_hits += 1;
-            if (fabsf(kk - k_in) <= band_in) {
+            if (fabsf(kk - k_in) <= band_e) {
                 if (e != st.in_edge) k_flag = fminf(k_flag, kk);  // another edge at the entry point
                 continue;                                         // else: the entry point itself
             }
@@ -247,7 +265,7 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
     // ---- the reference's accepted travel angles in this cell: (a1, b1) and (a2, b2)
     float a1 = k_in, b1 = avail ? k_out : 2.0f, a2 = 2.0f, b2 = 2.0f;
     if (o.quirk) {
-        const float bm = T32_REL * o.km + T32_ABS, bm2 = T32_REL * o.km2;
+        const float bm = T32_REL * o.km + T32_END, bm2 = T32_REL * o.km2 + T32_END;
         if (first) {
             if (avail && o.ccw) {
                 if (t32_near(k_out, o.km, bm)) bad = true;
@@ -375,10 +393,16 @@ __device__ __forceinline__ bool t32_disk(const Disk32Ctx& c, double dx, double d
         in[k] = (kk[k] > c.a1 && kk[k] < c.b1) || (kk[k] > c.a2 && kk[k] < c.b2);
         // A crossing within error of a range end (a cell boundary) may belong
         // to either cell, and FP32 keys can tie there.
-        const float rb = 2e-6f * kk[k] + T32_ABS;
+        const float rb = 2e-6f * kk[k] + T32_END;
         at_end[k] = (c.a1 > 0.0f && t32_near(kk[k], c.a1, rb)) || (c.b1 < 2.0f && t32_near(kk[k], c.b1, rb)) ||
                     (c.a2 > 0.0f && c.a2 < 2.0f && t32_near(kk[k], c.a2, rb)) ||
                     (c.b2 < 2.0f && t32_near(kk[k], c.b2, rb));
+        // First cell: a root just behind the start may come out just ahead of
+        // it in the reference's angles (a hit at once): treat it as at the start.
+        if (c.k_in == 0.0f && kk[k] >= 2.0f - T32_END) {
+            at_end[k] = true;
+            kk[k] = 0.0f;
+        }
     }
     return true;
 }
@@ -392,9 +416,9 @@ __device__ __forceinline__ bool t32_finish(const Orb32& o, int cap, const Cell32
     st.best_k = best_k;
     st.best = best;
     if (!bad && o.quirk && best_k < 2.0f &&
-        (t32_near(best_k, o.km, T32_REL * o.km + T32_ABS) || t32_near(best_k, o.km2, T32_REL * o.km2)))
+        (t32_near(best_k, o.km, T32_REL * o.km + T32_END) || t32_near(best_k, o.km2, T32_REL * o.km2 + T32_END)))
         bad = true;  // a hit at the quirk angles
-    if (!bad && best_k < 2.0f * T32_ABS) bad = true;  // a hit within the FP64 floor of the start
+    if (!bad && best_k < T32_END) bad = true;  // a hit within the FP64 floor of the start
     if (bad) {
         flags = ES_F_RECOMPUTE;
         theta = 0.0;
@@ -467,7 +491,7 @@ __device__ __forceinline__ bool t32_step(const Orb32& o, const DevGrid& g, int c
             st.cross_calls += 1;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                const float rb = 2e-6f * kk[k] + T32_ABS;
+                const float rb = 2e-6f * kk[k] + T32_END;
                 if (at_end[k] && kk[k] < best_k + rb) bad = true;
                 if (in[k] && kk[k] < best_k) {
                     best_k = kk[k];

@@ -184,7 +184,10 @@ def solver_runs(materials, fields, temperatures=None, positions_per_row=128, par
 
     ``fields[i]`` / ``temperatures[i]`` are material i's B values and
     temperatures (default: its own T).  Returns (sigma list, device seconds,
-    traced work items[, start positions outside every impurity per material]).
+    traced work items[, start positions outside every impurity per material,
+    escaped fraction per material and field]); the escaped fraction is the
+    share of the real particles of both passes whose trace ended without a hit
+    (left the grid or closed a full revolution: lifetime 0, lifetime.h:131-146).
     With a dict ``reuse``, the batch is kept in it and cleared for the next
     call (its drawn fields and buffers stay; the caller closes
     ``reuse["batch"]``); otherwise a batch is made and closed per call.
@@ -217,8 +220,12 @@ def _solver_batch(b, materials, fields, temperatures, positions_per_row, particl
     sig, mets, secs = b.run(metrics=with_free)
     out = [sig[2 * i] + sig[2 * i + 1] for i in range(len(materials))]
     if with_free:
-        free = [mets[2 * i + 1][0]["real_particles"] // (4 * particles_per_quadrant) for i in range(len(materials))]
-        return out, secs, items, free
+        free = [int(mets[2 * i + 1][0]["real_particles"]) // (4 * particles_per_quadrant)
+                for i in range(len(materials))]
+        esc = [(mets[2 * i]["particles_escaped"] + mets[2 * i + 1]["particles_escaped"]) /
+               np.maximum(mets[2 * i]["real_particles"] + mets[2 * i + 1]["real_particles"], 1)
+               for i in range(len(materials))]
+        return out, secs, items, free, esc
     return out, secs, items
 
 
@@ -355,11 +362,14 @@ class _Curve:
         self.prior = prior      # sigma [members][n][2] of the surrogate, or None
         self.solved = {}        # index -> sigma [2]
         self.active = set()     # gaps (a, b) of consecutive solved indices still to resolve
+        self.dense = set()      # indices to solve next round where the curve is rough at grid scale
         self.phase = "init"     # init -> refine -> check -> done
         self.pending = []       # [(index, gap or None)] asked of the solver this round
         self.checks = []        # (index, error) of the acceptance checks
         self.accepted = False
         self.skipped = 0        # gaps the surrogate resolved without a solver point
+        self.rough = 0          # points found rough at grid scale
+        self.escaped = {}       # index -> escaped fraction of the solver run
 
     def estimate(self):
         idx = np.array(sorted(self.solved))
@@ -368,26 +378,52 @@ class _Curve:
         est[idx] = vals
         return est
 
+    def grow(self, i, width, n):
+        """Solve every unsolved index within `width` of i next round."""
+        self.dense |= {k for k in range(max(0, i - width), min(n, i + width + 1)) if k not in self.solved}
 
-def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.25, kappa=0.2, n_checks=3,
-                   max_rounds=40, seed=11, **solver_kw):
+
+def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.15, kappa=0.1, n_checks=3,
+                   dense=2, grow=8, rough=0.15, top_probe=True, full_below=2000, escape_max=0.01, max_rounds=60,
+                   seed=11, **solver_kw):
     """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
 
     Each curve starts from n_initial solver points spread evenly over the
     grid and is interpolated with PCHIP in (log B, asinh(sigma / S0)).  Every
     gap between consecutive solver points is resolved by a solver run at its
     midpoint, which is also a check: the gap is done when the interpolant
-    missed that point by at most safety * tol, otherwise both halves go on.
+    missed that point by at most safety * tol, otherwise both halves go on; a
+    gap of at most 2 * dense grid steps is solved at every point instead.
     With a surrogate, a gap is done without a solver run when the curve of
     every ensemble member, interpolated through the same points, stays within
     kappa * tol of that member over the gap: the surrogate predicts where
     interpolation fails, and members that disagree on the curve's shape (its
-    uncertainty) keep the solver run.  A curve is accepted once n_checks solver runs at randomly drawn
-    unsolved points all lie within tol of it; a failed check reopens the gaps
-    beside it.  Each round runs the solver once, in one batch, for every
-    point any unfinished curve asks for.  A material whose start positions
-    all lie inside impurities has sigma = 0 at every B (that test does not
-    depend on B): its curve is zero after the first round.
+    uncertainty) keep the solver run.  It does so only where the solver runs
+    at both ends of the gap had at most escape_max of their particles end
+    without a hit: such orbits make sigma step between neighbouring fields,
+    which a smooth surrogate cannot foresee.
+
+    The solver's sigma is not smooth everywhere: with few free start
+    positions, or where many orbits close a full revolution without a hit
+    (lifetime 0, lifetime.h:131-146), one grid step of B can change it by
+    more than tol.  No interpolant predicts that, so such stretches are solved
+    at every point: a solved point whose two grid neighbours are solved and
+    which the local interpolant through the solved points beside it misses by
+    more than rough * tol of |sigma| marks the curve rough there, and every unsolved point within `grow` steps of it is solved
+    next round (which repeats until the stretch's ends are smooth); the
+    surrogate's curves are smooth, so it no longer resolves gaps of a rough
+    curve.  The first round also solves the two fields below the last one
+    (`top_probe`): on sparse fields the roughness sits at high B, where orbits
+    are small.  A material with fewer than `full_below` free start positions
+    is solved at every field: its steps are large (few particles carry sigma)
+    and its runs cheap (work is proportional to the free positions).
+
+    A curve is accepted once n_checks solver runs at randomly drawn unsolved
+    points all lie within tol of it; a failed check reopens the gaps beside it
+    and solves its neighbourhood.  Each round runs the solver once, in one
+    batch, for every point any unfinished curve asks for.  A material whose
+    start positions all lie inside impurities has sigma = 0 at every B (that
+    test does not depend on B): its curve is zero after the first round.
     """
     rng = np.random.default_rng(seed)
     B = np.asarray(B, dtype=np.float64)
@@ -405,16 +441,43 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     reuse = solver_kw.pop("reuse", None)
     own = reuse is None
     reuse = {} if own else reuse
+    if top_probe and n >= 4:  # two more points beside the last: a second difference at the high-field end
+        init = sorted(set(init) | {n - 3, n - 2})
+    opts = dict(tol=tol, safety=safety, kappa=kappa, n_checks=n_checks, dense=dense, grow=grow, rough=rough,
+                full_below=full_below, escape_max=escape_max)
     try:
-        return _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_rounds,
-                             dict(solver_kw, reuse=reuse))
+        return _sweep_rounds(curves, B, u, n, init, rng, max_rounds, opts, dict(solver_kw, reuse=reuse))
     finally:
         if own and reuse.get("batch") is not None:
             reuse["batch"].close()
 
 
-def _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_rounds, solver_kw):
+def _rough_points(c, points, n, tol_r):
+    """Indices among `points` (and their grid neighbours) with both grid
+    neighbours solved whose value the local interpolant (PCHIP through up to 3
+    solved points on each side, the point itself left out) misses by more than
+    tol_r of |sigma|: smooth curvature is followed at this spacing, a step of
+    the solver's sigma between neighbouring fields is not."""
+    rough = []
+    cand = {k for i in points for k in (i - 1, i, i + 1) if 0 < k < n - 1}
+    if not cand:
+        return rough
+    idx = np.array(sorted(c.solved))
+    for k in sorted(cand):
+        if not (k in c.solved and k - 1 in c.solved and k + 1 in c.solved):
+            continue
+        j = int(np.searchsorted(idx, k))
+        loc = np.concatenate([idx[max(0, j - 3):j], idx[j + 1:j + 4]])
+        vals = np.array([c.solved[i] for i in loc])
+        est = _curve_from(c.u, loc, vals, at=np.array([k]))[0]
+        if tensor_error(est, c.solved[k]) > tol_r:
+            rough.append(k)
+    return rough
+
+
+def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
     """The rounds of adaptive_sweep; every solver call shares one batch (solver_kw["reuse"])."""
+    tol, safety, kappa = o["tol"], o["safety"], o["kappa"]
     solver_s = 0.0
     items = 0
     rounds = 0
@@ -430,13 +493,22 @@ def _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_
                 c.pending = [(i, None) for i in init]
             elif c.phase == "refine":
                 gaps = sorted(c.active)
-                if c.prior is not None and gaps:
+                # the surrogate may close a gap only where few orbits end without a hit
+                calm = [g for g in gaps if max(c.escaped.get(g[0], 1.0), c.escaped.get(g[1], 1.0)) <= o["escape_max"]]
+                if c.prior is not None and calm:
                     idx = np.array(sorted(c.solved))
-                    keep = _prior_errors(u, idx, c.prior, gaps) > kappa * tol
+                    keep = _prior_errors(u, idx, c.prior, calm) > kappa * tol
                     c.skipped += int(np.count_nonzero(~keep))
-                    gaps = [g for g, k in zip(gaps, keep) if k]
-                c.pending = [((a + b) // 2, (a, b)) for a, b in gaps]
-                c.active = set()
+                    closed = {g for g, k in zip(calm, keep) if not k}
+                    gaps = [g for g in gaps if g not in closed]
+                want = {i: None for i in c.dense if i not in c.solved}
+                for a, b in gaps:
+                    if b - a <= 2 * o["dense"]:
+                        want.update({k: None for k in range(a + 1, b)})
+                    else:
+                        want[(a + b) // 2] = (a, b)
+                c.pending = sorted(want.items())
+                c.active, c.dense = set(), set()
                 if not c.pending:
                     c.phase = "check"
             if c.phase == "check":
@@ -444,62 +516,80 @@ def _sweep_rounds(curves, B, u, n, init, rng, tol, safety, kappa, n_checks, max_
                 if free.size == 0:
                     c.phase, c.accepted = "done", True
                     continue
-                pts = rng.choice(free, size=min(n_checks, free.size), replace=False)
+                pts = rng.choice(free, size=min(o["n_checks"], free.size), replace=False)
                 c.pending = [(int(i), None) for i in sorted(pts)]
             if c.pending:
                 ask.append(ci)
         if not ask:
             break
         rounds += 1
-        sig, secs, it, free_pos = solver_runs([curves[ci].mat for ci in ask],
-                                              [B[[i for i, _ in curves[ci].pending]] for ci in ask],
-                                              with_free=True, **solver_kw)
+        sig, secs, it, free_pos, escaped = solver_runs([curves[ci].mat for ci in ask],
+                                                       [B[[i for i, _ in curves[ci].pending]] for ci in ask],
+                                                       with_free=True, **solver_kw)
         solver_s += secs
         items += it
-        for ci, s, nf in zip(ask, sig, free_pos):
+        for ci, s, nf, ef in zip(ask, sig, free_pos, escaped):
             c = curves[ci]
             vals = s[:, 0, :]
+            c.escaped.update({i: float(e) for (i, _), e in zip(c.pending, ef)})
             if c.phase == "init":
                 for (i, _), v in zip(c.pending, vals):
                     c.solved[i] = v
                 if nf == 0:
                     c.phase, c.accepted = "done", True
                     continue
+                if nf < o["full_below"]:
+                    # few free start positions: large steps, cheap runs; every field is solved
+                    c.dense = {k for k in range(n) if k not in c.solved}
+                    c.prior = None
                 s_idx = sorted(c.solved)
                 c.active = {(a, b) for a, b in zip(s_idx[:-1], s_idx[1:]) if b - a > 1}
                 c.phase = "refine"
+                for k in _rough_points(c, s_idx, n, o["rough"] * tol):
+                    c.rough += 1
+                    c.grow(k, o["grow"], n)
+                    c.prior = None
                 continue
             est = c.estimate()
             errs = tensor_error(est[[i for i, _ in c.pending]], vals)
             for (i, _), v in zip(c.pending, vals):
                 c.solved[i] = v
+            new = [i for i, _ in c.pending]
+            rough = _rough_points(c, new, n, o["rough"] * tol)
+            c.rough += len(rough)
+            for k in rough:
+                c.grow(k, o["grow"], n)
+            if rough:
+                c.prior = None  # the surrogate's curves are smooth: it cannot vouch for a rough one
             if c.phase == "refine":
                 for (i, gap), e in zip(c.pending, errs):
-                    if e > tol * safety:
+                    if gap is not None and e > tol * safety:
                         a, b = gap
                         c.active |= {g for g in ((a, i), (i, b)) if g[1] - g[0] > 1}
-                if not c.active:
+                if not c.active and not c.dense:
                     c.phase = "check"
             else:  # check
                 c.checks += [(i, float(e)) for (i, _), e in zip(c.pending, errs)]
-                if np.all(errs <= tol):
+                if np.all(errs <= tol) and not c.dense:
                     c.phase, c.accepted = "done", True
                 else:
                     s_idx = sorted(c.solved)
                     for (i, _), e in zip(c.pending, errs):
                         if e > tol:
+                            c.grow(i, o["grow"], n)
                             k = s_idx.index(i)
                             for g in ((s_idx[k - 1], i) if k > 0 else None,
                                       (i, s_idx[k + 1]) if k + 1 < len(s_idx) else None):
                                 if g and g[1] - g[0] > 1:
                                     c.active.add(g)
-                    c.phase = "refine" if c.active else "check"
+                    c.phase = "refine" if (c.active or c.dense) else "check"
     est = np.stack([c.estimate() for c in curves])
     report = dict(
         rounds=rounds, curves=len(curves), accepted=int(sum(c.accepted for c in curves)),
         solver_points=int(sum(len(c.solved) for c in curves)),
         points_per_curve=float(np.mean([len(c.solved) for c in curves])),
         gaps_skipped=int(sum(c.skipped for c in curves)),
+        rough_points=int(sum(c.rough for c in curves)),
         max_check_error=float(max((e for c in curves for _, e in c.checks), default=0.0)),
         solver_seconds=solver_s, traced_items=int(items), wall_seconds=time.time() - t0,
     )

@@ -101,3 +101,18 @@ def test_mixed_low_density_many_cells(es):
     announce("mixed vs fp64: 400 impurities on 2e-6 m, 128x128 cells")
     _compare(es, lambda s: s.generate_impurities(7, 1e-6, 5e-7, 1e-8, count=400, cells_per_row=128),
              48, 9, 1e-12, 0.5, 0, 7e5, 1, [0.1, 2.0, 30.0], [10.0])
+
+
+@pytest.mark.parametrize("material, fields", [(92, [0.11009727, 0.11187666]), (36, [0.1])])
+def test_mixed_first_exit_at_the_start(es, material, fields):
+    """Regression: a start point 5e-12 m from a cell edge at the bottom of a
+    ~180 um orbit (phi = 0): the first exit lies ~3e-8 rad along, where the
+    FP64 path's exit angle equals phi and GetNextCell finds no next cell; the
+    FP32 trace hands such exits to FP64."""
+    from elasticscattering_amd import surrogate as S
+
+    m = S.draw_materials(200, seed=77)[material]
+    announce(f"mixed vs fp64: material {material} of the 200-material sweep, first exit at the start")
+    _compare(es, lambda s: s.generate_impurities(m["seed"], 1e-6, 5e-7, m["radius"], density=m["density"],
+                                                 target_cell_population=4),
+             128, 65, m["tau"], m["alpha"], 1, 7e5, 0, fields, [m["T"]])

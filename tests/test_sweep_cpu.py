@@ -33,8 +33,9 @@ def fake_solver(monkeypatch):
             v = np.zeros((len(f), 2)) if m["zero"] else _drude(m, np.asarray(f))
             out.append(v[:, None, :])
         calls.append(sum(len(f) for f in fields))
-        free = [0 if m["zero"] else 1000 for m in materials]
-        return (out, 0.0, 0, free) if with_free else (out, 0.0, 0)
+        free = [0 if m["zero"] else 5000 for m in materials]
+        esc = [np.zeros(len(f)) for f in fields]
+        return (out, 0.0, 0, free, esc) if with_free else (out, 0.0, 0)
 
     monkeypatch.setattr(S, "solver_runs", run)
     return calls
