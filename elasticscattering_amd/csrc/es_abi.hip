@@ -62,6 +62,11 @@ struct Ctx {
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
     void* hbuf = nullptr;             // pinned host staging of the stage tables and results
     size_t hbuf_cap = 0;
+    DevBuf jobs;                      // binning / start-test job tables (device)
+    void* hjobs = nullptr;            // their pinned staging
+    size_t hjobs_cap = 0;
+    hipEvent_t jobs_ev = nullptr;     // recorded after the last job-table upload
+    bool jobs_copy_pending = false;
 };
 
 
@@ -169,6 +174,7 @@ int ctx_init(Ctx& c, int device, std::string& err)
         e = hipEventCreate(&ev);
         if (e == hipSuccess) c.ev.push_back(ev);
     }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.jobs_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         err = std::string("HIP setup failed: ") + hipGetErrorString(e);
@@ -187,6 +193,12 @@ void ctx_destroy(Ctx& c)
     if (c.hbuf) (void)hipHostFree(c.hbuf);
     c.hbuf = nullptr;
     c.hbuf_cap = 0;
+    release(c, c.jobs);
+    if (c.hjobs) (void)hipHostFree(c.hjobs);
+    c.hjobs = nullptr;
+    c.hjobs_cap = 0;
+    if (c.jobs_ev) (void)hipEventDestroy(c.jobs_ev);
+    c.jobs_ev = nullptr;
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -303,11 +315,67 @@ int field_sync(Ctx& c, Field& f, bool pos_errors = true)
     return field_sync_finish(c, f, pos_errors);
 }
 
+// Binnings and start-cell tests waiting to be launched together (one launch
+// per stage for all their fields), and the fields whose read-backs they fill.
+struct Jobs {
+    std::vector<es::BinJob> bin;
+    std::vector<es::PosJob> pos;
+    std::vector<Field*> fields;
+    bool empty() const { return bin.empty() && pos.empty(); }
+    void add_field(Field* f)
+    {
+        if (std::find(fields.begin(), fields.end(), f) == fields.end()) fields.push_back(f);
+    }
+};
+
+// Uploads the job tables (one copy from pinned staging), launches the
+// binning and then the start tests, and enqueues each field's read-back copy
+// behind them; field_sync_finish takes the read-backs after the next wait.
+int jobs_launch(Ctx& c, Jobs& q)
+{
+    if (q.empty()) return ES_OK;
+    const size_t bb = q.bin.size() * sizeof(es::BinJob), pb = q.pos.size() * sizeof(es::PosJob);
+    const size_t off = (bb + 255) & ~(size_t)255, bytes = off + pb;
+    int rc;
+    if ((rc = ensure(c, c.jobs, bytes, "binning job table"))) return rc;
+    // The staging may still feed the previous upload: wait for that copy first.
+    if (c.jobs_copy_pending) ES_HIP(c, hipEventSynchronize(c.jobs_ev));
+    if (c.hjobs_cap < bytes) {
+        if (c.hjobs) (void)hipHostFree(c.hjobs);
+        c.hjobs = nullptr;
+        c.hjobs_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 16 * 1024);
+        hipError_t e = hipHostMalloc(&c.hjobs, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            c.hjobs = nullptr;
+            return fail(c, ES_ERR_OOM, std::string("hipHostMalloc failed for the job table: ") + hipGetErrorString(e));
+        }
+        c.hjobs_cap = cap;
+    }
+    char* h = static_cast<char*>(c.hjobs);
+    if (bb) std::memcpy(h, q.bin.data(), bb);
+    if (pb) std::memcpy(h + off, q.pos.data(), pb);
+    ES_HIP(c, hipMemcpyAsync(c.jobs.p, h, bytes, hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipEventRecord(c.jobs_ev, c.stream));
+    c.jobs_copy_pending = true;
+    const es::BinJob* dbin = P<es::BinJob>(c.jobs);
+    const es::PosJob* dpos = reinterpret_cast<const es::PosJob*>(P<char>(c.jobs) + off);
+    ES_HIP(c, es::launch_bin_jobs(dbin, q.bin.data(), (int)q.bin.size(), c.stream));
+    ES_HIP(c, es::launch_positions_jobs(dpos, q.pos.data(), (int)q.pos.size(), c.stream));
+    for (Field* f : q.fields) {
+        f->copy_queued = false;
+        if ((rc = field_sync_enqueue(c, *f))) return rc;
+    }
+    q = Jobs();
+    return ES_OK;
+}
+
 // Bins the n centres in f.in_x / f.in_y (Grid.cpp:48-59, 77-180), drawing them
-// first when `generate` (key, range [lo, hi)).  Four launches, no wait: the
-// indexed count is read back by field_sync (at once when `out` is wanted).
-int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* out, bool generate = false,
-              uint64_t key = 0, double lo = 0.0, double hi = 0.0)
+// first when `generate` (key, range [lo, hi)): appends the job to q.  The
+// indexed count is read back by field_sync after jobs_launch.
+int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, Jobs& q, bool generate = false, uint64_t key = 0,
+              double lo = 0.0, double hi = 0.0)
 {
     if (d->target_cell_population <= 0 && d->cells_per_row <= 0)
         return fail(c, ES_ERR_INVALID_ARG, "target_cell_population must be > 0");
@@ -341,14 +409,29 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* 
     if ((rc = field_small(c, f))) return rc;
     // The counts start at zero: the scan of every binning leaves them so.
     if (!f.counts_clean) ES_HIP(c, hipMemsetAsync(f.b_counts.p, 0, f.b_counts.cap, c.stream));
-    f.counts_clean = false;
     g.cell_end = P<int32_t>(f.cell_end);
     g.ixy = P<double2>(f.ixy);
-    es::BinScratch bs{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts), P<int32_t>(f.b_fill),
-                      P<int64_t>(f.b_begin),   P<int64_t>(f.b_bsum),   P<int32_t>(f.b_order)};
-    int64_t* total_dev = P<int64_t>(f.small) + 1;
-    ES_HIP(c, es::launch_bin(P<double>(f.in_x), P<double>(f.in_y), n, generate ? 1 : 0, key, lo, hi, g, bs, total_dev,
-                             c.stream));
+    es::BinJob j{};
+    j.x = P<double>(f.in_x);
+    j.y = P<double>(f.in_y);
+    j.n = n;
+    j.generate = generate ? 1 : 0;
+    j.key = key;
+    j.gen_lo = lo;
+    j.gen_hi = hi;
+    j.lo = g.lo;
+    j.size = g.size;
+    j.cell_size = g.cell_size;
+    j.radius = g.radius;
+    j.cpr = cpr;
+    j.s = es::BinScratch{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts),
+                         P<int32_t>(f.b_fill),    P<int64_t>(f.b_begin),  P<int64_t>(f.b_bsum),
+                         P<int32_t>(f.b_order)};
+    j.cell_end = g.cell_end;
+    j.ixy = g.ixy;
+    j.total_out = P<int64_t>(f.small) + 1;
+    q.bin.push_back(j);
+    q.add_field(&f);
     f.counts_clean = true;
     f.total_pending = true;
     f.copy_queued = false;
@@ -356,21 +439,10 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, es_grid_info* 
     f.has_grid = true;
     f.pos_valid = false;
     f.pos_pending = false;
-    if (out) {
-        if ((rc = field_sync(c, f))) return rc;
-        out->region_size = d->region_size;
-        out->region_extends = d->region_extends;
-        out->cell_size = g.cell_size;
-        out->indexed_impurity_count = g.total;
-        out->unique_impurity_count = n;
-        out->index_size = (int32_t)ncell;
-        out->cells_per_row = cpr;
-    }
     return ES_OK;
 }
 
-int field_upload(Ctx& c, Field& f, const double* x, const double* y, int64_t n, const es_grid_desc* d,
-                 es_grid_info* out)
+int field_upload(Ctx& c, Field& f, const double* x, const double* y, int64_t n, const es_grid_desc* d, Jobs& q)
 {
     if (!d || n < 0 || (n > 0 && (!x || !y))) return fail(c, ES_ERR_INVALID_ARG, "bad impurity arguments");
     if (n > 10000000) return fail(c, ES_ERR_INVALID_ARG, "more than 1e7 impurities (Grid.cpp:36)");
@@ -381,10 +453,10 @@ int field_upload(Ctx& c, Field& f, const double* x, const double* y, int64_t n, 
         ES_HIP(c, hipMemcpyAsync(f.in_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
         ES_HIP(c, hipMemcpyAsync(f.in_y.p, y, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     }
-    return field_bin(c, f, n, d, out);
+    return field_bin(c, f, n, d, q);
 }
 
-int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid_desc* d, es_grid_info* out)
+int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid_desc* d, Jobs& q)
 {
     if (!d) return fail(c, ES_ERR_INVALID_ARG, "null grid description");
     const double lo = -d->region_extends, hi = d->region_size + d->region_extends;
@@ -400,13 +472,16 @@ int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid
     int rc;
     if ((rc = ensure(c, f.in_x, n * sizeof(double), "impurity input x"))) return rc;
     if ((rc = ensure(c, f.in_y, n * sizeof(double), "impurity input y"))) return rc;
-    return field_bin(c, f, n, d, out, true, esr_key(seed), lo, hi);
+    return field_bin(c, f, n, d, q, true, esr_key(seed), lo, hi);
 }
 
-// Launches the start cell + inside test for every position of a ppr x ppr
-// grid; field_sync collects the result.
-int field_positions_launch(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
+// Appends the start cell + inside test of a ppr x ppr position grid to q
+// unless the field's current (or pending) test is of the same positions;
+// field_sync collects the result after jobs_launch.
+int field_positions_job(Ctx& c, Field& f, int ppr, double dist, double offx, double offy, Jobs& q)
 {
+    const bool same = f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy;
+    if (same && (f.pos_valid || f.pos_pending)) return ES_OK;
     const int64_t n = (int64_t)ppr * ppr;
     int rc;
     if ((rc = ensure(c, f.flags, n, "position flags"))) return rc;
@@ -414,8 +489,24 @@ int field_positions_launch(Ctx& c, Field& f, int ppr, double dist, double offx, 
     if ((rc = ensure(c, f.bcount, ((n + 255) / 256) * sizeof(int32_t), "position block counts"))) return rc;
     if ((rc = field_small(c, f))) return rc;
     int32_t* small = P<int32_t>(f.small);
-    ES_HIP(c, es::launch_positions(f.g, ppr, dist, offx, offy, P<uint8_t>(f.flags), P<int32_t>(f.bcount), small,
-                                   P<int32_t>(f.free_pos), small + 1, c.stream));
+    es::PosJob j{};
+    j.ixy = f.g.ixy;
+    j.cell_end = f.g.cell_end;
+    j.lo = f.g.lo;
+    j.size = f.g.size;
+    j.radius = f.g.radius;
+    j.cpr = f.g.cpr;
+    j.ppr = ppr;
+    j.dist = dist;
+    j.offx = offx;
+    j.offy = offy;
+    j.flags = P<uint8_t>(f.flags);
+    j.bcount = P<int32_t>(f.bcount);
+    j.err = small;
+    j.free_pos = P<int32_t>(f.free_pos);
+    j.nfree = small + 1;
+    q.pos.push_back(j);
+    q.add_field(&f);
     f.pos_valid = false;
     f.pos_pending = true;
     f.copy_queued = false;
@@ -432,21 +523,11 @@ int field_positions(Ctx& c, Field& f, int ppr, double dist, double offx, double 
     if (f.pos_valid && !f.pos_pending && f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx &&
         f.pos_offy == offy && !f.total_pending)
         return ES_OK;
+    Jobs q;
     int rc;
-    if ((rc = field_positions_launch(c, f, ppr, dist, offx, offy))) return rc;
+    if ((rc = field_positions_job(c, f, ppr, dist, offx, offy, q))) return rc;
+    if ((rc = jobs_launch(c, q))) return rc;
     return field_sync(c, f);
-}
-
-// The same without the wait: the start test (when the positions are new) and
-// the copy of the read-backs are enqueued; field_sync_finish takes them after
-// the next stream synchronisation.
-int field_positions_deferred(Ctx& c, Field& f, int ppr, double dist, double offx, double offy)
-{
-    const bool same = f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy;
-    int rc;
-    if (!(same && (f.pos_valid || f.pos_pending)))
-        if ((rc = field_positions_launch(c, f, ppr, dist, offx, offy))) return rc;
-    return field_sync_enqueue(c, f);
 }
 
 // ----------------------------------------------------------------- stages
@@ -973,6 +1054,7 @@ struct es_batch {
     };
     std::vector<Field*> fields;
     std::vector<Recipe> recipes;     // per field
+    Jobs pending;                    // binnings and start tests of added entries, launched together
     std::vector<Entry> entries;
     Stages s;
     Shape shape;
@@ -1148,7 +1230,7 @@ const char* es_last_error(const es_solver* s) { return s ? s->c.err.c_str() : g_
 // when the solver was created without one, SimulationRunner.cpp:75-84), the
 // start test of its positions launched behind the binning, and one wait for
 // both read-backs.
-int solver_field_ready(es_solver* s, const es_grid_desc* desc, es_grid_info* out_info)
+int solver_field_ready(es_solver* s, const es_grid_desc* desc, es_grid_info* out_info, Jobs& q)
 {
     Field& f = s->f;
     es_grid_info info{};
@@ -1164,10 +1246,11 @@ int solver_field_ready(es_solver* s, const es_grid_desc* desc, es_grid_info* out
     const int ppr = s->desc.positions_per_row;
     if (s->desc.grid_info.region_size > 0) {
         const double dist = s->desc.grid_info.region_size / (double)(ppr - 1);  // Simulation.h:142-143
-        if ((rc = field_positions_launch(s->c, f, ppr, dist, s->desc.grid_info.cell_size * 0.01,
-                                         s->desc.grid_info.cell_size * 0.005)))
+        if ((rc = field_positions_job(s->c, f, ppr, dist, s->desc.grid_info.cell_size * 0.01,
+                                      s->desc.grid_info.cell_size * 0.005, q)))
             return rc;
     }
+    if ((rc = jobs_launch(s->c, q))) return rc;
     if ((rc = field_sync(s->c, f, false))) return rc;
     info.indexed_impurity_count = f.g.total;
     if (adopt) s->desc.grid_info.indexed_impurity_count = f.g.total;
@@ -1180,18 +1263,20 @@ int es_grid_set_impurities(es_solver* s, const double* x, const double* y, int64
 {
     if (!s) return ES_ERR_INVALID_ARG;
     ES_HIP(s->c, hipSetDevice(s->c.device));
-    const int rc = field_upload(s->c, s->f, x, y, n, desc, nullptr);
+    Jobs q;
+    const int rc = field_upload(s->c, s->f, x, y, n, desc, q);
     if (rc) return rc;
-    return solver_field_ready(s, desc, out_info);
+    return solver_field_ready(s, desc, out_info, q);
 }
 
 int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_desc* desc, es_grid_info* out_info)
 {
     if (!s) return ES_ERR_INVALID_ARG;
     ES_HIP(s->c, hipSetDevice(s->c.device));
-    const int rc = field_generate(s->c, s->f, seed, count, desc, nullptr);
+    Jobs q;
+    const int rc = field_generate(s->c, s->f, seed, count, desc, q);
     if (rc) return rc;
-    return solver_field_ready(s, desc, out_info);
+    return solver_field_ready(s, desc, out_info, q);
 }
 
 int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index)
@@ -1284,6 +1369,8 @@ int batch_plan(es_batch* bt)
     Plan& pl = bt->plan;
     pl = Plan();
     if (bt->entries.empty()) return fail(c, ES_ERR_INVALID_ARG, "empty batch");
+    int jrc;
+    if ((jrc = jobs_launch(c, bt->pending))) return jrc;
     // The read-backs of the entries' fields still in flight: one wait for all.
     std::vector<int> pending;
     for (const es_batch::Entry& en : bt->entries) {
@@ -1501,7 +1588,7 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
             // es_batch_counts) collects the read-backs of every new field at once.
             Field* f = spare >= 0 ? bt->fields[spare] : new Field();
             if (spare >= 0) bt->recipes[spare].valid = false;
-            rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, nullptr);
+            rc = field_generate(c, *f, smp->seed, smp->impurity_count, &smp->grid, bt->pending);
             if (rc) {
                 if (spare < 0) {
                     field_release(c, *f);
@@ -1533,7 +1620,7 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
     en.B.assign(smp->fields, smp->fields + smp->n_fields);
     en.T.assign(smp->temperatures, smp->temperatures + smp->n_temperatures);
     if ((rc = taus_for(c, en.st, en.coherent, en.T.data(), (int)en.T.size(), en.taus))) return rc;
-    if ((rc = field_positions_deferred(c, f, en.ppr, en.dist, en.offx, en.offy))) return rc;
+    if ((rc = field_positions_job(c, f, en.ppr, en.dist, en.offx, en.offy, bt->pending))) return rc;
     bt->entries.push_back(en);
     bt->plan_ready = false;
     if (index) *index = (int32_t)bt->entries.size() - 1;
@@ -1543,6 +1630,9 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
 int es_batch_clear(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    const int rc = jobs_launch(bt->c, bt->pending);  // kept fields stay consistent
+    if (rc) return rc;
     bt->entries.clear();
     bt->plan_ready = false;
     batch_release_graph(bt);
@@ -1552,6 +1642,9 @@ int es_batch_clear(es_batch* bt)
 int es_batch_reset(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    const int rc = jobs_launch(bt->c, bt->pending);
+    if (rc) return rc;
     bt->entries.clear();
     bt->plan_ready = false;
     batch_release_graph(bt);

@@ -118,15 +118,45 @@ struct BinScratch {
     int64_t* bsum;      // per scan block
     int32_t* order;     // per indexed impurity
 };
-// Generates the n centres into x, y first when `generate` (es_rng.h, key, range
-// [lo, hi)); writes the indexed count to *total_out (device memory).
-hipError_t launch_bin(double* x, double* y, int64_t n, int generate, uint64_t key, double lo, double hi, GridDev& g,
-                      BinScratch& s, int64_t* total_out, hipStream_t st);
 
-// Start-cell test per position (lifetime.h:84-99) and ordered compaction.
-// bcount: one int per 256 positions.
-hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
-                            int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st);
+// One field to bin: its centres (drawn first from es_rng.h with `key` over
+// [gen_lo, gen_hi) when `generate`), geometry, scratch and outputs.
+struct BinJob {
+    double* x;
+    double* y;
+    int64_t n;
+    int generate;
+    uint64_t key;
+    double gen_lo, gen_hi;
+    double lo, size, cell_size, radius;
+    int cpr;
+    BinScratch s;
+    int32_t* cell_end;
+    double2* ixy;
+    int64_t* total_out;  // indexed count (device memory)
+};
+
+// One start-cell test (lifetime.h:84-99) of a ppr x ppr position grid on a
+// binned field, with the ordered list of free positions.
+struct PosJob {
+    const double2* ixy;
+    const int32_t* cell_end;
+    double lo, size, radius;
+    int cpr;
+    int ppr;
+    double dist, offx, offy;
+    uint8_t* flags;      // per position
+    int32_t* bcount;     // per 256 positions: free count, bit 31 = a position outside the grid
+    int32_t* err;        // out: 1 when a position lies outside the cell grid
+    int32_t* free_pos;
+    int32_t* nfree;      // out
+};
+
+// Bins every job's field with one launch per stage for all of them; `jobs_dev`
+// is the job table in device memory, `jobs` the same table on the host.
+hipError_t launch_bin_jobs(const BinJob* jobs_dev, const BinJob* jobs, int njobs, hipStream_t st);
+// Start-cell tests of every job (after the binning of their fields).
+hipError_t launch_positions_jobs(const PosJob* jobs_dev, const PosJob* jobs, int njobs, hipStream_t st);
 
 // Trace kernel variants: register budget 2 or 3 waves per SIMD; coop = 0 for
 // one lane per particle throughout, 1 for wave-cooperative crossing solves, 2

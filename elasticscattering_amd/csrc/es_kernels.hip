@@ -173,28 +173,33 @@ __device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
     return n;
 }
 
+// Every kernel of the binning and of the start-cell test works on a table of
+// jobs, one field each (blockIdx.y, or blockIdx.x for the one-block scan): a
+// batch of new fields is binned with one launch per stage.
+
 // Targets and per-cell counts of every impurity; with `generate` the centres
 // are drawn first (es_rng.h) and stored in x, y.  counts are zero on entry.
-__global__ void k_bin_count(double* x, double* y, int64_t n, BinGeom g, int generate, uint64_t key, double lo,
-                            double hi, int32_t* targets, uint8_t* ntarg, int32_t* counts)
+__global__ __launch_bounds__(256) void k_bin_count(const BinJob* jobs)
 {
+    const BinJob& J = jobs[blockIdx.y];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= J.n) return;
     double px, py;
-    if (generate) {
-        esr_impurity(key, (uint64_t)i, lo, hi, &px, &py);
-        x[i] = px;
-        y[i] = py;
+    if (J.generate) {
+        esr_impurity(J.key, (uint64_t)i, J.gen_lo, J.gen_hi, &px, &py);
+        J.x[i] = px;
+        J.y[i] = py;
     } else {
-        px = x[i];
-        py = y[i];
+        px = J.x[i];
+        py = J.y[i];
     }
+    const BinGeom g{J.lo, J.size, J.cell_size, J.radius, J.cpr};
     int32_t t[9];
     const int k = bin_targets(g, px, py, t);
-    ntarg[i] = (uint8_t)k;
+    J.s.ntarg[i] = (uint8_t)k;
     for (int m = 0; m < k; m++) {
-        targets[9 * i + m] = t[m];
-        atomicAdd(&counts[t[m]], 1);
+        J.s.targets[9 * i + m] = t[m];
+        atomicAdd(&J.s.counts[t[m]], 1);
     }
 }
 
@@ -232,54 +237,63 @@ __device__ __forceinline__ int64_t block_prefix_of(const int64_t* bsum, int nb, 
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_scan_block(const int32_t* counts, int64_t n, int64_t* begin, int64_t* bsum)
+// Two-level scan for one job of a large cell grid (more than kScanOneCells).
+__global__ __launch_bounds__(256) void k_scan_block(const BinJob* job)
 {
+    const BinJob& J = *job;
+    const int64_t n = (int64_t)J.cpr * J.cpr;
     __shared__ int64_t sh[256];
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t v = (c < n) ? counts[c] : 0;
+    const int64_t v = (c < n) ? J.s.counts[c] : 0;
     int64_t total;
     const int64_t ex = block_exclusive_scan(v, sh, total);
-    if (c < n) begin[c] = ex;
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    if (c < n) J.s.begin[c] = ex;
+    if (threadIdx.x == 0) J.s.bsum[blockIdx.x] = total;
 }
 
 // Second level of the two-level scan; also clears the fill counters for
 // k_bin_fill and the counts for the next binning, and writes the total.
-__global__ __launch_bounds__(256) void k_scan_add(int32_t* counts, int64_t n, int64_t* begin, const int64_t* bsum,
-                                                  int32_t* incl_end, int32_t* fill, int64_t* total_out)
+__global__ __launch_bounds__(256) void k_scan_add(const BinJob* job)
 {
+    const BinJob& J = *job;
+    const int64_t n = (int64_t)J.cpr * J.cpr;
     __shared__ int64_t sh[256];
-    const int64_t off = block_prefix_of(bsum, blockIdx.x, sh);
+    const int64_t off = block_prefix_of(J.s.bsum, blockIdx.x, sh);
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c < n) {
-        const int64_t b = begin[c] + off;
-        const int32_t cnt = counts[c];
-        begin[c] = b;
-        incl_end[c] = (int32_t)(b + cnt);
-        fill[c] = 0;
-        counts[c] = 0;
+        const int64_t b = J.s.begin[c] + off;
+        const int32_t cnt = J.s.counts[c];
+        J.s.begin[c] = b;
+        J.cell_end[c] = (int32_t)(b + cnt);
+        J.s.fill[c] = 0;
+        J.s.counts[c] = 0;
         if (c == n - 1) {
-            begin[n] = b + cnt;
-            *total_out = b + cnt;
+            J.s.begin[n] = b + cnt;
+            *J.total_out = b + cnt;
         }
     }
 }
 
-// The same scan in one block of 1024 threads for cell grids up to
-// kScanOneCells cells, staged in LDS: coalesced loads of the counts, each
-// thread sums a contiguous run, the block scans the run sums, the runs are
-// turned into exclusive offsets in place, and coalesced stores write them out.
+// The same scan in one block of 1024 threads per job (blockIdx.x) for cell
+// grids up to kScanOneCells cells, staged in LDS: coalesced loads of the
+// counts, each thread sums a contiguous run, the block scans the run sums, the
+// runs are turned into exclusive offsets in place, and coalesced stores write
+// them out.  Jobs with larger grids are left to the two-level scan.
 constexpr int kScanOneCells = 32768;
 
-__global__ __launch_bounds__(1024) void k_scan_one(int32_t* counts, int64_t n, int64_t* begin, int32_t* incl_end,
-                                                   int32_t* fill, int64_t* total_out)
+__global__ __launch_bounds__(1024) void k_scan_one(const BinJob* jobs)
 {
+    const BinJob& J = jobs[blockIdx.x];
+    const int64_t n = (int64_t)J.cpr * J.cpr;
+    if (n > kScanOneCells) return;
     extern __shared__ int32_t sh_cnt[];  // [n]
     __shared__ int64_t sh[1024];
     const int tid = threadIdx.x;
     const int ni = (int)n;
+    int32_t* __restrict__ counts = J.s.counts;
+    int32_t* __restrict__ fill = J.s.fill;
+    for (int c = tid; c < ni; c += 1024) sh_cnt[c] = counts[c];
     for (int c = tid; c < ni; c += 1024) {
-        sh_cnt[c] = counts[c];
         counts[c] = 0;
         fill[c] = 0;
     }
@@ -305,25 +319,26 @@ __global__ __launch_bounds__(1024) void k_scan_one(int32_t* counts, int64_t n, i
     }
     __syncthreads();
     for (int c = tid; c < ni; c += 1024) {
-        begin[c] = sh_cnt[c];
-        incl_end[c] = (c + 1 < ni) ? sh_cnt[c + 1] : (int32_t)total;
+        J.s.begin[c] = sh_cnt[c];
+        J.cell_end[c] = (c + 1 < ni) ? sh_cnt[c + 1] : (int32_t)total;
     }
     if (tid == 0) {
-        begin[n] = total;
-        *total_out = total;
+        J.s.begin[n] = total;
+        *J.total_out = total;
     }
 }
 
-__global__ void k_bin_fill(int64_t n, const int32_t* targets, const uint8_t* ntarg, const int64_t* begin,
-                           int32_t* fill, int32_t* order)
+__global__ __launch_bounds__(256) void k_bin_fill(const BinJob* jobs)
 {
+    const BinJob& J = jobs[blockIdx.y];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int k = ntarg[i];
+    if (i >= J.n) return;
+    const int k = J.s.ntarg[i];
+    int32_t c[9];
+    for (int m = 0; m < k; m++) c[m] = J.s.targets[9 * i + m];
     for (int m = 0; m < k; m++) {
-        const int c = targets[9 * i + m];
-        const int slot = atomicAdd(&fill[c], 1);
-        order[begin[c] + slot] = (int32_t)i;
+        const int slot = atomicAdd(&J.s.fill[c[m]], 1);
+        J.s.order[J.s.begin[c[m]] + slot] = (int32_t)i;
     }
 }
 
@@ -346,16 +361,19 @@ __device__ void sift_down(int32_t* a, int64_t root, int64_t n)
 // pairs in that order.  One wave per cell; ids in a cell are distinct, so a
 // lane's rank is the count of smaller ids (a heap sort by one lane for cells
 // of more than 64).
-__global__ __launch_bounds__(256) void k_bin_sort_gather(int64_t ncell, const int64_t* begin, int32_t* order,
-                                                         const double* x, const double* y, double2* ixy)
+__global__ __launch_bounds__(256) void k_bin_sort_gather(const BinJob* jobs)
 {
+    const BinJob& J = jobs[blockIdx.y];
+    const int64_t ncell = (int64_t)J.cpr * J.cpr;
     const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (c >= ncell) return;
-    const int64_t b = begin[c];
-    int32_t* a = order + b;
-    const int64_t n = begin[c + 1] - b;
+    const int64_t b = J.s.begin[c];
+    int32_t* a = J.s.order + b;
+    const int64_t n = J.s.begin[c + 1] - b;
     if (n <= 0) return;
+    const double* x = J.x;
+    const double* y = J.y;
     if (n <= 64) {
         const int32_t v = (lane < n) ? a[lane] : INT32_MAX;
         int rank = 0;
@@ -365,7 +383,7 @@ __global__ __launch_bounds__(256) void k_bin_sort_gather(int64_t ncell, const in
         }
         if (lane < n) {
             a[rank] = v;
-            ixy[b + rank] = make_double2(x[v], y[v]);
+            J.ixy[b + rank] = make_double2(x[v], y[v]);
         }
         return;
     }
@@ -383,38 +401,46 @@ __global__ __launch_bounds__(256) void k_bin_sort_gather(int64_t ncell, const in
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int64_t k = lane; k < n; k += 64) {
         const int32_t v = a[k];
-        ixy[b + k] = make_double2(x[v], y[v]);
+        J.ixy[b + k] = make_double2(x[v], y[v]);
     }
 }
 
 // ----------------------------------------------------------------- positions
 
-__global__ __launch_bounds__(256) void k_positions(const double2* ixy, const int32_t* cell_end, double start,
-                                                   double size, double radius, int cpr, int ppr, double dist,
-                                                   double offx, double offy, uint8_t* flags, int32_t* bcount,
-                                                   int32_t* err)
+constexpr int32_t kPosErrBit = (int32_t)0x80000000;
+
+// Start cell + inside test per position; per block the free count, with
+// kPosErrBit set when a position lies outside the cell grid.
+__global__ __launch_bounds__(256) void k_positions(const PosJob* jobs)
 {
+    const PosJob& J = jobs[blockIdx.y];
     __shared__ int sh[4];
-    const int64_t pidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ int sh_err;
+    const int ppr = J.ppr;
     const int64_t n = (int64_t)ppr * ppr;
+    if ((int64_t)blockIdx.x * 256 >= n) return;  // whole block past this job's positions
+    if (threadIdx.x == 0) sh_err = 0;
+    __syncthreads();
+    const int64_t pidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint8_t f = 1;
     if (pidx < n) {
         const int i = (int)(pidx % ppr), j = (int)(pidx / ppr);
-        const double px = (double)i * dist + offx;  // SimulationCPU.cpp:114
-        const double py = (double)j * dist + offy;
-        const int cx = (int)((px - start) / size * (double)(cpr));  // cell_grid.h:52-58
-        const int cy = (int)((py - start) / size * (double)(cpr));
+        const double px = (double)i * J.dist + J.offx;  // SimulationCPU.cpp:114
+        const double py = (double)j * J.dist + J.offy;
+        const int cpr = J.cpr;
+        const int cx = (int)((px - J.lo) / J.size * (double)(cpr));  // cell_grid.h:52-58
+        const int cy = (int)((py - J.lo) / J.size * (double)(cpr));
         if (!(cx >= 0 && cx < cpr && cy >= 0 && cy < cpr)) {
             f = 2;
-            atomicOr(err, 1);
+            sh_err = 1;
         } else {
             const int ci = cy * cpr + cx;
-            const int s = (ci == 0) ? 0 : cell_end[ci - 1];
-            const int e = cell_end[ci];
-            const double r2 = radius * radius;
+            const int s = (ci == 0) ? 0 : J.cell_end[ci - 1];
+            const int e = J.cell_end[ci];
+            const double r2 = J.radius * J.radius;
             f = 0;
             for (int k = s; k < e; k++) {  // InsideImpurity, details.h:208-213
-                const double2 q = ixy[k];
+                const double2 q = J.ixy[k];
                 const double dx = px - q.x, dy = py - q.y;
                 if (r2 > (dx * dx + dy * dy)) {
                     f = 1;
@@ -422,30 +448,47 @@ __global__ __launch_bounds__(256) void k_positions(const double2* ixy, const int
                 }
             }
         }
-        flags[pidx] = f;
+        J.flags[pidx] = f;
     }
     const unsigned long long m = __ballot(pidx < n && f == 0);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = __popcll(m);
     __syncthreads();
-    if (threadIdx.x == 0) bcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (threadIdx.x == 0) J.bcount[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) | (sh_err ? kPosErrBit : 0);
 }
 
-__global__ __launch_bounds__(256) void k_compact(const uint8_t* flags, int64_t n, const int32_t* bcount, int nblocks,
-                                                 int32_t* free_pos, int32_t* nfree)
+// Ordered compaction of the free positions; the job's last block writes the
+// free count and the out-of-grid flag.
+__global__ __launch_bounds__(256) void k_compact(const PosJob* jobs)
 {
+    const PosJob& J = jobs[blockIdx.y];
+    const int64_t n = (int64_t)J.ppr * J.ppr;
+    const int nblocks = (int)((n + 255) / 256);
+    if ((int)blockIdx.x >= nblocks) return;
     __shared__ int64_t sh[256];
+    __shared__ int sh_e[256];
     __shared__ int wsum[4];
+    const bool last = (int)blockIdx.x == nblocks - 1;
     int64_t acc = 0;
-    for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) acc += bcount[k];
+    int err = 0;
+    const int upto = last ? nblocks : (int)blockIdx.x;  // the last block also ORs its own error bit
+    for (int k = threadIdx.x; k < upto; k += 256) {
+        const int32_t v = J.bcount[k];
+        if (k < (int)blockIdx.x) acc += v & ~kPosErrBit;
+        err |= (v & kPosErrBit) ? 1 : 0;
+    }
     sh[threadIdx.x] = acc;
+    sh_e[threadIdx.x] = err;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        if (threadIdx.x < off) {
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+            sh_e[threadIdx.x] |= sh_e[threadIdx.x + off];
+        }
         __syncthreads();
     }
     const int64_t base = sh[0];
     const int64_t pidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool is_free = pidx < n && flags[pidx] == 0;
+    const bool is_free = pidx < n && J.flags[pidx] == 0;
     const unsigned long long m = __ballot(is_free);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) wsum[wave] = __popcll(m);
@@ -453,9 +496,11 @@ __global__ __launch_bounds__(256) void k_compact(const uint8_t* flags, int64_t n
     int woff = 0;
     for (int w = 0; w < wave; w++) woff += wsum[w];
     const unsigned long long lower = (lane == 0) ? 0ull : (m & (~0ull >> (64 - lane)));
-    if (is_free) free_pos[base + woff + __popcll(lower)] = (int32_t)pidx;
-    if (blockIdx.x == nblocks - 1 && threadIdx.x == 0)
-        *nfree = (int32_t)(base + wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+    if (is_free) J.free_pos[base + woff + __popcll(lower)] = (int32_t)pidx;
+    if (last && threadIdx.x == 0) {
+        *J.nfree = (int32_t)(base + wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+        *J.err = sh_e[0];
+    }
 }
 
 // --------------------------------------------------------------------- trace
@@ -1785,18 +1830,22 @@ inline unsigned int blocks_for(int64_t n, int bs) { return (unsigned int)((n + b
 
 }  // namespace
 
-hipError_t launch_bin(double* x, double* y, int64_t n, int generate, uint64_t key, double lo, double hi, GridDev& g,
-                      BinScratch& s, int64_t* total_out, hipStream_t st)
+hipError_t launch_bin_jobs(const BinJob* jobs_dev, const BinJob* jobs, int njobs, hipStream_t st)
 {
-    const int64_t ncell = (int64_t)g.cpr * g.cpr;
+    if (njobs <= 0) return hipSuccess;
+    int64_t max_n = 0, max_cell = 0, max_small = 0;
+    for (int j = 0; j < njobs; j++) {
+        const int64_t nc = (int64_t)jobs[j].cpr * jobs[j].cpr;
+        max_n = std::max<int64_t>(max_n, jobs[j].n);
+        max_cell = std::max<int64_t>(max_cell, nc);
+        if (nc <= kScanOneCells) max_small = std::max<int64_t>(max_small, nc);
+    }
     hipError_t e;
-    if (n > 0) {
-        BinGeom bg{g.lo, g.size, g.cell_size, g.radius, g.cpr};
-        hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, y, n, bg, generate, key, lo,
-                           hi, s.targets, s.ntarg, s.counts);
+    if (max_n > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(max_n, 256), njobs), dim3(256), 0, st, jobs_dev);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (ncell <= kScanOneCells) {
+    if (max_small > 0) {
         static bool lds_set = false;  // allow the LDS staging beyond the 64 KB default (once per process)
         if (!lds_set) {
             if ((e = hipFuncSetAttribute((const void*)k_scan_one, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1804,38 +1853,36 @@ hipError_t launch_bin(double* x, double* y, int64_t n, int generate, uint64_t ke
                 return e;
             lds_set = true;
         }
-        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), (size_t)ncell * sizeof(int32_t), st, s.counts, ncell,
-                           s.begin, g.cell_end, s.fill, total_out);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    } else {
-        const unsigned int nb = blocks_for(ncell, 256);
-        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, s.counts, ncell, s.begin, s.bsum, g.cell_end,
-                           s.fill, total_out);
+        hipLaunchKernelGGL(k_scan_one, dim3(njobs), dim3(1024), (size_t)max_small * sizeof(int32_t), st, jobs_dev);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (n > 0) {
-        hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, s.targets, s.ntarg, s.begin,
-                           s.fill, s.order);
+    for (int j = 0; j < njobs; j++) {
+        const int64_t nc = (int64_t)jobs[j].cpr * jobs[j].cpr;
+        if (nc <= kScanOneCells) continue;
+        const unsigned int nb = blocks_for(nc, 256);
+        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, st, jobs_dev + j);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, jobs_dev + j);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bin_sort_gather, dim3(blocks_for(ncell, 4)), dim3(256), 0, st, ncell, s.begin, s.order, x, y,
-                       g.ixy);
+    if (max_n > 0) {
+        hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(max_n, 256), njobs), dim3(256), 0, st, jobs_dev);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_bin_sort_gather, dim3(blocks_for(max_cell, 4), njobs), dim3(256), 0, st, jobs_dev);
     return hipGetLastError();
 }
 
-hipError_t launch_positions(const GridDev& g, int ppr, double dist, double offx, double offy, uint8_t* flags,
-                            int32_t* bcount, int32_t* err, int32_t* free_pos, int32_t* nfree, hipStream_t st)
+hipError_t launch_positions_jobs(const PosJob* jobs_dev, const PosJob* jobs, int njobs, hipStream_t st)
 {
-    const int64_t n = (int64_t)ppr * ppr;
-    const unsigned int nb = blocks_for(n, 256);
-    hipError_t e = hipMemsetAsync(err, 0, sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_positions, dim3(nb), dim3(256), 0, st, g.ixy, g.cell_end, g.lo, g.size, g.radius, g.cpr, ppr,
-                       dist, offx, offy, flags, bcount, err);
+    if (njobs <= 0) return hipSuccess;
+    int64_t max_n = 0;
+    for (int j = 0; j < njobs; j++) max_n = std::max<int64_t>(max_n, (int64_t)jobs[j].ppr * jobs[j].ppr);
+    const unsigned int nb = blocks_for(max_n, 256);
+    hipLaunchKernelGGL(k_positions, dim3(nb, njobs), dim3(256), 0, st, jobs_dev);
+    hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, st, flags, n, bcount, (int)nb, free_pos, nfree);
+    hipLaunchKernelGGL(k_compact, dim3(nb, njobs), dim3(256), 0, st, jobs_dev);
     return hipGetLastError();
 }
 
