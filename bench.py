@@ -5,6 +5,9 @@ A step is the whole production sweep: 8 impurity samples, each field drawn
 and binned on the GPU, and for each sample the incoherent and the coherent
 pass over the 40 fields with the 3 temperatures (SimulationRunner.cpp:88-98),
 run as one batch: one launch per stage for all samples, passes and fields.
+Steps are pipelined over two batches: the next step's samples are added (and
+their fields binned) while the current one runs.  The roofline's kernel times
+come from two more steps run alone.
 
 Prints one JSON line: lifetimes/s of the FP64 path (incoherent lifetimes, the
 coherent pass is run but not counted; `value`), the sweep wall time, the
@@ -169,26 +172,10 @@ def main():
             i = b.add(coherent=False, **kw)
             b.add(coherent=True, share_field_of=i, **kw)
 
-    def step(b, stats):
-        production_entries(b)
-        sig, mets, secs = b.run()
-        if stats is not None:
-            stats["runs"].append((sig, mets, secs))  # reduced after the timed loop
-
-    def timed(precision):
-        with es.Batch() as b:
-            b.set_precision(precision)
-            for _ in range(args.warmup):
-                step(b, None)
-            st = {"runs": []}
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                step(b, st)
-            dt_ = time.perf_counter() - t0
-        st["launch_s"], st["flops"], st["ref_flops"], st["sigma_flops"], st["sigma_s"] = [], [], [], [], []
-        st["sigma"] = []
+    def reduce_runs(runs):
+        st = {"launch_s": [], "flops": [], "ref_flops": [], "sigma_flops": [], "sigma_s": [], "sigma": []}
         tot = dict(traced=0, cross=0, exact=0, recomputed=0)
-        for sig, mets, _ in st["runs"]:
+        for sig, mets, _ in runs:
             # Every (entry, field) pair of the one launch, with the share of its counters
             # that was executed: the coherent window at alpha = pi/4 has zero width, so its
             # ppq phi points are one particle, traced once, and its counters (which count
@@ -204,6 +191,39 @@ def main():
                              ("recomputed", "fp64_recomputed")):
                 tot[key] += sum(round(m[col] * x) for m, x in ms)
         st.update(tot)
+        return st
+
+    def timed(precision):
+        """K pipelined steps: two batches take turns, so the host adds the next step's
+        samples (and the GPU bins their fields) while the current step runs.  Then two
+        steps run alone, for the kernel times of the roofline."""
+        with es.Batch() as b0, es.Batch() as b1:
+            bufs = (b0, b1)
+            for b in bufs:
+                b.set_precision(precision)
+            for _ in range(max(1, args.warmup)):
+                for b in bufs:
+                    production_entries(b)
+                    b.run(metrics=False)
+            runs = []
+            t0 = time.perf_counter()
+            production_entries(bufs[0])
+            bufs[0].launch()
+            for k in range(args.steps):
+                cur, nxt = bufs[k % 2], bufs[(k + 1) % 2]
+                if k + 1 < args.steps:
+                    production_entries(nxt)
+                    nxt.launch()
+                runs.append(cur.collect())
+            dt_ = time.perf_counter() - t0
+            alone = []
+            for _ in range(2):
+                production_entries(b0)
+                alone.append(b0.run())
+        st = reduce_runs(runs)
+        iso = reduce_runs(alone)
+        st["launch_s"], st["flops"], st["ref_flops"] = iso["launch_s"], iso["flops"], iso["ref_flops"]
+        st["sigma_flops"], st["sigma_s"] = iso["sigma_flops"], iso["sigma_s"]
         return st, dt_
 
     stats, dt = timed("fp64")

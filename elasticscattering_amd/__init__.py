@@ -178,6 +178,8 @@ _SIGNATURES = {
     "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
     "es_batch_set_precision": (C.c_int, [_P, C.c_int32]),
     "es_batch_run": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
+    "es_batch_launch": (C.c_int, [_P]),
+    "es_batch_collect": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_batch_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int64)]),
     "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
@@ -479,12 +481,24 @@ class Batch:
 
         The metrics of an entry are a numpy record array with the es_metrics fields
         (``mets[e][b]["cells_passed"]``)."""
+        return self._results(self.lib.es_batch_run, metrics)
+
+    def launch(self):
+        """Enqueues the run and returns; :meth:`collect` waits for it (the host may prepare
+        another batch meanwhile)."""
+        self._check(self.lib.es_batch_launch(self.h))
+
+    def collect(self, metrics=True):
+        """The launched run's results, as :meth:`run` returns them."""
+        return self._results(self.lib.es_batch_collect, metrics)
+
+    def _results(self, fn, metrics):
         nout = sum(b * t for b, t in self.entries)
         npairs = sum(b for b, _ in self.entries)
         out = (Sigma * max(nout, 1))()
         ms = (Metrics * max(npairs, 1))() if metrics else None
         secs = C.c_double()
-        self._check(self.lib.es_batch_run(self.h, out, ms, C.byref(secs)))
+        self._check(fn(self.h, out, ms, C.byref(secs)))
         flat = np.frombuffer(out, dtype=np.float64, count=2 * nout) if nout else np.empty(0)
         recs = np.frombuffer(ms, dtype=_METRICS_DTYPE, count=npairs).copy() if metrics else None
         sig, mets, o, p = [], [] if metrics else None, 0, 0

@@ -919,20 +919,29 @@ struct StageOut {
     double t_trace = 0, t_temp = 0;
 };
 
-int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, StageOut& o, bool events = true)
+// Enqueues the copy of the results into pinned staging (sized by
+// stages_prepare), one copy; stages_collect_finish takes them after a wait.
+int stages_collect_enqueue(Ctx& c, Stages& s, const Plan& pl, bool metrics)
+{
+    const size_t bsig = (size_t)pl.nout * es::kMaxT * 2 * sizeof(double);
+    const size_t bm = (size_t)pl.nout * es::kMetricSlots * sizeof(int64_t);
+    const size_t bytes = metrics ? s.o_mout + bm : bsig;
+    if (c.hbuf_cap < bytes) return fail(c, ES_ERR_INVALID_ARG, "result staging smaller than the plan's results");
+    ES_HIP(c, hipMemcpyAsync(c.hbuf, s.outs.p, bytes, hipMemcpyDeviceToHost, c.stream));
+    return ES_OK;
+}
+
+// After the stream has passed the copy of stages_collect_enqueue: the results
+// and the stage times of the events.
+int stages_collect_finish(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, StageOut& o, bool events)
 {
     o.sig.resize((size_t)pl.nout * es::kMaxT * 2);
     o.ltsum.resize(pl.nout);
     o.m.resize((size_t)pl.nout * es::kMetricSlots);
-    // Into pinned staging (sized by stages_prepare) in one copy, then one wait.
     const size_t bsig = o.sig.size() * sizeof(double), blt = o.ltsum.size() * sizeof(double);
     const size_t bm = o.m.size() * sizeof(int64_t);
-    const size_t bytes = metrics ? s.o_mout + bm : bsig;
-    if (c.hbuf_cap < bytes) return fail(c, ES_ERR_INVALID_ARG, "result staging smaller than the plan's results");
-    char* h = static_cast<char*>(c.hbuf);
-    ES_HIP(c, hipMemcpyAsync(h, s.outs.p, bytes, hipMemcpyDeviceToHost, c.stream));
-    ES_HIP(c, hipStreamSynchronize(c.stream));
     s.queue_dirty = false;
+    const char* h = static_cast<const char*>(c.hbuf);
     std::memcpy(o.sig.data(), h, bsig);
     if (metrics) {
         std::memcpy(o.ltsum.data(), h + s.o_ltsum, blt);
@@ -950,6 +959,15 @@ int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, 
         o.t_temp += ms * 1e-3;
     }
     return ES_OK;
+}
+
+// The results in one copy and one wait.
+int stages_collect(Ctx& c, Stages& s, const Plan& pl, bool trace, bool metrics, StageOut& o, bool events = true)
+{
+    int rc;
+    if ((rc = stages_collect_enqueue(c, s, pl, metrics))) return rc;
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    return stages_collect_finish(c, s, pl, trace, metrics, o, events);
 }
 
 void fill_metrics(es_metrics& o, const int64_t* m, double ltsum, int64_t P, int64_t inside, double t_life,
@@ -1064,6 +1082,9 @@ struct es_batch {
     std::vector<int> graph_blocks;
     Plan plan;
     bool plan_ready = false;
+    // es_batch_launch: the run in flight (its results staged, collected by es_batch_collect)
+    bool launched = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
 namespace {
@@ -1441,12 +1462,14 @@ void batch_release_graph(es_batch* bt)
     bt->graph_ready = false;
 }
 
-// One run with a given shape; fills `so` (sigma, sums, metrics, event times) and
-// the wall time between events around the whole stage sequence.
-int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
+// Enqueues one run with a given shape: the stage kernels (or the captured
+// graph of them) between two events, and the copy of the results.  No wait
+// except the one of batch_plan for the read-backs of new fields.
+int batch_launch(es_batch* bt, const Shape& sh)
 {
     Ctx& c = bt->c;
     int rc;
+    if (bt->launched) return fail(c, ES_ERR_INVALID_ARG, "a launched run has not been collected");
     if (!bt->plan_ready) {
         batch_release_graph(bt);
         if ((rc = batch_plan(bt))) return rc;
@@ -1456,54 +1479,90 @@ int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
     std::vector<int> nblocks;
     int nblocks64 = 0;
     if ((rc = trace_blocks(c, bt->plan.clockwise, sh, bt->plan, nblocks, &nblocks64))) return rc;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    ES_HIP(c, hipEventCreate(&t0));
-    ES_HIP(c, hipEventCreate(&t1));
-    rc = ES_OK;
-    hipError_t e = hipEventRecord(t0, c.stream);
-    if (e != hipSuccess) rc = hip_fail(c, e, "hipEventRecord");
-    if (!rc && bt->capture) {
+    if (!bt->t0) ES_HIP(c, hipEventCreate(&bt->t0));
+    if (!bt->t1) ES_HIP(c, hipEventCreate(&bt->t1));
+    ES_HIP(c, hipEventRecord(bt->t0, c.stream));
+    if (bt->capture) {
         if (bt->graph_ready && bt->graph_blocks != nblocks) batch_release_graph(bt);
         if (!bt->graph_ready) {
-            e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess) rc = hip_fail(c, e, "hipStreamBeginCapture");
-            if (!rc) {
-                rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, false, nblocks64);
-                hipGraph_t g = nullptr;
-                e = hipStreamEndCapture(c.stream, &g);
-                if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
-                if (!rc) {
-                    bt->s.graph = g;
-                    e = hipGraphInstantiate(&bt->s.exec, g, nullptr, nullptr, 0);
-                    if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphInstantiate");
-                    bt->graph_ready = (rc == ES_OK);
-                    bt->graph_blocks = nblocks;
-                } else if (g) {
-                    (void)hipGraphDestroy(g);
-                }
+            hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(c, e, "hipStreamBeginCapture");
+            rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, false, nblocks64);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(c.stream, &g);
+            if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
             }
+            bt->s.graph = g;
+            e = hipGraphInstantiate(&bt->s.exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+            bt->graph_ready = true;
+            bt->graph_blocks = nblocks;
         }
-        if (!rc) {
-            e = hipGraphLaunch(bt->s.exec, c.stream);
-            if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphLaunch");
-        }
-    } else if (!rc) {
-        rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, true, nblocks64);
+        ES_HIP(c, hipGraphLaunch(bt->s.exec, c.stream));
+    } else if ((rc = stages_enqueue(c, bt->s, bt->plan, sh, true, 1, nullptr, nblocks, true, nblocks64))) {
+        return rc;
     }
-    if (!rc) {
-        e = hipEventRecord(t1, c.stream);
-        if (e != hipSuccess) rc = hip_fail(c, e, "hipEventRecord");
-    }
-    if (!rc) rc = stages_collect(c, bt->s, bt->plan, true, true, so, !bt->capture);
+    ES_HIP(c, hipEventRecord(bt->t1, c.stream));
+    if ((rc = stages_collect_enqueue(c, bt->s, bt->plan, true))) return rc;
+    bt->launched = true;
+    return ES_OK;
+}
+
+// Waits for the launched run; fills `so` (sigma, sums, metrics, event times)
+// and the device time between the events around the stage sequence.
+int batch_collect(es_batch* bt, StageOut& so, double* wall)
+{
+    Ctx& c = bt->c;
+    if (!bt->launched) return fail(c, ES_ERR_INVALID_ARG, "no launched run to collect");
+    bt->launched = false;
+    ES_HIP(c, hipStreamSynchronize(c.stream));
+    int rc;
+    if ((rc = stages_collect_finish(c, bt->s, bt->plan, true, true, so, !bt->capture))) return rc;
     float ms = 0;
-    if (!rc) {
-        e = hipEventElapsedTime(&ms, t0, t1);
-        if (e != hipSuccess) rc = hip_fail(c, e, "hipEventElapsedTime");
-    }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    ES_HIP(c, hipEventElapsedTime(&ms, bt->t0, bt->t1));
     if (wall) *wall = ms * 1e-3;
-    return rc;
+    return ES_OK;
+}
+
+// One run with a given shape (launch, then collect).
+int batch_run_once(es_batch* bt, const Shape& sh, StageOut& so, double* wall)
+{
+    int rc;
+    if ((rc = batch_launch(bt, sh))) return rc;
+    return batch_collect(bt, so, wall);
+}
+
+// Outputs of a collected run in the es_batch_run layout.
+void batch_outputs(es_batch* bt, const StageOut& so, es_sigma* out, es_metrics* metrics)
+{
+    const Plan& pl = bt->plan;
+    size_t o = 0;
+    for (const es::PairDev& p : pl.pairs) {
+        const es_batch::Entry& en = bt->entries[p.sample];
+        const int nT = (int)en.T.size();
+        for (int t = 0; t < nT && out; t++) {
+            out[o + t].xx = so.sig[((size_t)p.out * es::kMaxT + t) * 2];
+            out[o + t].xy = so.sig[((size_t)p.out * es::kMaxT + t) * 2 + 1];
+        }
+        o += nT;
+        if (metrics) {
+            const Field& f = *bt->fields[en.field];
+            const int64_t P_ = (int64_t)en.ppr * en.ppr * 4 * en.ppq;
+            fill_metrics(metrics[p.out], &so.m[(size_t)p.out * es::kMetricSlots], so.ltsum[p.out], P_,
+                         f.n_inside * 4 * en.ppq, so.t_trace / pl.nout, so.t_temp / pl.nout);
+        }
+    }
+}
+
+// Mutating calls are refused while a launched run has not been collected:
+// its results wait in the staging that a new plan would overwrite.
+int batch_idle(es_batch* bt)
+{
+    if (bt->launched) return fail(bt->c, ES_ERR_INVALID_ARG, "a launched run has not been collected (es_batch_collect)");
+    return ES_OK;
 }
 
 }  // namespace
@@ -1533,6 +1592,8 @@ void es_batch_destroy(es_batch* b)
     (void)hipSetDevice(b->c.device);
     if (b->c.stream) (void)hipStreamSynchronize(b->c.stream);
     batch_release_graph(b);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
     for (Field* f : b->fields) {
         field_release(b->c, *f);
         delete f;
@@ -1547,6 +1608,7 @@ const char* es_batch_last_error(const es_batch* b) { return b ? b->c.err.c_str()
 int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     Ctx& c = bt->c;
     ES_HIP(c, hipSetDevice(c.device));
     if (!smp) return fail(c, ES_ERR_INVALID_ARG, "null sample");
@@ -1630,6 +1692,7 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
 int es_batch_clear(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     ES_HIP(bt->c, hipSetDevice(bt->c.device));
     const int rc = jobs_launch(bt->c, bt->pending);  // kept fields stay consistent
     if (rc) return rc;
@@ -1642,6 +1705,7 @@ int es_batch_clear(es_batch* bt)
 int es_batch_reset(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     ES_HIP(bt->c, hipSetDevice(bt->c.device));
     const int rc = jobs_launch(bt->c, bt->pending);
     if (rc) return rc;
@@ -1655,6 +1719,7 @@ int es_batch_reset(es_batch* bt)
 int es_batch_set_shape(es_batch* bt, const es_batch_shape* shape)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     if (!shape || (shape->waves_per_eu != 2 && shape->waves_per_eu != 3) || shape->chunk < 1 || shape->chunk > 4096 ||
         shape->grid_mult < 1 || shape->grid_mult > 8 || shape->cooperative < -1 || shape->cooperative > 2)
         return fail(bt->c, ES_ERR_INVALID_ARG,
@@ -1680,6 +1745,7 @@ int es_batch_get_shape(es_batch* bt, es_batch_shape* shape)
 int es_batch_set_capture(es_batch* bt, int32_t enable)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     bt->capture = enable != 0;
     batch_release_graph(bt);
     return ES_OK;
@@ -1688,6 +1754,7 @@ int es_batch_set_capture(es_batch* bt, int32_t enable)
 int es_batch_set_precision(es_batch* bt, int32_t precision)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     if (precision != ES_PRECISION_FP64 && precision != ES_PRECISION_MIXED)
         return fail(bt->c, ES_ERR_INVALID_ARG, "unsupported precision mode");
     if (precision != bt->precision) {
@@ -1708,29 +1775,34 @@ int es_batch_run(es_batch* bt, es_sigma* out, es_metrics* metrics, double* secon
     const int rc = batch_run_once(bt, bt->shape, so, &wall);
     if (rc) return rc;
     if (seconds) *seconds = wall;
-    const Plan& pl = bt->plan;
-    size_t o = 0;
-    for (const es::PairDev& p : pl.pairs) {
-        const es_batch::Entry& en = bt->entries[p.sample];
-        const int nT = (int)en.T.size();
-        for (int t = 0; t < nT && out; t++) {
-            out[o + t].xx = so.sig[((size_t)p.out * es::kMaxT + t) * 2];
-            out[o + t].xy = so.sig[((size_t)p.out * es::kMaxT + t) * 2 + 1];
-        }
-        o += nT;
-        if (metrics) {
-            const Field& f = *bt->fields[en.field];
-            const int64_t P_ = (int64_t)en.ppr * en.ppr * 4 * en.ppq;
-            fill_metrics(metrics[p.out], &so.m[(size_t)p.out * es::kMetricSlots], so.ltsum[p.out], P_,
-                         f.n_inside * 4 * en.ppq, so.t_trace / pl.nout, so.t_temp / pl.nout);
-        }
-    }
+    batch_outputs(bt, so, out, metrics);
+    return ES_OK;
+}
+
+int es_batch_launch(es_batch* bt)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    return batch_launch(bt, bt->shape);
+}
+
+int es_batch_collect(es_batch* bt, es_sigma* out, es_metrics* metrics, double* seconds)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    StageOut so;
+    double wall = 0;
+    const int rc = batch_collect(bt, so, &wall);
+    if (rc) return rc;
+    if (seconds) *seconds = wall;
+    batch_outputs(bt, so, out, metrics);
     return ES_OK;
 }
 
 int es_batch_counts(es_batch* bt, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     if (!bt->plan_ready) {
         batch_release_graph(bt);
         const int rc = batch_plan(bt);
@@ -1750,6 +1822,7 @@ int es_batch_counts(es_batch* bt, int32_t* n_samples, int32_t* n_pairs, int32_t*
 int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
     Ctx& c = bt->c;
     ES_HIP(c, hipSetDevice(c.device));
     const bool cap = bt->capture;

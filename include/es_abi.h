@@ -246,6 +246,17 @@ int es_batch_set_precision(es_batch* b, int32_t precision);
  * seconds (or NULL) the device time of the whole stage sequence.
  */
 int es_batch_run(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
+/*
+ * es_batch_run in two halves: es_batch_launch enqueues the run on the batch's
+ * own stream and returns (it waits only for the read-backs of newly added
+ * fields); es_batch_collect waits for it and fills the same outputs.  Between
+ * them the host may prepare another batch, whose work then queues behind or
+ * beside this one on the GPU; this batch refuses other calls until collected.
+ * (No counterpart in the reference, whose ComputeSigmas blocks,
+ * SimulationCL.cpp:310-341; for callers that pipeline samples.)
+ */
+int es_batch_launch(es_batch* b);
+int es_batch_collect(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
 /* Sizes of the batch: entries, (entry, field) pairs, sigma outputs, traced work items. */
 int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items);
 /* Times every trace launch shape on this batch and keeps the fastest. */

@@ -267,6 +267,35 @@ def test_reset_redraws_into_kept_memory_with_same_bits(es, batch_eager):
     _same_results(want, again)
 
 
+def test_launch_collect_pipelined_with_same_bits(es, batch_eager):
+    """es_batch_launch / es_batch_collect on two batches whose runs overlap on
+    the GPU give the bits of es_batch_run; a launched batch refuses other
+    calls until collected."""
+    entries, _, eager = batch_eager
+    announce("batch launch / collect: two pipelined batches")
+    with es.Batch() as b1, es.Batch() as b2:
+        for e in entries:
+            _add(b1, e)
+        b1.launch()
+        with pytest.raises(es.EsError):
+            _add(b1, entries[0])  # not collected yet
+        with pytest.raises(es.EsError):
+            b1.launch()
+        for e in entries:
+            _add(b2, e)
+        b2.launch()
+        r1 = b1.collect()
+        r2 = b2.collect()
+        with pytest.raises(es.EsError):
+            b1.collect()  # nothing launched
+        _same_results(eager, r1)
+        _same_results(eager, r2)
+        b1.reset()
+        for e in entries:
+            _add(b1, e)
+        _same_results(eager, b1.run())
+
+
 def test_batch_errors(es):
     announce("batch argument errors")
     with es.Batch() as b:
