@@ -714,7 +714,7 @@ void plan_offsets(Plan& pl, long long seg_items)
         pl.max_items = std::max(pl.max_items, g.items);
         pl.max_blocks = std::max(pl.max_blocks, g.blocks);
     }
-    pl.NT = maxT <= 1 ? 1 : maxT <= 2 ? 2 : maxT <= 4 ? 4 : 8;
+    pl.NT = maxT <= 3 ? std::max(maxT, 1) : maxT <= 4 ? 4 : 8;
 }
 
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }

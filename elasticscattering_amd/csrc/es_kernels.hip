@@ -1981,6 +1981,7 @@ hipError_t launch_sigma(const SigmaArgs& a, int NT, hipStream_t st)
     switch (NT) {
     case 1: return launch_sigma_t<1>(a, st);
     case 2: return launch_sigma_t<2>(a, st);
+    case 3: return launch_sigma_t<3>(a, st);
     case 4: return launch_sigma_t<4>(a, st);
     case 8: return launch_sigma_t<8>(a, st);
     default: return hipErrorInvalidValue;
