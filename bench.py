@@ -65,8 +65,10 @@ def algorithmic_flops(m, model):
     return float(sum(model[k] * m[k].sum() if hasattr(m[k], "sum") else model[k] * m[k] for k in model))
 
 
-def cpu_baseline(w, seconds_target=15.0):
-    """The oracle (libm build, 1 thread) on the first sample's incoherent pass at the lowest and highest field."""
+def cpu_baseline(w, seconds_target=20.0):
+    """The oracle (libm build) on the first sample's incoherent pass at 4 fields spread over the sweep:
+    1 thread (the reference CPU path is single-threaded, SimulationCPU.cpp:112-135), then the same
+    sample over the host's cores (rows split across threads, up to 16)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from elasticscattering_amd.workloads import region
@@ -77,28 +79,36 @@ def cpu_baseline(w, seconds_target=15.0):
     rs, ext = region(w)
     og = O.OracleGrid.generate(lib, w["seeds"][0], rs, ext, 0.0, w["impurity_radius"], 15, w["cells_per_row"],
                                n=w["impurity_count"])
-    prm = O.sim_params(w["positions_per_row"], w["particles_per_quadrant"], rs, ext, og.cell_size, w["tau"],
-                       w["alpha"], w["clockwise"], w["particle_speed"], w["incoherent_tau"], nthreads=1)
-    fields = [w["fields"][0], w["fields"][-1]]
-    t0 = time.perf_counter()
-    done = 0
-    for B in fields:
-        O.compute_sigmas(lib, og, prm, 0, B, w["temperatures"])
-        done += 1
-        if time.perf_counter() - t0 > seconds_target:
-            break
-    dt = time.perf_counter() - t0
+    fs = w["fields"]
+    fields = [fs[0], fs[len(fs) // 3], fs[2 * len(fs) // 3], fs[-1]]
     P = w["positions_per_row"] ** 2 * 4 * w["particles_per_quadrant"]
+
+    def timed_run(nthreads):
+        prm = O.sim_params(w["positions_per_row"], w["particles_per_quadrant"], rs, ext, og.cell_size, w["tau"],
+                           w["alpha"], w["clockwise"], w["particle_speed"], w["incoherent_tau"], nthreads=nthreads)
+        t0 = time.perf_counter()
+        done = 0
+        for B in fields:
+            O.compute_sigmas(lib, og, prm, 0, B, w["temperatures"])
+            done += 1
+            if time.perf_counter() - t0 > seconds_target:
+                break
+        return done, time.perf_counter() - t0
+
+    done, dt = timed_run(1)
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    done_t, dt_t = timed_run(threads)
+    sample = (f"CPU oracle (restated reference CPU path, glibc libm, -O2) on sample seed {w['seeds'][0]}, "
+              f"incoherent pass, B in {[round(float(f), 6) for f in fields[:done]]} T, "
+              f"{len(w['temperatures'])} temperatures each: {done * P} lifetimes + sigma stage in {dt:.1f} s")
     return {
         "value": done * P / dt,
         "unit": "lifetimes/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"CPU oracle (restated reference CPU path, glibc libm, -O2) on sample seed {w['seeds'][0]}, "
-                   f"incoherent pass, B in {[round(float(f), 6) for f in fields[:done]]} T, "
-                   f"{len(w['temperatures'])} temperatures each: "
-                   f"{done * P} lifetimes + sigma stage in {dt:.1f} s"),
+        "sample": sample,
         "seconds": dt,
+        "threads": {"value": done_t * P / dt_t, "cores": threads, "seconds": dt_t},
     }
 
 
@@ -299,6 +309,8 @@ def main():
         cb = cpu_baseline(w)
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            # the same sample over the host's cores (a second baseline, not the contract's)
+            line["cpu_baseline_threads"] = dict(cb["threads"], unit="lifetimes/s", kind="port")
     print(_json(line), flush=True)
     return 0
 
