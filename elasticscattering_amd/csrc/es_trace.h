@@ -141,8 +141,54 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
     return tt;
 }
 
+// One of the 8 points where the orbit meets the lines of the cell's edges
+// (UpdateFirstBoundaryIntersect, cell_grid.h:91-129): edge e in the
+// reference order — top (0,+1), bottom (0,-1), right (+1,0), left (-1,0) —
+// and point k (i1, i2).  Returns false when the edge line misses the circle;
+// else the point and whether it lies inside the edge segment.
+struct CellCorners {
+    double llx, lly, lrx, lry, trx, try_, tlx, tly, qx, qy;
+};
+
+__device__ __forceinline__ bool d_edge_point(const Orb& o, const CellCorners& c, int e, int k, double& ix, double& iy,
+                                             bool& in_segment)
+{
+    // corner selection by flags (selects, not a table in scratch)
+    const bool top = e == 0, bot = e == 1, right = e == 2, horiz = e < 2;
+    double p1x = c.llx, p1y = c.lly;
+    p1x = top ? c.tlx : p1x;
+    p1y = top ? c.tly : p1y;
+    p1x = right ? c.lrx : p1x;
+    p1y = right ? c.lry : p1y;
+    double p2x = c.tlx, p2y = c.tly;
+    p2x = bot ? c.lrx : p2x;
+    p2y = bot ? c.lry : p2y;
+    p2x = (top || right) ? c.trx : p2x;
+    p2y = (top || right) ? c.try_ : p2y;
+    const double ux = horiz ? c.qx : 0.0, uy = horiz ? 0.0 : c.qy;
+    const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
+    const double projx = p1x + ux * pd, projy = p1y + uy * pd;
+    const double dx = projx - o.cx, dy = projy - o.cy;
+    const double pcd = dx * dx + dy * dy;
+    if (pcd >= o.R2) return false;
+    const double s = sqrt(o.R2 - pcd);
+    const double tx = ux * s, ty = uy * s;
+    const bool horizontal = fabs(ux) > fabs(uy);
+    const double l0 = horizontal ? p1x : p1y, l1 = horizontal ? p2x : p2y;
+    ix = k ? (projx - tx) : (projx + tx);
+    iy = k ? (projy - ty) : (projy + ty);
+    const double a = horizontal ? ix : iy;
+    in_segment = (a > l0 && a < l1) || (a < l0 && a > l1);
+    return true;
+}
+
 // GetNextCell (cell_grid.h:132-181) with UpdateFirstBoundaryIntersect
 // (:91-129) and Update_Best_Intersect (:80-89).  `phi0`: the start angle.
+// A first pass finds which of the 8 edge-line points lie in their segment and
+// differ from the entry point (DifferentPoint, :74-77: a point it rejects is
+// rejected whatever its angle); a second pass takes only those, in the
+// reference's order, for their angles (asin) and the smallest dphi — so a wave
+// runs the angle once or twice per cell step, not once per point any lane has.
 template <bool CW>
 __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const DevGrid& g, const Isect& last, Isect& next,
                                             int& edge_hits)
@@ -151,59 +197,51 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     next = last;
     const double L = g.L;
     const double dp_eps = 1e-6 * L;
-    const double llx = g.start + (double)last.cx * L;
-    const double lly = g.start + (double)last.cy * L;
-    const double lrx = llx + L, lry = lly + 0.0;
-    const double trx = llx + L, try_ = lly + L;
-    const double tlx = llx + 0.0, tly = lly + L;
+    CellCorners c;
+    c.llx = g.start + (double)last.cx * L;
+    c.lly = g.start + (double)last.cy * L;
+    c.lrx = c.llx + L;
+    c.lry = c.lly + 0.0;
+    c.trx = c.llx + L;
+    c.try_ = c.lly + L;
+    c.tlx = c.llx + 0.0;
+    c.tly = c.lly + L;
+    const double llx = c.llx, lly = c.lly;
     // The edge directions (p2 - p1) / L of cell_grid.h:98-99: the horizontal
     // edges share trx - tlx = lrx - llx = (llx + L) - llx, the vertical ones
     // try - lry = tly - lly = (lly + L) - lly, and the other components are
     // exactly 0 (0 / L = +0); two quotients per cell.
-    const double qx = esm_div_rcp(trx - tlx, L, g.inv_L);
-    const double qy = esm_div_rcp(try_ - lry, L, g.inv_L);
+    c.qx = esm_div_rcp(c.trx - c.tlx, L, g.inv_L);
+    c.qy = esm_div_rcp(c.try_ - c.lry, L, g.inv_L);
 
-    // Edges in the reference order: top (0,+1), bottom (0,-1), right (+1,0), left (-1,0).
+    unsigned int cand = 0;  // bit 2 e + k: a point in its segment, not the entry point
 #pragma unroll 1
     for (int e = 0; e < 4; e++) {
-        const double p1x = (e == 0) ? tlx : (e == 2) ? lrx : llx;
-        const double p1y = (e == 0) ? tly : (e == 2) ? lry : lly;
-        const double p2x = (e == 0 || e == 2) ? trx : (e == 1) ? lrx : tlx;
-        const double p2y = (e == 0 || e == 2) ? try_ : (e == 1) ? lry : tly;
-        const int ncx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
-        const int ncy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
-
-        const double ux = (e < 2) ? qx : 0.0, uy = (e < 2) ? 0.0 : qy;
-        const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
-        const double projx = p1x + ux * pd, projy = p1y + uy * pd;
-        const double dx = projx - o.cx, dy = projy - o.cy;
-        const double pcd = dx * dx + dy * dy;
-        if (pcd >= o.R2) continue;
-        const double s = sqrt(o.R2 - pcd);
-        const double tx = ux * s, ty = uy * s;
-        const bool horizontal = fabs(ux) > fabs(uy);
-        const double l0 = horizontal ? p1x : p1y, l1 = horizontal ? p2x : p2y;
-#pragma unroll 1
+#pragma unroll
         for (int k = 0; k < 2; k++) {
-            const double ix = k ? (projx - tx) : (projx + tx);
-            const double iy = k ? (projy - ty) : (projy + ty);
-            const double a = horizontal ? ix : iy;
-            if ((a > l0 && a < l1) || (a < l0 && a > l1)) {
-                edge_hits += 1;
-                // DifferentPoint (cell_grid.h:74-77) first: a point it rejects is
-                // rejected whatever its angle, so its asin is not needed.
-                if (!(fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps)) continue;
-                const double ia = d_angle<CW>(ix, iy, o);
-                const double dphi = d_cross_angle<CW>(last.ang, ia);
-                if (dphi < best_dphi) {
-                    next.x = ix;
-                    next.y = iy;
-                    next.cx = ncx;
-                    next.cy = ncy;
-                    next.ang = ia;
-                    best_dphi = dphi;
-                }
-            }
+            double ix, iy;
+            bool in_seg = false;
+            if (!d_edge_point(o, c, e, k, ix, iy, in_seg) || !in_seg) continue;
+            edge_hits += 1;
+            if (fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps) cand |= 1u << (2 * e + k);
+        }
+    }
+    while (cand) {
+        const int bit = __builtin_ctz(cand);
+        cand &= cand - 1u;
+        const int e = bit >> 1, k = bit & 1;
+        double ix = 0.0, iy = 0.0;
+        bool in_seg = false;
+        (void)d_edge_point(o, c, e, k, ix, iy, in_seg);  // the same operations: the same point
+        const double ia = d_angle<CW>(ix, iy, o);
+        const double dphi = d_cross_angle<CW>(last.ang, ia);
+        if (dphi < best_dphi) {
+            next.x = ix;
+            next.y = iy;
+            next.cx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
+            next.cy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
+            next.ang = ia;
+            best_dphi = dphi;
         }
     }
 
