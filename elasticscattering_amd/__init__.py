@@ -456,7 +456,7 @@ class Batch:
         self._check(self.lib.es_batch_get_shape(self.h, C.byref(sh)))
         return sh.as_dict()
 
-    def set_shape(self, waves_per_eu=3, chunk=128, grid_mult=1, cooperative=-1):
+    def set_shape(self, waves_per_eu=3, chunk=256, grid_mult=1, cooperative=-1):
         """Trace launch shape.  cooperative: 0 crossing disks solved per lane, 1 shared by the
         wave, 2 shared and screened, -1 (default) 2 or 1 by the fields' impurities per cell."""
         sh = BatchShape(int(waves_per_eu), int(chunk), int(grid_mult), int(cooperative))

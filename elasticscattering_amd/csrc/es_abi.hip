@@ -585,7 +585,7 @@ int taus_for(Ctx& c, const es_settings& st, int coherent, const double* T, int n
 
 struct Shape {
     int waves_per_eu = 3;
-    int chunk = 128;
+    int chunk = 256;
     int grid_mult = 1;
     int coop = -1;  // FP64 trace: 0 per lane, 1 wave-cooperative, 2 cooperative + screened, -1 by the field
 };

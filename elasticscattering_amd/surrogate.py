@@ -452,27 +452,61 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
             reuse["batch"].close()
 
 
+def _pchip_rows(x, y, xe):
+    """pchip for many small interpolation problems of the same size at once:
+    row r interpolates (x[r], y[r]) at xe[r]; x [m][k] ascending (k >= 3),
+    y [m][k][2], xe [m].  The same operations as pchip, row by row."""
+    h = np.diff(x, axis=1)[:, :, None]
+    m = np.diff(y, axis=1) / h
+    d = np.empty_like(y)
+    w1 = 2 * h[:, 1:] + h[:, :-1]
+    w2 = h[:, 1:] + 2 * h[:, :-1]
+    flat = (np.sign(m[:, 1:]) != np.sign(m[:, :-1])) | (m[:, 1:] == 0) | (m[:, :-1] == 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hm = (w1 / m[:, :-1] + w2 / m[:, 1:]) / (w1 + w2)
+        d[:, 1:-1] = np.where(flat, 0.0, 1.0 / hm)
+    d[:, 0] = _pchip_end(h[:, 0], h[:, 1], m[:, 0], m[:, 1])
+    d[:, -1] = _pchip_end(h[:, -1], h[:, -2], m[:, -1], m[:, -2])
+    r = np.arange(x.shape[0])
+    i = np.clip((x <= xe[:, None]).sum(axis=1) - 1, 0, x.shape[1] - 2)
+    t = (xe - x[r, i])[:, None]
+    hi, mi, d0, d1 = h[r, i], m[r, i], d[r, i], d[r, i + 1]
+    c2 = (3 * mi - 2 * d0 - d1) / hi
+    c3 = (d0 + d1 - 2 * mi) / (hi * hi)
+    return y[r, i] + t * (d0 + t * (c2 + t * c3))
+
+
 def _rough_points(c, points, n, tol_r):
     """Indices among `points` (and their grid neighbours) with both grid
     neighbours solved whose value the local interpolant (PCHIP through up to 3
     solved points on each side, the point itself left out) misses by more than
     tol_r of |sigma|: smooth curvature is followed at this spacing, a step of
-    the solver's sigma between neighbouring fields is not."""
-    rough = []
-    cand = {k for i in points for k in (i - 1, i, i + 1) if 0 < k < n - 1}
+    the solver's sigma between neighbouring fields is not.  The candidates'
+    interpolations run together, grouped by the size of their windows."""
+    cand = sorted({k for i in points for k in (i - 1, i, i + 1)
+                   if 0 < k < n - 1 and k in c.solved and k - 1 in c.solved and k + 1 in c.solved})
     if not cand:
-        return rough
+        return []
     idx = np.array(sorted(c.solved))
-    for k in sorted(cand):
-        if not (k in c.solved and k - 1 in c.solved and k + 1 in c.solved):
-            continue
-        j = int(np.searchsorted(idx, k))
-        loc = np.concatenate([idx[max(0, j - 3):j], idx[j + 1:j + 4]])
-        vals = np.array([c.solved[i] for i in loc])
-        est = _curve_from(c.u, loc, vals, at=np.array([k]))[0]
-        if tensor_error(est, c.solved[k]) > tol_r:
-            rough.append(k)
-    return rough
+    vals = np.array([c.solved[i] for i in idx])
+    y = targets(vals)
+    cand = np.array(cand)
+    pos = np.searchsorted(idx, cand)
+    nl = pos - np.maximum(pos - 3, 0)
+    nr = np.minimum(pos + 4, len(idx)) - pos - 1
+    est = np.empty((len(cand), 2))
+    for a, b in set(zip(nl.tolist(), nr.tolist())):
+        sel = (nl == a) & (nr == b)
+        win = pos[sel][:, None] + np.concatenate([np.arange(-a, 0), np.arange(1, b + 1)])[None, :]
+        X, Y, xe = c.u[idx[win]], y[win], c.u[cand[sel]]
+        if a + b == 2:  # as _curve_from for two points: linear, clamped
+            t = np.clip((xe - X[:, 0]) / (X[:, 1] - X[:, 0]), 0.0, 1.0)[:, None]
+            e = Y[:, 0] + t * (Y[:, 1] - Y[:, 0])
+        else:
+            e = _pchip_rows(X, Y, xe)
+        est[sel] = from_targets(e)
+    err = tensor_error(est, vals[pos])
+    return [int(k) for k, e in zip(cand, err) if e > tol_r]
 
 
 def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):

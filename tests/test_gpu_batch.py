@@ -212,7 +212,7 @@ def test_tuner_shapes_give_same_results(es, batch_eager):
                   (3, 64, 1, 0)):
         b.set_shape(*shape)
         _same_results(eager, b.run())
-    b.set_shape(3, 128, 1, -1)
+    b.set_shape(3, 256, 1, -1)  # back to the default shape
 
 
 def test_clear_reuses_fields_with_same_bits(es, batch_eager):
