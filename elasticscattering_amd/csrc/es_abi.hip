@@ -107,6 +107,8 @@ int ensure(Ctx& c, DevBuf& b, size_t bytes, const char* what)
     if (c.byte_cap && c.bytes_held - b.cap + bytes > c.byte_cap)
         return fail(c, ES_ERR_OOM, std::string("device size cap exceeded allocating ") + what);
     if (b.p) {
+        // Work already enqueued may still use the old buffer.
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
         (void)hipFree(b.p);
         c.bytes_held -= b.cap;
         b.p = nullptr;
