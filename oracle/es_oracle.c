@@ -1007,36 +1007,97 @@ static inline void neum_add(neum* a, double x)
     a->s = t;
 }
 
+/* One weighted term of ApplySigma + IntegrateResult (Simulation.h:178-222): raw lifetime r of
+ * particle (q, p) at a position of Simpson weight wp. */
+static inline void sigma_term(const orc_state* st, double tau, double max_lt, double r, double wp, int q, int p,
+                              double* txx, double* txy)
+{
+    const double lt = (r < max_lt) ? r : max_lt;
+    const double phi = st->ps.phi_start + q * C_HALF_PI + p * st->ps.phi_step_size;
+    const double sigma_base = orc_get_sigma(lt, phi, tau, st->signed_angular_speed);
+    const double sxx = sigma_base * O_COS(phi);
+    const double sxy = sigma_base * O_SIN(phi);
+    const double w = wp * orc_simpson_weight(p, st->ppq);
+    *txx = sxx * w;
+    *txy = sxy * w;
+}
+
+typedef struct {
+    const orc_state* st;
+    double tau, max_lt;
+    const double* raw;
+    double* terms;  /* [idx][xx, xy] */
+    int j0, j1;
+} term_job;
+
+static void* sigma_term_rows(void* arg)
+{
+    term_job* job = (term_job*)arg;
+    const orc_state* st = job->st;
+    for (int j = job->j0; j < job->j1; j++) {
+        const double wy = orc_simpson_weight(j, st->ppr);
+        for (int i = 0; i < st->ppr; i++) {
+            const double wp = wy * orc_simpson_weight(i, st->ppr);
+            for (int q = 0; q < 4; q++)
+                for (int p = 0; p < st->ppq; p++) {
+                    const int64_t idx = sim_index(st, i, j, q, p);
+                    sigma_term(st, job->tau, job->max_lt, job->raw[idx], wp, q, p, &job->terms[2 * idx],
+                               &job->terms[2 * idx + 1]);
+                }
+        }
+    }
+    return NULL;
+}
+
 /*
  * SimulationCPU::DeriveTemperature (SimulationCPU.cpp:143-155):
  * GetTruncatedLifetimes (Simulation.h:78-86), ApplySigma (:178-204),
- * IntegrateResult (:206-222), each in reference order.
+ * IntegrateResult (:206-222), each in reference order.  With nthreads > 1 the
+ * terms are formed by row-split threads and then summed on one thread in the
+ * same order as the single-threaded loop: identical results.
  */
-static orc_sigma derive_temperature(const orc_state* st, double tau, const double* raw)
+static orc_sigma derive_temperature(const orc_state* st, double tau, const double* raw, int nthreads)
 {
     const double max_lt = 15.0 * tau;  /* Simulation.h:91 */
     double total_xx = 0, total_xy = 0;
     neum cxx = {0, 0}, cxy = {0, 0};
-    for (int j = 0; j < st->ppr; j++) {
-        const double wy = orc_simpson_weight(j, st->ppr);
-        for (int i = 0; i < st->ppr; i++) {
-            const double wp = wy * orc_simpson_weight(i, st->ppr);
-            for (int q = 0; q < 4; q++) {
-                for (int p = 0; p < st->ppq; p++) {
-                    const int64_t idx = sim_index(st, i, j, q, p);
-                    const double r = raw[idx];
-                    const double lt = (r < max_lt) ? r : max_lt;
-                    const double phi = st->ps.phi_start + q * C_HALF_PI + p * st->ps.phi_step_size;
-                    const double sigma_base = orc_get_sigma(lt, phi, tau, st->signed_angular_speed);
-                    const double sxx = sigma_base * O_COS(phi);
-                    const double sxy = sigma_base * O_SIN(phi);
-                    const double w = wp * orc_simpson_weight(p, st->ppq);
-                    const double txx = sxx * w, txy = sxy * w;
-                    total_xx += txx;
-                    total_xy += txy;
-                    neum_add(&cxx, txx);
-                    neum_add(&cxy, txy);
-                }
+    if (nthreads > st->ppr) nthreads = st->ppr;
+    double* terms = (nthreads > 1) ? (double*)malloc((size_t)st->total_particles * 2 * sizeof(double)) : NULL;
+    if (terms) {
+        term_job* jobs = (term_job*)calloc((size_t)nthreads, sizeof(term_job));
+        pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+        for (int t = 0; t < nthreads; t++) {
+            term_job jb = {st, tau, max_lt, raw, terms, (int)((int64_t)st->ppr * t / nthreads),
+                           (int)((int64_t)st->ppr * (t + 1) / nthreads)};
+            jobs[t] = jb;
+            pthread_create(&th[t], NULL, sigma_term_rows, &jobs[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        free(jobs);
+        free(th);
+        /* sim_index runs 0 .. P-1 in the (j, i, q, p) loop order */
+        for (int64_t idx = 0; idx < st->total_particles; idx++) {
+            const double txx = terms[2 * idx], txy = terms[2 * idx + 1];
+            total_xx += txx;
+            total_xy += txy;
+            neum_add(&cxx, txx);
+            neum_add(&cxy, txy);
+        }
+        free(terms);
+    } else {
+        for (int j = 0; j < st->ppr; j++) {
+            const double wy = orc_simpson_weight(j, st->ppr);
+            for (int i = 0; i < st->ppr; i++) {
+                const double wp = wy * orc_simpson_weight(i, st->ppr);
+                for (int q = 0; q < 4; q++)
+                    for (int p = 0; p < st->ppq; p++) {
+                        double txx, txy;
+                        sigma_term(st, tau, max_lt, raw[sim_index(st, i, j, q, p)], wp, q, p, &txx, &txy);
+                        total_xx += txx;
+                        total_xy += txy;
+                        neum_add(&cxx, txx);
+                        neum_add(&cxy, txy);
+                    }
             }
         }
     }
@@ -1079,7 +1140,7 @@ EXPORT int orc_compute_sigmas(const orc_grid* g, const orc_sim_params* prm, int 
     }
     for (int t = 0; t < nT; t++) {
         const double tau = get_tau(&st, temperatures[t], prm->incoherent_tau);
-        out[t] = derive_temperature(&st, tau, lts);
+        out[t] = derive_temperature(&st, tau, lts, prm->nthreads);
     }
     if (own) free(lts);
     return 0;
@@ -1105,7 +1166,7 @@ EXPORT int orc_derive_temperatures(const orc_grid* g, const orc_sim_params* prm,
     init_state(&st, prm, g, coherent, B);
     for (int t = 0; t < nT; t++) {
         const double tau = get_tau(&st, temperatures[t], prm->incoherent_tau);
-        out[t] = derive_temperature(&st, tau, raw_lifetimes);
+        out[t] = derive_temperature(&st, tau, raw_lifetimes, prm->nthreads);
     }
     return 0;
 }

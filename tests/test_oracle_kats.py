@@ -500,3 +500,17 @@ def test_get_angle_error_floor_near_top_of_orbit(oracle_esm):
                 far = max(far, d)
     assert 1e-9 < near <= 1.5e-8
     assert far <= 1e-15
+
+
+@pytest.mark.parametrize("nthreads", [3, 8])
+def test_oracle_threads_give_identical_sigma(lib, nthreads):
+    """Row-split lifetimes and temperature-stage terms, summed on one thread in reference order
+    (SimulationCPU.cpp:143-155): the same bits as one thread, for uneven row splits too."""
+    x, y = O.read_points(os.path.join(GOLDEN, "test_impurities.dat"))
+    g = O.OracleGrid.from_points(lib, x, y, 1e-7, 1e-7, 2e-9, 10)
+    out = []
+    for nt in (1, nthreads):
+        prm = O.sim_params(31, 7, 1e-7, 1e-7, g.cell_size, 1.0, 0.3, 1, 1.67834e5, nthreads=nt)
+        sig, _, _ = O.compute_sigmas(lib, g, prm, 0, 24.006, [1.0, 10.0, 300.0])
+        out.append([(s.xx_naive, s.xy_naive, s.xx_comp, s.xy_comp) for s in sig])
+    assert out[0] == out[1]
