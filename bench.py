@@ -131,7 +131,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="production")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 production line (profiling runs)")
@@ -204,27 +204,34 @@ def main():
         return st
 
     def timed(precision):
-        """K pipelined steps: two batches take turns, so the host adds the next step's
-        samples (and the GPU bins their fields) while the current step runs.  Then two
-        steps run alone, for the kernel times of the roofline."""
-        with es.Batch() as b0, es.Batch() as b1:
-            bufs = (b0, b1)
+        """K pipelined steps over three batches: step k's fields are drawn and binned
+        (prepare) before step k-1's trace is enqueued, so that binning runs ahead and
+        step k's launch finds its read-backs done and enqueues its trace at once; then
+        step k-2's results are collected.  Then two steps run alone, for the kernel
+        times of the roofline."""
+        with es.Batch() as b0, es.Batch() as b1, es.Batch() as b2:
+            bufs = (b0, b1, b2)
             for b in bufs:
                 b.set_precision(precision)
-            for _ in range(max(1, args.warmup)):
-                for b in bufs:
-                    production_entries(b)
-                    b.run(metrics=False)
-            runs = []
+
+            def pipelined(n):
+                production_entries(bufs[0])
+                bufs[0].prepare()
+                out = []
+                for k in range(n):
+                    if k + 1 < n:
+                        nxt = bufs[(k + 1) % 3]
+                        production_entries(nxt)
+                        nxt.prepare()
+                    bufs[k % 3].launch()
+                    if k > 0:
+                        out.append(bufs[(k - 1) % 3].collect())
+                out.append(bufs[(n - 1) % 3].collect())
+                return out
+
+            pipelined(max(1, args.warmup))
             t0 = time.perf_counter()
-            production_entries(bufs[0])
-            bufs[0].launch()
-            for k in range(args.steps):
-                cur, nxt = bufs[k % 2], bufs[(k + 1) % 2]
-                if k + 1 < args.steps:
-                    production_entries(nxt)
-                    nxt.launch()
-                runs.append(cur.collect())
+            runs = pipelined(args.steps)
             dt_ = time.perf_counter() - t0
             alone = []
             for _ in range(2):

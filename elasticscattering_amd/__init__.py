@@ -178,6 +178,7 @@ _SIGNATURES = {
     "es_batch_set_capture": (C.c_int, [_P, C.c_int32]),
     "es_batch_set_precision": (C.c_int, [_P, C.c_int32]),
     "es_batch_run": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
+    "es_batch_prepare": (C.c_int, [_P]),
     "es_batch_launch": (C.c_int, [_P]),
     "es_batch_collect": (C.c_int, [_P, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_batch_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -482,6 +483,11 @@ class Batch:
         The metrics of an entry are a numpy record array with the es_metrics fields
         (``mets[e][b]["cells_passed"]``)."""
         return self._results(self.lib.es_batch_run, metrics)
+
+    def prepare(self):
+        """Enqueues the binning of the fields added since the last run, and their read-backs,
+        without waiting (es_batch_prepare): launch() then only waits for the read-backs."""
+        self._check(self.lib.es_batch_prepare(self.h))
 
     def launch(self):
         """Enqueues the run and returns; :meth:`collect` waits for it (the host may prepare

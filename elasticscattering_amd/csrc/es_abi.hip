@@ -1781,6 +1781,14 @@ int es_batch_run(es_batch* bt, es_sigma* out, es_metrics* metrics, double* secon
     return ES_OK;
 }
 
+int es_batch_prepare(es_batch* bt)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
+    ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    return jobs_launch(bt->c, bt->pending);
+}
+
 int es_batch_launch(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;

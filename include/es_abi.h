@@ -257,6 +257,16 @@ int es_batch_run(es_batch* b, es_sigma* out, es_metrics* metrics, double* second
  */
 int es_batch_launch(es_batch* b);
 int es_batch_collect(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
+/*
+ * Enqueues the drawing, binning and start-cell tests of the fields added since
+ * the last run, and their read-backs, on the batch's stream, and returns
+ * without waiting; es_batch_launch (or es_batch_run) then only waits for those
+ * read-backs.  Calling it before another batch's trace is enqueued lets this
+ * batch's binning run ahead instead of queuing behind that trace.  A no-op when
+ * nothing is pending.  (No counterpart in the reference, which bins on the host
+ * when a grid is built, Grid.cpp:28-59; for callers that pipeline samples.)
+ */
+int es_batch_prepare(es_batch* b);
 /* Sizes of the batch: entries, (entry, field) pairs, sigma outputs, traced work items. */
 int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items);
 /* Times every trace launch shape on this batch and keeps the fastest. */

@@ -296,6 +296,38 @@ def test_launch_collect_pipelined_with_same_bits(es, batch_eager):
         _same_results(eager, b1.run())
 
 
+def test_prepare_ahead_of_another_trace_with_same_bits(es, batch_eager):
+    """es_batch_prepare enqueues a batch's binning before another batch's trace
+    is enqueued, three batches in the rotation of a pipelined loop: every run
+    gives the bits of es_batch_run; prepare with nothing pending is a no-op, and
+    a launched batch refuses it."""
+    entries, _, eager = batch_eager
+    announce("batch prepare: three batches, binning one step ahead")
+    with es.Batch() as b0, es.Batch() as b1, es.Batch() as b2:
+        bufs = (b0, b1, b2)
+        results = []
+        for e in entries:
+            _add(b0, e)
+        b0.prepare()
+        b0.prepare()  # nothing pending
+        n = 4
+        for k in range(n):
+            if k + 1 < n:
+                nxt = bufs[(k + 1) % 3]
+                nxt.reset()
+                for e in entries:
+                    _add(nxt, e)
+                nxt.prepare()
+            bufs[k % 3].launch()
+            with pytest.raises(es.EsError):
+                bufs[k % 3].prepare()  # launched, not collected
+            if k > 0:
+                results.append(bufs[(k - 1) % 3].collect())
+        results.append(bufs[(n - 1) % 3].collect())
+        for r in results:
+            _same_results(eager, r)
+
+
 def test_batch_errors(es):
     announce("batch argument errors")
     with es.Batch() as b:
