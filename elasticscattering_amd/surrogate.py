@@ -150,13 +150,20 @@ def features(B, T, density, radius, tau, alpha):
 S0 = 1e-7  # target scale: asinh is logarithmic well above it, linear below
 
 
-def targets(sig):
-    """asinh(sigma / S0) of sigma [..., 2] (both components)."""
-    return np.arcsinh(np.asarray(sig, dtype=np.float64) / S0)
+def targets(sig, scale=S0):
+    """asinh(sigma / scale) of sigma [..., 2] (both components)."""
+    return np.arcsinh(np.asarray(sig, dtype=np.float64) / scale)
 
 
-def from_targets(y):
-    return S0 * np.sinh(np.asarray(y, dtype=np.float64))
+def from_targets(y, scale=S0):
+    return scale * np.sinh(np.asarray(y, dtype=np.float64))
+
+
+# A curve is interpolated in asinh(sigma / s) with s this fraction of the
+# smallest |sigma| solved on it (at least S0): a component smaller than that
+# (sigma_xy crossing zero) is interpolated linearly, as the error measure
+# (relative to |sigma|) sees it, a larger one logarithmically.
+LINEAR_FRACTION = 1e-3
 
 
 # ------------------------------------------------------------- solver runs
@@ -328,11 +335,11 @@ def pchip(x, y, xe):
     return y[i] + t * (d0 + t * (c2 + t * c3))
 
 
-def _curve_from(u, idx, vals, at=None):
-    """PCHIP through the solved points in (log B, asinh(sigma / S0)) at the
+def _curve_from(u, idx, vals, at=None, scale=S0):
+    """PCHIP through the solved points in (log B, asinh(sigma / scale)) at the
     grid indices `at` (all by default); vals [len(idx)][...][2]."""
     ue = u if at is None else u[at]
-    y = targets(vals)
+    y = targets(vals, scale)
     if len(idx) == 1:
         return np.repeat(vals[:1], len(ue), axis=0)
     if len(idx) == 2:
@@ -341,17 +348,17 @@ def _curve_from(u, idx, vals, at=None):
         est = y[0] + t * (y[1] - y[0])
     else:
         est = pchip(u[idx], y, ue)
-    return from_targets(est)
+    return from_targets(est, scale)
 
 
-def _prior_errors(u, idx, prior, gaps):
+def _prior_errors(u, idx, prior, gaps, scale=S0):
     """Per gap (a, b) of consecutive solved indices, the largest error over its
     points and the ensemble members of interpolating each member's own curve
     through the solved indices (one interpolation for all members and gaps)."""
     at = np.concatenate([np.arange(a, b + 1) for a, b in gaps])
     starts = np.cumsum([0] + [b - a + 1 for a, b in gaps[:-1]])
     p = np.moveaxis(prior, 0, 1)  # [n][members][2]
-    err = tensor_error(_curve_from(u, idx, p[idx], at), p[at]).max(axis=1)
+    err = tensor_error(_curve_from(u, idx, p[idx], at, scale), p[at]).max(axis=1)
     return np.maximum.reduceat(err, starts)
 
 
@@ -371,10 +378,17 @@ class _Curve:
         self.rough = 0          # points found rough at grid scale
         self.escaped = {}       # index -> escaped fraction of the solver run
 
+    def scale(self):
+        """The interpolation scale of this curve (LINEAR_FRACTION)."""
+        if not self.solved:
+            return S0
+        m = min(float(np.hypot(*v)) for v in self.solved.values())
+        return max(S0, LINEAR_FRACTION * m)
+
     def estimate(self):
         idx = np.array(sorted(self.solved))
         vals = np.array([self.solved[i] for i in idx])
-        est = _curve_from(self.u, idx, vals)
+        est = _curve_from(self.u, idx, vals, scale=self.scale())
         est[idx] = vals
         return est
 
@@ -489,7 +503,8 @@ def _rough_points(c, points, n, tol_r):
         return []
     idx = np.array(sorted(c.solved))
     vals = np.array([c.solved[i] for i in idx])
-    y = targets(vals)
+    sc = c.scale()
+    y = targets(vals, sc)
     cand = np.array(cand)
     pos = np.searchsorted(idx, cand)
     nl = pos - np.maximum(pos - 3, 0)
@@ -504,7 +519,7 @@ def _rough_points(c, points, n, tol_r):
             e = Y[:, 0] + t * (Y[:, 1] - Y[:, 0])
         else:
             e = _pchip_rows(X, Y, xe)
-        est[sel] = from_targets(e)
+        est[sel] = from_targets(e, sc)
     err = tensor_error(est, vals[pos])
     return [int(k) for k, e in zip(cand, err) if e > tol_r]
 
@@ -531,7 +546,7 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
                 calm = [g for g in gaps if max(c.escaped.get(g[0], 1.0), c.escaped.get(g[1], 1.0)) <= o["escape_max"]]
                 if c.prior is not None and calm:
                     idx = np.array(sorted(c.solved))
-                    keep = _prior_errors(u, idx, c.prior, calm) > kappa * tol
+                    keep = _prior_errors(u, idx, c.prior, calm, c.scale()) > kappa * tol
                     c.skipped += int(np.count_nonzero(~keep))
                     closed = {g for g, k in zip(calm, keep) if not k}
                     gaps = [g for g in gaps if g not in closed]
