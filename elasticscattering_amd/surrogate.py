@@ -403,7 +403,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
 
     Each curve starts from n_initial solver points spread evenly over the
-    grid and is interpolated with PCHIP in (log B, asinh(sigma / S0)).  Every
+    grid and is interpolated with PCHIP in (log B, asinh(sigma / s)), s per curve (LINEAR_FRACTION).  Every
     gap between consecutive solver points is resolved by a solver run at its
     midpoint, which is also a check: the gap is done when the interpolant
     missed that point by at most safety * tol, otherwise both halves go on; a
