@@ -12,7 +12,7 @@ namespace es {
 
 constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kMaxT = 8;           // temperatures per sample
-constexpr int kSigmaItemsPerThread = 8;  // temperature stage: work items per thread (block count rule)
+constexpr int kSigmaItemsPerThread = 32;  // temperature stage: work items per thread (block count rule)
 // int64 metric partials per block: 0 cells, 1 tested, 2 escaped, 3 at_bound,
 // 4 cap hits, 5 max tested (max), 6 max cells (max), 7 traced particles,
 // 8 edge hits, 9 cross-time calls, 10 sigma evaluations, 11 exact cross-time
