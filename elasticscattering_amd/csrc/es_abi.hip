@@ -218,7 +218,7 @@ int cells_per_row_rule(int64_t unique, int target, int override_cpr)
 // of TraceOrbit, lifetime.h:84-99, hoisted: it depends on neither phi nor B).
 struct Field {
     DevBuf in_x, in_y, ixy, cell_end;
-    DevBuf b_targets, b_ntarg, b_counts, b_fill, b_begin, b_bsum, b_order;
+    DevBuf b_targets, b_ntarg, b_counts, b_slot, b_begin, b_bsum, b_order;
     DevBuf flags, bcount, free_pos, small;  // small: [0] out-of-grid flag, [1] nfree
     es::GridDev g;
     int64_t unique = 0;
@@ -239,7 +239,7 @@ struct Field {
 
 void field_release(Ctx& c, Field& f)
 {
-    DevBuf* bufs[] = {&f.in_x, &f.in_y, &f.ixy, &f.cell_end, &f.b_targets, &f.b_ntarg, &f.b_counts, &f.b_fill,
+    DevBuf* bufs[] = {&f.in_x, &f.in_y, &f.ixy, &f.cell_end, &f.b_targets, &f.b_ntarg, &f.b_counts, &f.b_slot,
                       &f.b_begin, &f.b_bsum, &f.b_order, &f.flags, &f.bcount, &f.free_pos, &f.small};
     for (DevBuf* b : bufs) release(c, *b);
     if (f.hsmall) (void)hipHostFree(f.hsmall);
@@ -398,11 +398,11 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, Jobs& q, bool 
     const int64_t max_total = 9 * n;
     int rc;
     if ((rc = ensure(c, f.b_targets, 9 * n * sizeof(int32_t), "bin targets"))) return rc;
+    if ((rc = ensure(c, f.b_slot, 9 * n * sizeof(int32_t), "bin slots"))) return rc;
     if ((rc = ensure(c, f.b_ntarg, n, "bin target counts"))) return rc;
     const size_t had = f.b_counts.cap;
     if ((rc = ensure(c, f.b_counts, ncell * sizeof(int32_t), "cell counts"))) return rc;
     if (f.b_counts.cap != had) f.counts_clean = false;
-    if ((rc = ensure(c, f.b_fill, ncell * sizeof(int32_t), "cell fill"))) return rc;
     if ((rc = ensure(c, f.b_begin, (ncell + 1) * sizeof(int64_t), "cell begin"))) return rc;
     if ((rc = ensure(c, f.b_bsum, ((ncell + 255) / 256) * sizeof(int64_t), "scan block sums"))) return rc;
     if ((rc = ensure(c, f.cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
@@ -427,7 +427,7 @@ int field_bin(Ctx& c, Field& f, int64_t n, const es_grid_desc* d, Jobs& q, bool 
     j.radius = g.radius;
     j.cpr = cpr;
     j.s = es::BinScratch{P<int32_t>(f.b_targets), P<uint8_t>(f.b_ntarg), P<int32_t>(f.b_counts),
-                         P<int32_t>(f.b_fill),    P<int64_t>(f.b_begin),  P<int64_t>(f.b_bsum),
+                         P<int32_t>(f.b_slot),    P<int64_t>(f.b_begin),  P<int64_t>(f.b_bsum),
                          P<int32_t>(f.b_order)};
     j.cell_end = g.cell_end;
     j.ixy = g.ixy;

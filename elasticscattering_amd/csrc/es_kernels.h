@@ -113,7 +113,7 @@ struct BinScratch {
     int32_t* targets;   // 9 per impurity
     uint8_t* ntarg;     // per impurity
     int32_t* counts;    // per cell; zero on entry, left zero
-    int32_t* fill;      // per cell
+    int32_t* slot;      // 9 per impurity: its entry's rank in each target cell
     int64_t* begin;     // per cell + 1
     int64_t* bsum;      // per scan block
     int32_t* order;     // per indexed impurity

@@ -2,12 +2,12 @@
 //
 //   k_bin_count     per impurity (drawn first from es_rng.h when generating,
 //                   the Grid.cpp:61-75 analogue): target cells (Grid.cpp:
-//                   112-180), per-cell counts
+//                   112-180), per-cell counts, each entry's slot in its cell
 //   k_scan_one /
 //   k_scan_block,
 //   k_scan_add      exclusive scan -> cell begin offsets + the reference's
 //                   inclusive ends (one block up to 32,768 cells)
-//   k_bin_fill      scatter impurity ids into their cells
+//   k_bin_fill      scatter impurity ids into their slots (no atomics)
 //   k_bin_sort_gather  ascending impurity id inside each cell (the
 //                   reference's order) and the cell-ordered (x, y) pairs, one
 //                   wave per cell
@@ -197,9 +197,11 @@ __global__ __launch_bounds__(256) void k_bin_count(const BinJob* jobs)
     int32_t t[9];
     const int k = bin_targets(g, px, py, t);
     J.s.ntarg[i] = (uint8_t)k;
+    // The count's old value is the entry's slot in its cell (any order: the
+    // cell is sorted by impurity id afterwards).
     for (int m = 0; m < k; m++) {
         J.s.targets[9 * i + m] = t[m];
-        atomicAdd(&J.s.counts[t[m]], 1);
+        J.s.slot[9 * i + m] = atomicAdd(&J.s.counts[t[m]], 1);
     }
 }
 
@@ -251,8 +253,8 @@ __global__ __launch_bounds__(256) void k_scan_block(const BinJob* job)
     if (threadIdx.x == 0) J.s.bsum[blockIdx.x] = total;
 }
 
-// Second level of the two-level scan; also clears the fill counters for
-// k_bin_fill and the counts for the next binning, and writes the total.
+// Second level of the two-level scan; also clears the counts for the next
+// binning, and writes the total.
 __global__ __launch_bounds__(256) void k_scan_add(const BinJob* job)
 {
     const BinJob& J = *job;
@@ -265,7 +267,6 @@ __global__ __launch_bounds__(256) void k_scan_add(const BinJob* job)
         const int32_t cnt = J.s.counts[c];
         J.s.begin[c] = b;
         J.cell_end[c] = (int32_t)(b + cnt);
-        J.s.fill[c] = 0;
         J.s.counts[c] = 0;
         if (c == n - 1) {
             J.s.begin[n] = b + cnt;
@@ -291,12 +292,8 @@ __global__ __launch_bounds__(1024) void k_scan_one(const BinJob* jobs)
     const int tid = threadIdx.x;
     const int ni = (int)n;
     int32_t* __restrict__ counts = J.s.counts;
-    int32_t* __restrict__ fill = J.s.fill;
     for (int c = tid; c < ni; c += 1024) sh_cnt[c] = counts[c];
-    for (int c = tid; c < ni; c += 1024) {
-        counts[c] = 0;
-        fill[c] = 0;
-    }
+    for (int c = tid; c < ni; c += 1024) counts[c] = 0;
     __syncthreads();
     const int per = (ni + 1023) / 1024;
     const int c0 = tid * per < ni ? tid * per : ni, c1 = (c0 + per < ni) ? c0 + per : ni;
@@ -334,12 +331,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const BinJob* jobs)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= J.n) return;
     const int k = J.s.ntarg[i];
-    int32_t c[9];
-    for (int m = 0; m < k; m++) c[m] = J.s.targets[9 * i + m];
-    for (int m = 0; m < k; m++) {
-        const int slot = atomicAdd(&J.s.fill[c[m]], 1);
-        J.s.order[J.s.begin[c[m]] + slot] = (int32_t)i;
-    }
+    for (int m = 0; m < k; m++) J.s.order[J.s.begin[J.s.targets[9 * i + m]] + J.s.slot[9 * i + m]] = (int32_t)i;
 }
 
 __device__ void sift_down(int32_t* a, int64_t root, int64_t n)
