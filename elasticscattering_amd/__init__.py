@@ -185,6 +185,7 @@ _SIGNATURES = {
                                   C.POINTER(C.c_int64)]),
     "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
     "es_version": (C.c_char_p, []),
+    "es_math_eval": (C.c_int, [C.c_int32, C.c_int32, _PD, _PD, C.c_int64]),
     # include/es_surrogate.h
     "es_mlp_create": (C.c_int, [C.c_void_p, C.POINTER(_P)]),
     "es_mlp_destroy": (None, [_P]),
@@ -256,6 +257,20 @@ def log_images(path, lifetimes, sigma_xx, sigma_xy):
     rc = load_library().es_log_images(str(path).encode(), dim, _dptr(lt), _dptr(xx), _dptr(xy))
     if rc != ES_OK:
         raise EsError(rc, f"cannot write {path}")
+
+
+MATH_FUNCTIONS = {"sin": 0, "cos": 1, "asin": 2, "exp": 3}  # ES_MATH_* of include/es_abi.h
+
+
+def math_eval(fn, x, device=0):
+    """The device's correctly rounded sin / cos / asin / exp (es_math.h) applied to x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib = load_library()
+    rc = lib.es_math_eval(int(device), MATH_FUNCTIONS[fn], _dptr(x), _dptr(y), x.size)
+    if rc != ES_OK:
+        raise EsError(rc, lib.es_last_error(None).decode())
+    return y
 
 
 def _dptr(a):

@@ -1195,9 +1195,61 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
 
 }  // namespace
 
+// Device elementary functions over an array (es_math_eval).
+__global__ __launch_bounds__(256) void k_math_eval(int fn, const double* x, double* y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    double r, s, c;
+    switch (fn) {
+    case ES_MATH_SIN: esm_sincos(v, &s, &c); r = s; break;
+    case ES_MATH_COS: esm_sincos(v, &s, &c); r = c; break;
+    case ES_MATH_ASIN: r = esm_asin(v); break;
+    default: r = esm_exp(v); break;
+    }
+    y[i] = r;
+}
+
 extern "C" {
 
-const char* es_version(void) { return "es-mi355x 0.3 (gfx950, FP64 + mixed FP32/FP64, batched, MLP surrogate)"; }
+const char* es_version(void) { return "es-mi355x 0.4 (gfx950, FP64 with correctly rounded sin/cos/asin/exp + mixed FP32/FP64, batched, MLP surrogate)"; }
+
+int es_math_eval(int32_t device, int32_t fn, const double* x, double* y, int64_t n)
+{
+    if (fn < ES_MATH_SIN || fn > ES_MATH_EXP || n < 0 || (n > 0 && (!x || !y))) {
+        g_create_error = "es_math_eval: bad function id, size or pointer";
+        return ES_ERR_INVALID_ARG;
+    }
+    if (n == 0) return ES_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "no HIP device";
+        return ES_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev || hipSetDevice(device) != hipSuccess) {
+        g_create_error = "es_math_eval: bad device";
+        return ES_ERR_INVALID_ARG;
+    }
+    double* d = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    if (hipMalloc(&d, 2 * bytes) != hipSuccess) {
+        g_create_error = "es_math_eval: device allocation failed";
+        return ES_ERR_OOM;
+    }
+    hipError_t e = hipMemcpy(d, x, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, d, d + n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(y, d + n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        g_create_error = std::string("es_math_eval: ") + hipGetErrorString(e);
+        return ES_ERR_HIP;
+    }
+    return ES_OK;
+}
 
 int es_solver_create(const es_solver_desc* desc, es_solver** out)
 {

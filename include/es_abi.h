@@ -272,6 +272,21 @@ int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* 
 /* Times every trace launch shape on this batch and keeps the fastest. */
 int es_batch_tune(es_batch* b, es_batch_shape* best, double* best_trace_seconds);
 
+/*
+ * The device's elementary functions (elasticscattering_amd/csrc/es_math.h:
+ * correctly rounded sin, cos, asin, exp, the ones the kernels call) applied to
+ * n host values on `device`; y receives the results.  No reference
+ * counterpart: the reference calls the C runtime's libm (details.h:100,
+ * lifetime.h:43, util.h:34-46); this lets a caller compare the device's
+ * results with any other implementation.  The message of a failure is
+ * es_last_error(NULL).
+ */
+#define ES_MATH_SIN 0
+#define ES_MATH_COS 1
+#define ES_MATH_ASIN 2
+#define ES_MATH_EXP 3
+int es_math_eval(int32_t device, int32_t fn, const double* x, double* y, int64_t n);
+
 /* Build/version string of the library. */
 const char* es_version(void);
 

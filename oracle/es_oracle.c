@@ -7,13 +7,17 @@
  * Each function cites the reference file:line it follows, operation by
  * operation (the same evaluation order; build with -ffp-contract=off).
  *
- * Two builds:
+ * Three builds:
  *   liboracle_libm.so  — glibc sin/cos/asin/exp (the reference calls the C
  *                        runtime's libm; this build is compared bit for bit
  *                        with the reference headers compiled under oracle/_ref)
- *   liboracle_esm.so   — the project's FP64 math (es_math.h), which is also
- *                        what the GPU kernels call; the GPU lifetimes are
- *                        compared bit for bit with this build.
+ *   liboracle_esm.so   — the project's correctly rounded FP64 math
+ *                        (es_math.h), which is also what the GPU kernels call;
+ *                        the GPU lifetimes are compared bit for bit with it.
+ *   liboracle_esm_libm_asin.so — es_math.h except glibc's asin: the two
+ *                        builds above differ only where glibc misrounds, and
+ *                        this one shows which function those differences
+ *                        come from.
  *
  * Deviations kept explicit (each is a documented decision, not a fix):
  *   - pow(v, 2) is evaluated as v*v (the correctly rounded square).
@@ -44,6 +48,10 @@
 #define O_COS esm_cos
 #define O_ASIN esm_asin
 #define O_EXP esm_exp
+#ifdef ORACLE_ASIN_LIBM
+#undef O_ASIN
+#define O_ASIN asin
+#endif
 #else
 #define O_SIN sin
 #define O_COS cos

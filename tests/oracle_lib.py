@@ -1,10 +1,13 @@
 """ctypes access to the CPU oracle (oracle/es_oracle.c) for the tests.
 
 Test infrastructure: this module and the oracle it loads are the checker, never
-the thing measured or shipped.  Three libraries:
+the thing measured or shipped.  Four libraries:
 
-* ``esm``  — oracle built on the project's es_math.h (the GPU's math);
+* ``esm``  — oracle built on the project's correctly rounded es_math.h (the
+  GPU's math);
 * ``libm`` — the same restatement on glibc's sin/cos/asin/exp;
+* ``esm_libm_asin`` — es_math.h except glibc's asin (attributes the
+  differences between the two builds above to one function);
 * ``ref``  — the reference's own kernel headers compiled from the reference
   tree (oracle/_ref/libesref.so, only where that tree was present at build).
 """
@@ -116,6 +119,7 @@ def lib_path(kind):
     return {
         "esm": os.path.join(ORACLE_DIR, "liboracle_esm.so"),
         "libm": os.path.join(ORACLE_DIR, "liboracle_libm.so"),
+        "esm_libm_asin": os.path.join(ORACLE_DIR, "liboracle_esm_libm_asin.so"),
         "ref": os.path.join(ORACLE_DIR, "_ref", "libesref.so"),
     }[kind]
 
@@ -247,3 +251,28 @@ def derive_images(lib, grid, params, coherent, B, raw_lifetimes, temperature, pp
     rc = lib.derive_images(grid.h, C.byref(params), int(coherent), float(B), dptr(lt), float(temperature), dptr(out))
     assert rc == 0
     return out.reshape(3, ppr, ppr)
+
+
+# Cases on which the glibc and correctly rounded builds (and the GPU) are compared.
+AGREEMENT_FIELDS = [0.01, 0.1, 1.0, 20.0, 40.0]
+
+
+def agreement_case(lib, case):
+    """(grid, params, temperatures) of the three fields the libm/esm agreement is checked on:
+    the reference's test_impurities.dat (SigmaIntegrationTest.h:17-19 settings), the small
+    FP64 workload (2,000 disks, 64^2 positions, 17 phi) and the production field (60,000
+    disks, 128^2 cells) at 64^2 positions, 65 phi, T in {2, 40, 100}."""
+    from elasticscattering_amd import workloads as W
+
+    if case == "file1":
+        x, y = read_points(os.path.join(ROOT, "tests", "golden", "test_impurities.dat"))
+        g = OracleGrid.from_points(lib, x, y, 1e-7, 1e-7, 2e-9, 10)
+        return g, sim_params(31, 7, 1e-7, 1e-7, g.cell_size, 1.0, 0.3, 1, 1.67834e5, nthreads=8), [1.0, 10.0]
+    w = W.SMALL_FP64 if case == "small" else W.PRODUCTION_FP64
+    rs, ext = W.region(w)
+    g = OracleGrid.generate(lib, w["seeds"][0], rs, ext, 0.0, w["impurity_radius"], 15, w["cells_per_row"],
+                            n=w["impurity_count"])
+    if case == "small":
+        return g, sim_params(64, 17, rs, ext, g.cell_size, 1e-12, w["alpha"], 1, 7e5, incoherent_tau=1e-12,
+                             nthreads=8), [0.0]
+    return g, sim_params(64, 65, rs, ext, g.cell_size, 1e-12, w["alpha"], 1, 7e5, nthreads=8), [2.0, 40.0, 100.0]
