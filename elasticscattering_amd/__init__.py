@@ -37,6 +37,8 @@ ES_ERR_NO_SAMPLE = -4
 ES_ERR_OOM = -5
 ES_ERR_OUT_OF_GRID = -6
 ES_ERR_NO_DEVICE = -7
+ES_NOALLOC_MAX_IMPURITIES = 131072  # es_grid_bin_noalloc scratch limits (include/es_abi.h)
+ES_NOALLOC_MAX_CELLS = 65536
 ES_PRECISION_FP64 = 0
 ES_PRECISION_MIXED = 1
 _PRECISIONS = {"fp64": ES_PRECISION_FP64, "mixed": ES_PRECISION_MIXED}
@@ -186,6 +188,8 @@ _SIGNATURES = {
     "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
     "es_version": (C.c_char_p, []),
     "es_math_eval": (C.c_int, [C.c_int32, C.c_int32, _PD, _PD, C.c_int64]),
+    "es_grid_bin_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(GridDesc), C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.POINTER(GridInfo), C.c_void_p]),
     # include/es_surrogate.h
     "es_mlp_create": (C.c_int, [C.c_void_p, C.POINTER(_P)]),
     "es_mlp_destroy": (None, [_P]),

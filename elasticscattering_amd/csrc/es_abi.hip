@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1195,6 +1196,68 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
 
 }  // namespace
 
+// ------------------------------------------------------ binning without allocation
+//
+// es_grid_bin_noalloc bins into caller-provided device buffers; its scratch
+// (per-impurity targets and slots, per-cell counts and offsets, the job table)
+// is this file-scope array, one copy per GPU for the life of the process.  Host
+// code never passes `scratch` itself to a kernel: its address on each device
+// comes from hipGetSymbolAddress (g_noalloc_scratch below).
+constexpr int64_t kNoAllocMaxImpurities = ES_NOALLOC_MAX_IMPURITIES;
+constexpr int64_t kNoAllocMaxCells = ES_NOALLOC_MAX_CELLS;
+constexpr size_t kNoAllocAlign = 256;
+
+constexpr size_t noalloc_round(size_t b) { return (b + kNoAllocAlign - 1) & ~(kNoAllocAlign - 1); }
+
+// Layout of the scratch array (byte offsets).
+constexpr size_t kNaJob = 0;
+constexpr size_t kNaTargets = kNaJob + noalloc_round(sizeof(es::BinJob));
+constexpr size_t kNaSlot = kNaTargets + noalloc_round(9 * kNoAllocMaxImpurities * sizeof(int32_t));
+constexpr size_t kNaOrder = kNaSlot + noalloc_round(9 * kNoAllocMaxImpurities * sizeof(int32_t));
+constexpr size_t kNaNtarg = kNaOrder + noalloc_round(9 * kNoAllocMaxImpurities * sizeof(int32_t));
+constexpr size_t kNaCounts = kNaNtarg + noalloc_round(kNoAllocMaxImpurities * sizeof(uint8_t));
+constexpr size_t kNaBegin = kNaCounts + noalloc_round(kNoAllocMaxCells * sizeof(int32_t));
+constexpr size_t kNaBsum = kNaBegin + noalloc_round((kNoAllocMaxCells + 1) * sizeof(int64_t));
+constexpr size_t kNaBytes = kNaBsum + noalloc_round(((kNoAllocMaxCells + 255) / 256) * sizeof(int64_t));
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// The job table is written by a kernel from its by-value argument (no host
+// staging, no copy), and the counts are cleared by a kernel (no memset node if
+// a caller captures the stream).
+__global__ void k_noalloc_put_job(es::BinJob job, es::BinJob* dst) { *dst = job; }
+
+__global__ __launch_bounds__(256) void k_noalloc_clear(int32_t* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+std::mutex g_noalloc_mu;
+void* g_noalloc_scratch[64] = {nullptr};  // device address of `scratch` per device ordinal
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there (hipGetSymbolAddress with that device current).
+int noalloc_scratch(int device, unsigned char** out, std::string& err)
+{
+    if (device < 0 || device >= 64) {
+        err = "device ordinal out of range";
+        return ES_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return ES_ERR_HIP;
+        }
+        g_noalloc_scratch[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_noalloc_scratch[device]);
+    return ES_OK;
+}
+
 // Device elementary functions over an array (es_math_eval).
 __global__ __launch_bounds__(256) void k_math_eval(int fn, const double* x, double* y, int64_t n)
 {
@@ -1214,6 +1277,91 @@ __global__ __launch_bounds__(256) void k_math_eval(int fn, const double* x, doub
 extern "C" {
 
 const char* es_version(void) { return "es-mi355x 0.4 (gfx950, FP64 with correctly rounded sin/cos/asin/exp + mixed FP32/FP64, batched, MLP surrogate)"; }
+
+int es_grid_bin_noalloc(const double* x, const double* y, int64_t n, const es_grid_desc* desc, double* ordered_xy,
+                        int32_t* imp_index, int64_t* indexed_count, es_grid_info* out_info, void* stream)
+{
+    std::string& err = g_create_error;
+    if (!desc || n < 0 || (n > 0 && (!x || !y)) || !ordered_xy || !imp_index || !indexed_count) {
+        err = "es_grid_bin_noalloc: null pointer or negative count";
+        return ES_ERR_INVALID_ARG;
+    }
+    if (n > kNoAllocMaxImpurities) {
+        err = "es_grid_bin_noalloc: more than " + std::to_string(kNoAllocMaxImpurities) +
+              " impurities (the size of its fixed scratch)";
+        return ES_ERR_INVALID_ARG;
+    }
+    if (desc->target_cell_population <= 0 && desc->cells_per_row <= 0) {
+        err = "target_cell_population must be > 0";
+        return ES_ERR_INVALID_ARG;
+    }
+    if (!(desc->impurity_radius >= 0)) {
+        err = "impurity_radius must be >= 0";
+        return ES_ERR_INVALID_ARG;
+    }
+    if (!(desc->region_size + 2 * desc->region_extends > 0)) {
+        err = "empty spawn region";
+        return ES_ERR_INVALID_ARG;
+    }
+    const int cpr = cells_per_row_rule(n, desc->target_cell_population, desc->cells_per_row);
+    if ((int64_t)cpr * cpr > kNoAllocMaxCells) {
+        err = "es_grid_bin_noalloc: more than " + std::to_string(kNoAllocMaxCells) +
+              " cells (the size of its fixed scratch)";
+        return ES_ERR_INVALID_ARG;
+    }
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) {
+        (void)hipGetLastError();
+        err = "no HIP device";
+        return ES_ERR_NO_DEVICE;
+    }
+    unsigned char* base = nullptr;
+    int rc = noalloc_scratch(device, &base, err);
+    if (rc != ES_OK) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t ncell = (int64_t)cpr * cpr;
+    es::BinJob j{};
+    j.x = const_cast<double*>(x);  // read only: generate = 0
+    j.y = const_cast<double*>(y);
+    j.n = n;
+    j.generate = 0;
+    j.lo = -desc->region_extends;  // Grid.cpp:52
+    j.size = (desc->region_size + desc->region_extends) - j.lo;
+    j.cell_size = j.size / (double)cpr;  // Grid.cpp:55
+    j.radius = desc->impurity_radius;
+    j.cpr = cpr;
+    j.s = es::BinScratch{reinterpret_cast<int32_t*>(base + kNaTargets), reinterpret_cast<uint8_t*>(base + kNaNtarg),
+                         reinterpret_cast<int32_t*>(base + kNaCounts),  reinterpret_cast<int32_t*>(base + kNaSlot),
+                         reinterpret_cast<int64_t*>(base + kNaBegin),   reinterpret_cast<int64_t*>(base + kNaBsum),
+                         reinterpret_cast<int32_t*>(base + kNaOrder)};
+    j.cell_end = imp_index;
+    j.ixy = reinterpret_cast<double2*>(ordered_xy);
+    j.total_out = indexed_count;
+    es::BinJob* djob = reinterpret_cast<es::BinJob*>(base + kNaJob);
+    hipLaunchKernelGGL(k_noalloc_clear, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, j.s.counts, ncell);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_noalloc_put_job, dim3(1), dim3(1), 0, st, j, djob);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = es::launch_bin_jobs(djob, &j, 1, st);
+    if (e != hipSuccess) {
+        err = std::string("es_grid_bin_noalloc: ") + hipGetErrorString(e);
+        return ES_ERR_HIP;
+    }
+    if (out_info) {
+        es_grid_info info{};
+        info.region_size = desc->region_size;
+        info.region_extends = desc->region_extends;
+        info.cell_size = j.cell_size;
+        info.indexed_impurity_count = -1;  // on the device: *indexed_count once the stream has run
+        info.unique_impurity_count = n;
+        info.index_size = (int32_t)ncell;
+        info.cells_per_row = cpr;
+        *out_info = info;
+    }
+    return ES_OK;
+}
 
 int es_math_eval(int32_t device, int32_t fn, const double* x, double* y, int64_t n)
 {

@@ -273,6 +273,29 @@ int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* 
 int es_batch_tune(es_batch* b, es_batch_shape* best, double* best_trace_seconds);
 
 /*
+ * Binning without device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory at the point of the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It bins the n impurity centres x, y (device arrays) as
+ * es_grid_set_impurities + es_grid_read do (Grid.cpp:48-59, 77-180) into
+ * caller-provided device buffers: ordered_xy (capacity 9 n (x, y) pairs: a disk
+ * lands in at most 9 cells), imp_index (cells_per_row^2 int32, inclusive ends)
+ * and indexed_count (one int64).  Work is enqueued on `stream` (a hipStream_t;
+ * NULL for the default stream) and the call returns without waiting;
+ * out_info (may be NULL) receives the geometry, with indexed_impurity_count -1
+ * (the count is on the device).  Its scratch is a fixed array in the library
+ * (about 16 MB per GPU, held until the process ends): at most
+ * ES_NOALLOC_MAX_IMPURITIES centres and ES_NOALLOC_MAX_CELLS cells, larger
+ * inputs are ES_ERR_INVALID_ARG.  Calls that overlap on one GPU share that
+ * scratch and race: serialise them (one stream, or wait between them).
+ * The message of a failure is es_last_error(NULL).
+ */
+#define ES_NOALLOC_MAX_IMPURITIES 131072
+#define ES_NOALLOC_MAX_CELLS 65536
+int es_grid_bin_noalloc(const double* x, const double* y, int64_t n, const es_grid_desc* desc, double* ordered_xy,
+                        int32_t* imp_index, int64_t* indexed_count, es_grid_info* out_info, void* stream);
+
+/*
  * The device's elementary functions (elasticscattering_amd/csrc/es_math.h:
  * correctly rounded sin, cos, asin, exp, the ones the kernels call) applied to
  * n host values on `device`; y receives the results.  No reference
