@@ -141,54 +141,73 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
     return tt;
 }
 
-// One of the 8 points where the orbit meets the lines of the cell's edges
-// (UpdateFirstBoundaryIntersect, cell_grid.h:91-129): edge e in the
-// reference order — top (0,+1), bottom (0,-1), right (+1,0), left (-1,0) —
-// and point k (i1, i2).  Returns false when the edge line misses the circle;
-// else the point and whether it lies inside the edge segment.
-struct CellCorners {
-    double llx, lly, lrx, lry, trx, try_, tlx, tly, qx, qy;
+// The cell's edges as UpdateFirstBoundaryIntersect (cell_grid.h:91-129) sees
+// them.  With low_left = (X0, Y0) the corners are (X0, Y0), (X1, Y0), (X1, Y1),
+// (X0, Y1), X1 = X0 + L, Y1 = Y0 + L (cell_grid.h:140-143); the edge direction
+// u = (p2 - p1) / L is (qx, 0) for the horizontal edges and (0, qy) for the
+// vertical ones (0 / L = +0), qx = (X1 - X0) / L and qy = (Y1 - Y0) / L.  Both
+// horizontal edges start at x = X0, so their projection distance
+// u . (centre - p1) = qx (cx - X0) + 0 (cy - Y) and projected x are one value,
+// and both vertical edges share the projected y: adding the +-0 of the zero
+// component changes at most the sign of a zero, which no later comparison or
+// difference of the path sees.  PointInSegment(a, l0, l1) (cell_grid.h:69-72)
+// is l0 < a < l1 here, as l0 < l1.
+struct CellEdges {
+    double X0, X1, Y0, Y1;  // edge lines
+    double qx, qy;          // edge directions
+    double projx, projy;    // projection of the centre onto the horizontal / vertical edge lines
+    double dxh, dyv;        // projx - cx, projy - cy
 };
 
-__device__ __forceinline__ bool d_edge_point(const Orb& o, const CellCorners& c, int e, int k, double& ix, double& iy,
-                                             bool& in_segment)
+__device__ __forceinline__ void d_cell_edges(const Orb& o, const DevGrid& g, int cx, int cy, CellEdges& c)
 {
-    // corner selection by flags (selects, not a table in scratch)
-    const bool top = e == 0, bot = e == 1, right = e == 2, horiz = e < 2;
-    double p1x = c.llx, p1y = c.lly;
-    p1x = top ? c.tlx : p1x;
-    p1y = top ? c.tly : p1y;
-    p1x = right ? c.lrx : p1x;
-    p1y = right ? c.lry : p1y;
-    double p2x = c.tlx, p2y = c.tly;
-    p2x = bot ? c.lrx : p2x;
-    p2y = bot ? c.lry : p2y;
-    p2x = (top || right) ? c.trx : p2x;
-    p2y = (top || right) ? c.try_ : p2y;
-    const double ux = horiz ? c.qx : 0.0, uy = horiz ? 0.0 : c.qy;
-    const double pd = ux * (o.cx - p1x) + uy * (o.cy - p1y);
-    const double projx = p1x + ux * pd, projy = p1y + uy * pd;
-    const double dx = projx - o.cx, dy = projy - o.cy;
+    const double L = g.L;
+    c.X0 = g.start + (double)cx * L;  // to_world (cell_grid.h:60-63)
+    c.Y0 = g.start + (double)cy * L;
+    c.X1 = c.X0 + L;
+    c.Y1 = c.Y0 + L;
+    c.qx = esm_div_rcp(c.X1 - c.X0, L, g.inv_L);
+    c.qy = esm_div_rcp(c.Y1 - c.Y0, L, g.inv_L);
+    c.projx = c.X0 + c.qx * (c.qx * (o.cx - c.X0));
+    c.projy = c.Y0 + c.qy * (c.qy * (o.cy - c.Y0));
+    c.dxh = c.projx - o.cx;
+    c.dyv = c.projy - o.cy;
+}
+
+// Edge e in the reference order — top (0,+1), bottom (0,-1), right (+1,0),
+// left (-1,0): false when its line misses the circle, else the half chord t
+// (|u| sqrt(R^2 - distance^2), signed by u) and the projected coordinate along
+// the edge; point k lies at base + t (k = 0) or base - t (k = 1).
+__device__ __forceinline__ bool d_edge_chord(const Orb& o, const CellEdges& c, int e, double& base, double& t)
+{
+    const bool h = e < 2;
+    const double dx = h ? c.dxh : (((e == 2) ? c.X1 : c.X0) - o.cx);
+    const double dy = h ? (((e == 0) ? c.Y1 : c.Y0) - o.cy) : c.dyv;
     const double pcd = dx * dx + dy * dy;
     if (pcd >= o.R2) return false;
-    const double s = sqrt(o.R2 - pcd);
-    const double tx = ux * s, ty = uy * s;
-    const bool horizontal = fabs(ux) > fabs(uy);
-    const double l0 = horizontal ? p1x : p1y, l1 = horizontal ? p2x : p2y;
-    ix = k ? (projx - tx) : (projx + tx);
-    iy = k ? (projy - ty) : (projy + ty);
-    const double a = horizontal ? ix : iy;
-    in_segment = (a > l0 && a < l1) || (a < l0 && a > l1);
+    t = (h ? c.qx : c.qy) * sqrt(o.R2 - pcd);
+    base = h ? c.projx : c.projy;
     return true;
+}
+
+// Point k of edge e from its chord: (a, Y) on a horizontal edge, (X, a) on a vertical one.
+__device__ __forceinline__ void d_edge_xy(const CellEdges& c, int e, int k, double base, double t, double& ix,
+                                          double& iy)
+{
+    const double a = k ? (base - t) : (base + t);
+    const bool h = e < 2;
+    ix = h ? a : ((e == 2) ? c.X1 : c.X0);
+    iy = h ? ((e == 0) ? c.Y1 : c.Y0) : a;
 }
 
 // GetNextCell (cell_grid.h:132-181) with UpdateFirstBoundaryIntersect
 // (:91-129) and Update_Best_Intersect (:80-89).  `phi0`: the start angle.
-// A first pass finds which of the 8 edge-line points lie in their segment and
-// differ from the entry point (DifferentPoint, :74-77: a point it rejects is
-// rejected whatever its angle); a second pass takes only those, in the
-// reference's order, for their angles (asin) and the smallest dphi — so a wave
-// runs the angle once or twice per cell step, not once per point any lane has.
+// A first pass over the four edges finds which of the 8 edge-line points lie
+// in their segment and differ from the entry point (DifferentPoint, :74-77: a
+// point it rejects is rejected whatever its angle); a second pass takes only
+// those, in the reference's order, for their angles (asin) and the smallest
+// dphi — so a wave runs the angle once or twice per cell step, not once per
+// point any lane has.
 template <bool CW>
 __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const DevGrid& g, const Isect& last, Isect& next,
                                             int& edge_hits)
@@ -197,32 +216,23 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     next = last;
     const double L = g.L;
     const double dp_eps = 1e-6 * L;
-    CellCorners c;
-    c.llx = g.start + (double)last.cx * L;
-    c.lly = g.start + (double)last.cy * L;
-    c.lrx = c.llx + L;
-    c.lry = c.lly + 0.0;
-    c.trx = c.llx + L;
-    c.try_ = c.lly + L;
-    c.tlx = c.llx + 0.0;
-    c.tly = c.lly + L;
-    const double llx = c.llx, lly = c.lly;
-    // The edge directions (p2 - p1) / L of cell_grid.h:98-99: the horizontal
-    // edges share trx - tlx = lrx - llx = (llx + L) - llx, the vertical ones
-    // try - lry = tly - lly = (lly + L) - lly, and the other components are
-    // exactly 0 (0 / L = +0); two quotients per cell.
-    c.qx = esm_div_rcp(c.trx - c.tlx, L, g.inv_L);
-    c.qy = esm_div_rcp(c.try_ - c.lry, L, g.inv_L);
+    CellEdges c;
+    d_cell_edges(o, g, last.cx, last.cy, c);
+    const double llx = c.X0, lly = c.Y0;
 
     unsigned int cand = 0;  // bit 2 e + k: a point in its segment, not the entry point
-#pragma unroll 1
+#pragma unroll
     for (int e = 0; e < 4; e++) {
+        double base, t;
+        if (!d_edge_chord(o, c, e, base, t)) continue;
+        const double l0 = (e < 2) ? c.X0 : c.Y0, l1 = (e < 2) ? c.X1 : c.Y1;
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            double ix, iy;
-            bool in_seg = false;
-            if (!d_edge_point(o, c, e, k, ix, iy, in_seg) || !in_seg) continue;
+            const double a = k ? (base - t) : (base + t);
+            if (!(a > l0 && a < l1)) continue;
             edge_hits += 1;
+            double ix, iy;
+            d_edge_xy(c, e, k, base, t, ix, iy);
             if (fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps) cand |= 1u << (2 * e + k);
         }
     }
@@ -230,9 +240,9 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
         const int bit = __builtin_ctz(cand);
         cand &= cand - 1u;
         const int e = bit >> 1, k = bit & 1;
-        double ix = 0.0, iy = 0.0;
-        bool in_seg = false;
-        (void)d_edge_point(o, c, e, k, ix, iy, in_seg);  // the same operations: the same point
+        double base = 0.0, t = 0.0, ix, iy;
+        (void)d_edge_chord(o, c, e, base, t);  // the same operations: the same point
+        d_edge_xy(c, e, k, base, t, ix, iy);
         const double ia = d_angle<CW>(ix, iy, o);
         const double dphi = d_cross_angle<CW>(last.ang, ia);
         if (dphi < best_dphi) {
