@@ -163,6 +163,8 @@ _SIGNATURES = {
     "es_grid_set_impurities": (C.c_int, [_P, _PD, _PD, C.c_int64, C.POINTER(GridDesc), C.POINTER(GridInfo)]),
     "es_grid_generate": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(GridDesc), C.POINTER(GridInfo)]),
     "es_grid_read": (C.c_int, [_P, _PD, C.POINTER(C.c_int32)]),
+    "es_grid_set_binned": (C.c_int, [_P, _PD, C.POINTER(C.c_int32), C.c_int32, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(GridInfo)]),
     "es_solver_init_sample": (C.c_int, [_P, C.POINTER(Settings), C.c_int32]),
     "es_solver_compute_sigmas": (C.c_int, [_P, C.c_double, _PD, C.c_int32, C.POINTER(Sigma), C.POINTER(Metrics), _PD]),
     "es_solver_compute_sigmas_sweep": (C.c_int, [_P, _PD, C.c_int32, _PD, C.c_int32, C.POINTER(Sigma),
@@ -335,6 +337,19 @@ class Simulation:
                      int(cells_per_row))
         info = GridInfo()
         self._check(self.lib.es_grid_generate(self.h, int(seed), int(count), C.byref(d), C.byref(info)))
+        return info
+
+    def set_binned(self, ordered_xy, imp_index, cells_per_row, spawn_region_start, spawn_region_size,
+                   impurity_radius):
+        """The reference Grid's own CSR as given: GetImpurities() [Ni, 2] (cell-ordered, duplicates),
+        GetIndex() [cpr^2] inclusive ends and GetSettings() (Grid.h:74-91); nothing is binned again."""
+        xy = np.ascontiguousarray(ordered_xy, dtype=np.float64).reshape(-1)
+        idx = np.ascontiguousarray(imp_index, dtype=np.int32)
+        info = GridInfo()
+        self._check(self.lib.es_grid_set_binned(self.h, _dptr(xy) if xy.size else None,
+                                                idx.ctypes.data_as(C.POINTER(C.c_int32)), int(cells_per_row),
+                                                float(spawn_region_start), float(spawn_region_size),
+                                                float(impurity_radius), C.byref(info)))
         return info
 
     def read_grid(self, info):

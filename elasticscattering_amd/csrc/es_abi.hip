@@ -481,6 +481,51 @@ int field_generate(Ctx& c, Field& f, uint64_t seed, int64_t count, const es_grid
 // Appends the start cell + inside test of a ppr x ppr position grid to q
 // unless the field's current (or pending) test is of the same positions;
 // field_sync collects the result after jobs_launch.
+// A field binned by the caller (Grid::GetImpurities / GetIndex / GetSettings,
+// Grid.h:74-91): the cell-ordered (x, y) pairs with duplicates and the
+// inclusive end offsets are uploaded as they are.
+int field_set_binned(Ctx& c, Field& f, const double* ordered_xy, const int32_t* imp_index, int cpr, double start,
+                     double size, double radius)
+{
+    if (!imp_index || cpr <= 0) return fail(c, ES_ERR_INVALID_ARG, "null index or cells_per_row <= 0");
+    if ((int64_t)cpr * cpr > (int64_t)1 << 28) return fail(c, ES_ERR_INVALID_ARG, "cell grid too large");
+    if (!(size > 0) || !(radius >= 0) || !std::isfinite(start))
+        return fail(c, ES_ERR_INVALID_ARG, "spawn region size must be > 0, radius >= 0, start finite");
+    const int64_t ncell = (int64_t)cpr * cpr;
+    int32_t prev = 0;
+    for (int64_t k = 0; k < ncell; k++) {  // inclusive ends (Grid.cpp:92-110): non-decreasing from 0
+        if (imp_index[k] < prev)
+            return fail(c, ES_ERR_INVALID_ARG, "imp_index must be non-decreasing inclusive end offsets");
+        prev = imp_index[k];
+    }
+    const int64_t total = prev;
+    if (total > 0 && !ordered_xy) return fail(c, ES_ERR_INVALID_ARG, "null ordered impurities");
+    int rc;
+    if ((rc = ensure(c, f.ixy, std::max<int64_t>(total, 1) * sizeof(double2), "impurities"))) return rc;
+    if ((rc = ensure(c, f.cell_end, ncell * sizeof(int32_t), "cell index"))) return rc;
+    if ((rc = field_small(c, f))) return rc;
+    if (total > 0)
+        ES_HIP(c, hipMemcpyAsync(f.ixy.p, ordered_xy, total * sizeof(double2), hipMemcpyHostToDevice, c.stream));
+    ES_HIP(c, hipMemcpyAsync(f.cell_end.p, imp_index, ncell * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    es::GridDev& g = f.g;
+    g.cpr = cpr;
+    g.lo = start;
+    g.size = size;
+    g.hi = start + size;
+    g.cell_size = size / (double)cpr;  // Grid::GetSettings (Grid.h:82-91)
+    g.radius = radius;
+    g.total = total;
+    g.ixy = P<double2>(f.ixy);
+    g.cell_end = P<int32_t>(f.cell_end);
+    f.unique = 0;  // not known from a binned field
+    f.has_grid = true;
+    f.total_pending = false;
+    f.pos_valid = false;
+    f.pos_pending = false;
+    f.copy_queued = false;
+    return ES_OK;
+}
+
 int field_positions_job(Ctx& c, Field& f, int ppr, double dist, double offx, double offy, Jobs& q)
 {
     const bool same = f.pos_ppr == ppr && f.pos_dist == dist && f.pos_offx == offx && f.pos_offy == offy;
@@ -1500,6 +1545,41 @@ int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_d
     const int rc = field_generate(s->c, s->f, seed, count, desc, q);
     if (rc) return rc;
     return solver_field_ready(s, desc, out_info, q);
+}
+
+int es_grid_set_binned(es_solver* s, const double* ordered_xy, const int32_t* imp_index, int32_t cells_per_row,
+                       double spawn_region_start, double spawn_region_size, double impurity_radius,
+                       es_grid_info* out_info)
+{
+    if (!s) return ES_ERR_INVALID_ARG;
+    Ctx& c = s->c;
+    if (!(s->desc.grid_info.region_size > 0) || !(s->desc.grid_info.cell_size > 0))
+        return fail(c, ES_ERR_INVALID_ARG,
+                    "es_grid_set_binned needs the solver created with its GridInformation (region_size, cell_size; "
+                    "Simulation.h:131-146): a binned field does not carry the start-position grid");
+    ES_HIP(c, hipSetDevice(c.device));
+    int rc = field_set_binned(c, s->f, ordered_xy, imp_index, cells_per_row, spawn_region_start, spawn_region_size,
+                              impurity_radius);
+    if (rc) return rc;
+    Jobs q;
+    const es_grid_info& gi = s->desc.grid_info;
+    const int ppr = s->desc.positions_per_row;
+    const double dist = gi.region_size / (double)(ppr - 1);  // Simulation.h:142-143
+    if ((rc = field_positions_job(c, s->f, ppr, dist, gi.cell_size * 0.01, gi.cell_size * 0.005, q))) return rc;
+    if ((rc = jobs_launch(c, q))) return rc;
+    if ((rc = field_sync(c, s->f, false))) return rc;
+    if (out_info) {
+        es_grid_info info{};
+        info.region_size = gi.region_size;
+        info.region_extends = gi.region_extends;
+        info.cell_size = s->f.g.cell_size;
+        info.indexed_impurity_count = s->f.g.total;
+        info.unique_impurity_count = 0;
+        info.index_size = (int32_t)((int64_t)cells_per_row * cells_per_row);
+        info.cells_per_row = cells_per_row;
+        *out_info = info;
+    }
+    return ES_OK;
 }
 
 int es_grid_read(es_solver* s, double* ordered_xy, int32_t* imp_index)

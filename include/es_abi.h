@@ -135,6 +135,24 @@ int es_grid_generate(es_solver* s, uint64_t seed, int64_t count, const es_grid_d
                      es_grid_info* out_info);
 
 /*
+ * The field exactly as the reference's Grid hands it to a Simulation backend
+ * (Simulation::ComputeSigmas(B, T, const Grid&, ...), Simulation.h:128):
+ * Grid::GetImpurities() — cell-ordered (x, y) pairs, a disk listed once per
+ * cell it overlaps — and Grid::GetIndex() — cells_per_row^2 inclusive end
+ * offsets (Grid.h:74-75, Grid.cpp:92-110); the indexed count is the last end.
+ * spawn_region_start, spawn_region_size, impurity_radius and cells_per_row
+ * are Grid::GetSettings() (Grid.h:82-91).  Uploaded as given (no binning), so
+ * the indexed count, impurities_tested and every metric are those of the
+ * reference's grid.  The solver must have been created with the sample-0
+ * GridInformation (es_solver_desc.grid_info: region_size, cell_size), which
+ * places the start positions (Simulation.h:131-146).  unique_impurity_count of
+ * out_info is 0 (a binned field does not carry it).
+ */
+int es_grid_set_binned(es_solver* s, const double* ordered_xy, const int32_t* imp_index, int32_t cells_per_row,
+                       double spawn_region_start, double spawn_region_size, double impurity_radius,
+                       es_grid_info* out_info);
+
+/*
  * Grid::GetImpurities / Grid::GetIndex (Grid.h:74-75) of the current grid:
  * cell-ordered AoS (x, y) pairs and inclusive end offsets per cell.
  * Either pointer may be NULL.
