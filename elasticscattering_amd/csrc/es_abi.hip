@@ -59,7 +59,9 @@ struct Ctx {
     int cus = 0;
     int trace_bpc[2][3][4] = {{{0}}};  // resident trace blocks per CU [clockwise][coop][waves_per_eu]
     int trace32_bpc[2][2] = {{0, 0}, {0, 0}};  // resident FP32 trace blocks per CU [clockwise][coop]
-    long long seg_items = 1ll << 29;  // work items per launch segment (bounds the per-item buffers)
+    // Work items per launch segment: bounds the per-item buffers (29 B per item, 31 GB at this size) and
+    // keeps the mixed path's int32 recompute list in range.
+    long long seg_items = 1ll << 30;
     std::vector<hipEvent_t> ev;       // 4 per segment: trace start/end, temperature start/end
     void* hbuf = nullptr;             // pinned host staging of the stage tables and results
     size_t hbuf_cap = 0;
@@ -765,6 +767,19 @@ void plan_offsets(Plan& pl, long long seg_items)
     pl.NT = maxT <= 3 ? std::max(maxT, 1) : maxT <= 4 ? 4 : 8;
 }
 
+// The mixed path lists the items it traces again as int32 item numbers of
+// their segment: a segment (one pair can exceed seg_items) must stay below 2^31.
+int plan_check(Ctx& c, const Plan& pl)
+{
+    if (pl.precision != ES_PRECISION_MIXED) return ES_OK;
+    for (const Segment& g : pl.segs)
+        if (g.items > (long long)INT32_MAX)
+            return fail(c, ES_ERR_INVALID_ARG,
+                        "mixed precision: a launch segment holds 2^31 or more work items (one field of this size); "
+                        "use ES_PRECISION_FP64");
+    return ES_OK;
+}
+
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
@@ -1204,6 +1219,7 @@ int solver_compute(es_solver* sv, const double* B, int nB, const double* T, int 
                 pl.factor[(size_t)b * es::kMaxT + t] =
                     integrand_factor(ppr, sv->ps.phi_integrand_factor, sv->st.particle_speed, w[b], taus[t0 + t]);
         plan_offsets(pl, c.seg_items);
+        if ((rc = plan_check(c, pl))) return rc;
         if ((rc = stages_prepare(c, sv->s, pl))) return rc;
         // Later temperature chunks reuse the item lifetimes of a one-segment
         // plan; with several segments only the last one's are still resident.
@@ -1731,6 +1747,8 @@ int batch_plan(es_batch* bt)
                 integrand_factor(en.ppr, en.ps.phi_integrand_factor, en.st.particle_speed, p.w, en.taus[t]);
     }
     plan_offsets(pl, c.seg_items);
+    const int rc = plan_check(c, pl);
+    if (rc) return rc;
     bt->plan_ready = true;
     return ES_OK;
 }
