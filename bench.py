@@ -182,8 +182,12 @@ def main():
             i = b.add(coherent=False, **kw)
             b.add(coherent=True, share_field_of=i, **kw)
 
+    COUNTERS = ("traced_particles", "cells_passed", "impurities_tested", "edge_hits", "cross_calls", "cross_exact",
+                "sigma_evals")
+
     def reduce_runs(runs):
-        st = {"launch_s": [], "flops": [], "ref_flops": [], "sigma_flops": [], "sigma_s": [], "sigma": []}
+        st = {"launch_s": [], "flops": [], "ref_flops": [], "sigma_flops": [], "sigma_s": [], "sigma": [],
+              "counters": []}
         tot = dict(traced=0, cross=0, exact=0, recomputed=0)
         for sig, mets, _ in runs:
             # Every (entry, field) pair of the one launch, with the share of its counters
@@ -197,6 +201,8 @@ def main():
             st["ref_flops"].append(sum(x * algorithmic_flops(m, REFERENCE_TRACE_WORK) for m, x in ms))
             st["sigma_flops"].append(sum(x * algorithmic_flops(m, SIGMA_WORK) for m, x in ms))
             st["sigma"].append(np.stack(sig[0::2]))  # incoherent passes [sample, field, T, 2]
+            # work the launch executed: the coherent window's particles counted once
+            st["counters"].append({k: int(round(sum(m[k] * x for m, x in ms))) for k in COUNTERS})
             for key, col in (("traced", "traced_particles"), ("cross", "cross_calls"), ("exact", "cross_exact"),
                              ("recomputed", "fp64_recomputed")):
                 tot[key] += sum(round(m[col] * x) for m, x in ms)
@@ -241,6 +247,7 @@ def main():
         iso = reduce_runs(alone)
         st["launch_s"], st["flops"], st["ref_flops"] = iso["launch_s"], iso["flops"], iso["ref_flops"]
         st["sigma_flops"], st["sigma_s"] = iso["sigma_flops"], iso["sigma_s"]
+        st["counters"] = iso["counters"]
         return st, dt_
 
     stats, dt = timed("fp64")
@@ -293,8 +300,9 @@ def main():
         "exact_cross_solves_fraction": stats["exact"] / max(stats["cross"], 1),
         "roofline": {
             "bound": "mfma",
-            "bound_detail": "FP64 vector ALU work, priced at MI355X's 78.6 TFLOP/s FP64 peak (vector and matrix FP64 "
-                            "peaks are equal); HBM traffic is ~2 % of peak (profiles/k_trace_production_fp64.md)",
+            "bound_detail": "FP64 vector ALU (VALU) work: the kernel issues no MFMA; 'mfma' is the contract's compute "
+                            "category and MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s.  HBM traffic is "
+                            "a few % of peak (profiles/k_trace_production_fp64.md)",
             "kernel": "k_trace (CreateParticle + TraceOrbit; one launch per step: 8 samples, both passes, 40 fields)",
             "achieved": achieved,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
@@ -306,6 +314,8 @@ def main():
             # the same launch priced at the reference's work (every crossing disk solved)
             "reference_work_tflops": sum(stats["ref_flops"]) / len(stats["ref_flops"]) / avg_launch / 1e12,
             "sigma_stage_tflops": sigma_tf,
+            "sigma_stage_ms": sum(stats["sigma_s"]) / len(stats["sigma_s"]) * 1e3,
+            "work_per_launch": stats["counters"][-1],
         },
     }
     if mixed is not None:
@@ -323,28 +333,49 @@ def main():
 
 
 def batched_uneven(es):
-    """The 64-sample uneven batch, one launch per stage, FP64 and mixed (one warm-up run each)."""
+    """The 64-sample uneven batch (config 3), FP64 and mixed.
+
+    The launch shape comes from the project's tuner (es_batch_tune) run before
+    the timed region on a stratified copy of the batch: all 64 samples at 4 of
+    the 40 fields (1, 3.4, 11.7, 40 T), a tenth of the work, so that trying
+    every shape takes seconds.  The full batch then runs with that shape and
+    hipGraph capture on: the first (untimed) run records every stage launch of
+    every segment in one graph, the timed run replays it."""
     import numpy as np
     from elasticscattering_amd.workloads import BATCHED_UNEVEN, batched_samples
 
     samples = batched_samples()
+    fields = BATCHED_UNEVEN["fields"]
+    tune_fields = [fields[0], fields[13], fields[26], fields[39]]
     out = {"workload": BATCHED_UNEVEN["name"]}
     sig = {}
     for prec in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        with es.Batch() as tb:
+            tb.set_precision(prec)
+            for smp in samples:
+                tb.add(**dict(smp, fields=tune_fields))
+            shape, tuned_s = tb.tune()
+        tune_wall = time.perf_counter() - t0
         with es.Batch() as b:
             b.set_precision(prec)
-            for s in samples:
-                b.add(**s)
+            b.set_shape(**shape)
+            b.set_capture(True)
+            for smp in samples:
+                b.add(**smp)
             c = b.counts()
-            b.run(metrics=False)
+            b.run(metrics=False)  # records the graph
             t0 = time.perf_counter()
             sig[prec], mets, secs = b.run()
             wall = time.perf_counter() - t0
+            launches = b.launches()
         traced = sum(m["traced_particles"] for per in mets for m in per)
         out[prec] = {
             "traced_lifetimes_per_s": traced / secs, "device_s": secs, "wall_s": wall,
-            "trace_s": sum(m["time_elapsed_lifetimes"] for per in mets for m in per),
             "fp64_recomputed": sum(m["fp64_recomputed"] for per in mets for m in per),
+            "tuned_shape": shape, "tune_trace_s_on_stratified_copy": tuned_s, "tune_wall_s": tune_wall,
+            "graph": True, "kernel_launches_per_run": launches["kernels"], "segments": launches["segments"],
+            "graph_launches_per_run": launches["graph_launches"],
         }
         out.update(samples=c["samples"], pairs=c["pairs"], traced_items=c["traced_items"])
     err = 0.0
@@ -407,6 +438,11 @@ def sweep_comparison(n_materials, train_runs):
                      "over_tolerance": [{"material": int(i), "error": float(cmax[i]), "solver_jitter": float(jitter[i])}
                                         for i in np.flatnonzero(cmax > 1e-3)],
                      "speedup_vs_full_fp64_wall": full_wall / wall}
+        if sg is not None:
+            # the network's cost: the solver runs that make its training data and its training
+            build = out["surrogate"]["wall_s"]
+            out[name]["wall_s_with_surrogate_build"] = wall + build
+            out[name]["speedup_vs_full_fp64_wall_with_surrogate_build"] = full_wall / (wall + build)
     sur.close()
     return out
 

@@ -188,6 +188,7 @@ _SIGNATURES = {
     "es_batch_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int64)]),
     "es_batch_tune": (C.c_int, [_P, C.POINTER(BatchShape), _PD]),
+    "es_batch_launches": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "es_version": (C.c_char_p, []),
     "es_math_eval": (C.c_int, [C.c_int32, C.c_int32, _PD, _PD, C.c_int64]),
     "es_grid_bin_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(GridDesc), C.c_void_p, C.c_void_p,
@@ -510,6 +511,12 @@ class Batch:
         t = C.c_double()
         self._check(self.lib.es_batch_tune(self.h, C.byref(sh), C.byref(t)))
         return sh.as_dict(), t.value
+
+    def launches(self):
+        """{"segments", "kernels" (kernel launches per run), "graph_launches" (1 when captured)}."""
+        sg, kn, gl = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.es_batch_launches(self.h, C.byref(sg), C.byref(kn), C.byref(gl)))
+        return {"segments": sg.value, "kernels": kn.value, "graph_launches": gl.value}
 
     def run(self, metrics=True):
         """([sigma [nB, nT, 2] per entry], [metrics record array [nB] per entry] | None, device seconds).

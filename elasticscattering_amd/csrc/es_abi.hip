@@ -2127,6 +2127,28 @@ int es_batch_counts(es_batch* bt, int32_t* n_samples, int32_t* n_pairs, int32_t*
     return ES_OK;
 }
 
+int es_batch_launches(es_batch* bt, int32_t* segments, int32_t* kernels_per_run, int32_t* graph_launches_per_run)
+{
+    if (!bt) return ES_ERR_INVALID_ARG;
+    if (batch_idle(bt)) return ES_ERR_INVALID_ARG;
+    if (!bt->plan_ready) {
+        batch_release_graph(bt);
+        const int rc = batch_plan(bt);
+        if (rc) return rc;
+        const int rc2 = stages_prepare(bt->c, bt->s, bt->plan);
+        if (rc2) return rc2;
+    }
+    // per segment: the trace (FP32 trace, queue reset and FP64 recompute when
+    // mixed), the temperature stage and the finalize stage (stages_enqueue)
+    const bool mixed = bt->plan.precision == ES_PRECISION_MIXED;
+    int32_t kernels = 0;
+    for (const Segment& g : bt->plan.segs) kernels += (g.items > 0 ? (mixed ? 3 : 1) : 0) + 2;
+    if (segments) *segments = (int32_t)bt->plan.segs.size();
+    if (kernels_per_run) *kernels_per_run = kernels;
+    if (graph_launches_per_run) *graph_launches_per_run = bt->capture ? 1 : 0;
+    return ES_OK;
+}
+
 int es_batch_tune(es_batch* bt, es_batch_shape* best, double* best_seconds)
 {
     if (!bt) return ES_ERR_INVALID_ARG;

@@ -287,6 +287,15 @@ int es_batch_collect(es_batch* b, es_sigma* out, es_metrics* metrics, double* se
 int es_batch_prepare(es_batch* b);
 /* Sizes of the batch: entries, (entry, field) pairs, sigma outputs, traced work items. */
 int es_batch_counts(es_batch* b, int32_t* n_samples, int32_t* n_pairs, int32_t* n_outputs, int64_t* traced_items);
+/*
+ * How a run of the batch is launched: launch segments (runs of whole (entry,
+ * field) pairs sharing the per-item buffers), kernel launches per run, and
+ * graph launches per run (1 with es_batch_set_capture, the kernels then
+ * replayed from one hipGraph; else 0).  No reference counterpart (the
+ * reference launches and waits per sample, pass and field,
+ * SimulationCL.cpp:113-177).
+ */
+int es_batch_launches(es_batch* b, int32_t* segments, int32_t* kernels_per_run, int32_t* graph_launches_per_run);
 /* Times every trace launch shape on this batch and keeps the fastest. */
 int es_batch_tune(es_batch* b, es_batch_shape* best, double* best_trace_seconds);
 
