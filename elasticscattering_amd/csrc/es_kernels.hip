@@ -548,9 +548,13 @@ __device__ __forceinline__ ItemPos item_pos(const TraceArgs& a, const PairDev& p
     ItemPos r;
     const int tpq = sd.tpq, tpp = 4 * sd.tpq;
     const long long k = item - pr.item_off;
-    r.slot = k / tpp;
+    if (pr.nitems <= 0x7fffffffll) {  // a 32-bit division (a 64-bit one is a long software sequence)
+        r.slot = (long long)((unsigned int)k / (unsigned int)tpp);
+    } else {
+        r.slot = k / tpp;
+    }
     const int v = (int)(k - r.slot * tpp);
-    r.q = v / tpq;
+    r.q = (int)((unsigned int)v / (unsigned int)tpq);
     r.p = v - r.q * tpq;
     r.pidx = sd.free_pos[r.slot];
     r.out_idx = a.lifetimes ? (long long)pr.out * a.P + (long long)r.pidx * (4 * sd.ppq) + r.q * sd.ppq + r.p : -1;
@@ -569,7 +573,7 @@ __device__ __forceinline__ void d_setup_item(const TraceArgs& a, const PairHint&
     const SampleDev& sd = a.samples[pr.sample];
     const ItemPos ip = item_pos(a, pr, sd, item);
     const int q = ip.q, p = ip.p, pidx = ip.pidx;
-    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    const int j = (int)((unsigned int)pidx / (unsigned int)sd.ppr), i = pidx - j * sd.ppr;
     out_idx = ip.out_idx;
     g.ixy = sd.ixy;
     g.cell_end = sd.cell_end;
@@ -729,11 +733,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 // solves which crossing disk, and which disks need solving at all.  Per cell
 // step of a wave:
 //   1. each lane runs CirclesCross (details.h:141-151) on its cell's disks,
-//      drops the crossing disks whose centre lies outside the window of the
+//      32 at a time into a bit mask, then takes its crossing disks one per
+//      pass: it drops those whose centre lies outside the window of the
 //      cell's angle range and gives the others a lower bound on the travel
 //      angle of their meeting points (d_range_window, d_disk_screen); it keeps
 //      the disk with the lowest bound and queues the others on the wave's
-//      stack in LDS;
+//      stack in LDS (the screen's cost follows the crossing disks, about half
+//      of those tested, and the wave's passes the lane with the most of them);
 //   2. each lane solves its lowest-bound disk (GetFirstCrossTime,
 //      details.h:153-206) and turns the time into a travel-angle threshold: a
 //      disk whose bound reaches it cannot meet the orbit before that time
@@ -752,7 +758,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 // is what shrinks.
 
 constexpr int kCoopStack = 320;  // FP32 trace: < 64 left over + up to 4 new disks per lane per group
-constexpr int kCoopCap = 576;    // FP64 trace: queued disks; screened and solved past kCoopCap - 64 G
+constexpr int kCoopCap = 576;    // FP64 trace: queued disks; solved past kCoopCap - 64 (screened) or - 256
 
 struct CoopWave {
     double orb[12][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry P0x P0y
@@ -793,8 +799,11 @@ __device__ __forceinline__ float pseudo_angle(float x, float y)
     const float ax = fabsf(x), ay = fabsf(y);
     const float sum = ax + ay;
     const float r = (sum > 0.0f) ? ay * __builtin_amdgcn_rcpf(sum) : 0.0f;
-    if (y >= 0.0f) return (x >= 0.0f) ? r : 2.0f - r;
-    return (x < 0.0f) ? 2.0f + r : 4.0f - r;
+    // y >= 0: x >= 0 ? r : 2 - r; y < 0: x < 0 ? 2 + r : 4 - r (selects, no branches)
+    const bool up = y >= 0.0f;
+    const bool plus = up ? (x >= 0.0f) : (x < 0.0f);
+    const float base = up ? (plus ? 0.0f : 2.0f) : (plus ? 2.0f : 4.0f);
+    return plus ? base + r : base - r;
 }
 
 // Screening of one lane's crossing disks in one cell.  Angles are travel
@@ -867,7 +876,7 @@ __device__ __forceinline__ bool d_disk_screen(const DiskScreen& ds, double qx, d
     const float pc = pseudo_angle(dot, CW ? -crs : crs);
     const float g2 = gx * gx + gy * gy;
     const bool near = g2 < ds.gmin2;
-    const float ld = (__builtin_sqrtf(g2) - ds.r) * ds.inv_R;
+    const float ld = (__builtin_amdgcn_sqrtf(g2) - ds.r) * ds.inv_R;  // v_sqrt_f32 (1 ulp; g2 is a normal number)
     lb = near ? ld : fmaxf(ld, pc - ds.dmax);
     return near || (pc > ds.lo_pa && pc < ds.hi_pa);
 }
@@ -903,6 +912,69 @@ __device__ __forceinline__ void d_coop_solve(CoopWave& L, int own, int gi)
     const double t = d_first_cross_time<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own]);
     if (t < ES_INF) atomicMin(&L.tmin[own], d_order_key(t));
     atomicAdd(&L.nexact[own], 1u);
+}
+
+// The cell's disks through global-memory loads (the binned field is global
+// memory; a generic pointer would make the compiler emit flat loads, which
+// wait on the LDS counter too).
+typedef double DiskXY __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const DiskXY GlobalDisk;
+
+__device__ __forceinline__ const GlobalDisk* global_disks(const double2* p)
+{
+    return (const GlobalDisk*)p;
+}
+
+__device__ __forceinline__ double2 load_disk(const GlobalDisk* p, int k)
+{
+    const DiskXY v = p[k];
+    return make_double2(v.x, v.y);
+}
+
+// Steps 2-4 over the `top` queued disks (see above); wave-uniform.
+template <bool CW, bool SCREEN>
+__device__ __forceinline__ void coop_flush(CoopWave& L, int lane, unsigned long long below, bool active, int first_i,
+                                           float first_lb, int top)
+{
+    // 2. Each lane solves its lowest-bound disk, unless the minimum so far
+    // already rules it out; the threshold that results.  (The orbit is read
+    // back from LDS, as for a shared solve.)
+    if (SCREEN && active) {
+        const double w = L.orb[5][lane];
+        float thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
+        if (first_i >= 0 && !(first_lb >= thr)) {
+            d_coop_solve<CW>(L, lane, first_i);
+            thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
+        }
+        L.thr[lane] = thr;
+    }
+    wave_lds_sync();
+
+    // 3. Drop the queued disks whose bound reaches their owner's threshold; compact.
+    int nsurv = SCREEN ? 0 : top;  // wave-uniform
+    for (int k0 = 0; SCREEN && k0 < top; k0 += 64) {
+        const int k = k0 + lane;
+        bool keep = false;
+        int own = 0, gi = 0;
+        if (k < top) {
+            own = L.qown[k];
+            gi = L.qidx[k];
+            keep = !(L.qlb[k] >= L.thr[own]);
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int pos = nsurv + __popcll(m & below);
+            L.qidx[pos] = gi;
+            L.qown[pos] = (uint8_t)own;
+        }
+        nsurv += __popcll(m);
+        wave_lds_sync();
+    }
+
+    // 4. The rest, 64 at a time.
+    for (int k0 = 0; k0 < nsurv; k0 += 64)
+        if (k0 + lane < nsurv) d_coop_solve<CW>(L, L.qown[k0 + lane], L.qidx[k0 + lane]);
+    wave_lds_sync();
 }
 
 template <bool CW, bool SCREEN>
@@ -1003,96 +1075,111 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         }
 
         int i = 0;
-        while (true) {
-            // 1. CirclesCross, G disks at a time (fewer registers when
-            // screening); with SCREEN, disks outside the range window are
-            // dropped, the one with the lowest bound stays with its lane and
-            // the others are queued with their bounds.
-            constexpr int G = SCREEN ? 2 : 4;
+        if constexpr (SCREEN) {
+            // 1a. CirclesCross of the next <= 32 disks of the lane's cell into
+            // a bit mask (bit k: disk s + base + k crosses the orbit); 1b. the
+            // crossing ones, one per lane per pass, through the screen: disks
+            // outside the range window are dropped, the one with the lowest
+            // bound stays with its lane and the others are queued with their
+            // bounds.  Only 1a runs over every disk; the screen runs over the
+            // crossing ones, packed (a lane's next crossing disk, not its next disk).
+            const GlobalDisk* gxy = global_disks(g.ixy);
+            unsigned int pend = 0u;
+            int base = 0;
+            bool more = true;
+            while (true) {
+                int top = 0;  // wave-uniform
+                float first_lb = 3.0e38f;
+                int first_i = -1;
+                while (top <= kCoopCap - 64) {
+                    if (__ballot(pend != 0u) == 0ull) {
+                        if (__ballot(i < n) == 0ull) {
+                            more = false;
+                            break;
+                        }
+                        base = i;
+                        constexpr int GA = 4;
+                        for (int k0 = 0; k0 < 32 && __ballot(base + k0 < n) != 0ull; k0 += GA) {
+                            double2 p[GA];
+#pragma unroll
+                            for (int u = 0; u < GA; u++)
+                                p[u] = (base + k0 + u < n) ? load_disk(gxy, s + base + k0 + u) : make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int u = 0; u < GA; u++) {
+                                const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
+                                const double d2 = qx * qx + qy * qy;
+                                const bool h = (base + k0 + u < n) && (d2 < o.radd2) && (d2 > o.rmin2);
+                                pend |= h ? (1u << (k0 + u)) : 0u;
+                            }
+                        }
+                        st.cross_calls += __popc(pend);
+                        i = base + 32;
+                        continue;
+                    }
+                    bool h = false;
+                    int qi = -1;
+                    float lb = 0.0f;
+                    if (pend) {
+                        const int k = __builtin_ctz(pend);
+                        pend &= pend - 1u;
+                        qi = s + base + k;
+                        const double2 p = load_disk(gxy, qi);
+                        h = d_disk_screen<CW>(ds, o.cx - p.x, o.cy - p.y, lb);
+                    }
+                    float ql = lb;
+                    if (h && lb < first_lb) {  // a new lowest bound: the previous one is queued
+                        const int t = qi;
+                        qi = first_i;
+                        ql = first_lb;
+                        h = first_i >= 0;
+                        first_lb = lb;
+                        first_i = t;
+                    }
+                    const unsigned long long m = __ballot(h);
+                    if (h) {
+                        const int pos = top + __popcll(m & below);
+                        L.qidx[pos] = qi;
+                        L.qlb[pos] = ql;
+                        L.qown[pos] = (uint8_t)lane;
+                    }
+                    top += __popcll(m);
+                }
+                coop_flush<CW, true>(L, lane, below, active, first_i, first_lb, top);
+                if (!more) break;
+            }
+        }
+        while (!SCREEN) {
+            // 1. CirclesCross, G disks at a time; every crossing disk is queued.
+            constexpr int G = 4;
             int top = 0;  // wave-uniform
-            float first_lb = 3.0e38f;
-            int first_i = -1;
             bool more;
             while ((more = (__ballot(i < n) != 0ull)) && top <= kCoopCap - 64 * G) {
                 double2 p[G];
 #pragma unroll
                 for (int u = 0; u < G; u++) p[u] = (i + u < n) ? g.ixy[s + i + u] : make_double2(0.0, 0.0);
-                int hits = 0, kept = 0;
-                float lbs[G];
+                int hits = 0;
 #pragma unroll
                 for (int u = 0; u < G; u++) {
                     const double qx = o.cx - p[u].x, qy = o.cy - p[u].y;
                     const double d2 = qx * qx + qy * qy;
                     const bool h = (i + u < n) && (d2 < o.radd2) && (d2 > o.rmin2);
                     hits |= h ? (1 << u) : 0;
-                    lbs[u] = 0.0f;
-                    const bool in_window = SCREEN ? d_disk_screen<CW>(ds, qx, qy, lbs[u]) : true;
-                    kept |= (h && in_window) ? (1 << u) : 0;
                 }
                 st.cross_calls += __popc(hits);
 #pragma unroll
                 for (int u = 0; u < G; u++) {
-                    bool h = (kept >> u) & 1;
-                    int qi = s + i + u;
-                    float ql = lbs[u];
-                    if (SCREEN && h && lbs[u] < first_lb) {  // a new lowest bound: the previous one is queued
-                        qi = first_i;
-                        ql = first_lb;
-                        h = first_i >= 0;
-                        first_lb = lbs[u];
-                        first_i = s + i + u;
-                    }
+                    const bool h = (hits >> u) & 1;
                     const unsigned long long m = __ballot(h);
                     if (h) {
                         const int pos = top + __popcll(m & below);
-                        L.qidx[pos] = qi;
-                        if (SCREEN) L.qlb[pos] = ql;
+                        L.qidx[pos] = s + i + u;
                         L.qown[pos] = (uint8_t)lane;
                     }
                     top += __popcll(m);
                 }
                 i += G;
             }
-
-            // 2. Each lane solves its lowest-bound disk, unless the minimum so
-            // far already rules it out; the threshold that results.
-            // (The orbit is read back from LDS, as for a shared solve.)
-            if (SCREEN && active) {
-                const double w = L.orb[5][lane];
-                float thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
-                if (first_i >= 0 && !(first_lb >= thr)) {
-                    d_coop_solve<CW>(L, lane, first_i);
-                    thr = d_time_threshold(w, d_from_order_key(L.tmin[lane]));
-                }
-                L.thr[lane] = thr;
-            }
-            wave_lds_sync();
-
-            // 3. Drop the queued disks whose bound reaches their owner's threshold; compact.
-            int nsurv = SCREEN ? 0 : top;  // wave-uniform
-            for (int k0 = 0; SCREEN && k0 < top; k0 += 64) {
-                const int k = k0 + lane;
-                bool keep = false;
-                int own = 0, gi = 0;
-                if (k < top) {
-                    own = L.qown[k];
-                    gi = L.qidx[k];
-                    keep = !(L.qlb[k] >= L.thr[own]);
-                }
-                const unsigned long long m = __ballot(keep);
-                if (keep) {
-                    const int pos = nsurv + __popcll(m & below);
-                    L.qidx[pos] = gi;
-                    L.qown[pos] = (uint8_t)own;
-                }
-                nsurv += __popcll(m);
-                wave_lds_sync();
-            }
-
-            // 4. The rest, 64 at a time.
-            for (int k0 = 0; k0 < nsurv; k0 += 64)
-                if (k0 + lane < nsurv) d_coop_solve<CW>(L, L.qown[k0 + lane], L.qidx[k0 + lane]);
-            wave_lds_sync();
+            coop_flush<CW, false>(L, lane, below, active, -1, 0.0f, top);
             if (!more) break;
         }
 
@@ -1131,7 +1218,7 @@ __device__ __forceinline__ void t32_setup_item(const TraceArgs& a, const PairHin
     const SampleDev& sd = a.samples[pr.sample];
     const ItemPos ip = item_pos(a, pr, sd, item);
     const int q = ip.q, p = ip.p, pidx = ip.pidx;
-    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    const int j = (int)((unsigned int)pidx / (unsigned int)sd.ppr), i = pidx - j * sd.ppr;
     out_idx = ip.out_idx;
     g.ixy = sd.ixy;
     g.cell_end = sd.cell_end;
@@ -1531,15 +1618,15 @@ __device__ __forceinline__ void sigma_unit(const SigmaArgs& a, const PairDev& pr
         q = (int)(u & 3);
         p0 = 0;
     } else {
-        slot = u / ppp;
+        slot = (pr.nitems <= 0x7fffffffll) ? (long long)((unsigned int)u / (unsigned int)ppp) : u / ppp;
         const int v = (int)(u - slot * ppp);
-        q = v / ppq;
+        q = (int)((unsigned int)v / (unsigned int)ppq);
         p0 = v - q * ppq;
     }
     const long long gi = pr.item_off + u;
     const double lt_raw = a.lt_item[gi];
     const int pidx = sd.free_pos[slot];
-    const int j = pidx / sd.ppr, i = pidx - (pidx / sd.ppr) * sd.ppr;
+    const int j = (int)((unsigned int)pidx / (unsigned int)sd.ppr), i = pidx - j * sd.ppr;
     if (a.with_metrics) {
         const int4 m = a.m_item[gi];
         const int f = a.f_item[gi];
