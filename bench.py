@@ -432,7 +432,9 @@ def sweep_comparison(n_materials, train_runs):
         out[name] = {"device_s": rep["solver_seconds"], "wall_s": wall, "solver_points": rep["solver_points"],
                      "points_per_curve": rep["points_per_curve"], "rounds": rep["rounds"],
                      "traced_items": rep["traced_items"], "work_vs_full": full_items / max(rep["traced_items"], 1),
-                     "rough_points": rep["rough_points"],
+                     "rough_points": rep["rough_points"], "gaps_closed_by_surrogate": rep["gaps_skipped"],
+                     "curves_surrogate_trusted": rep["surrogate_trusted"], "bisect_checks_passed": rep["bisect_passed"],
+                     "points_by_reason": rep["points_by_reason"], "work_share_by_reason": rep["work_share_by_reason"],
                      "accepted": rep["accepted"], "true_max_error": float(cmax.max()),
                      "curves_within_tolerance": int((cmax <= 1e-3).sum()),
                      "over_tolerance": [{"material": int(i), "error": float(cmax[i]), "solver_jitter": float(jitter[i])}
@@ -444,6 +446,11 @@ def sweep_comparison(n_materials, train_runs):
             out[name]["wall_s_with_surrogate_build"] = wall + build
             out[name]["speedup_vs_full_fp64_wall_with_surrogate_build"] = full_wall / (wall + build)
     sur.close()
+    g, p = out["adaptive_mixed_surrogate"], out["adaptive_mixed"]
+    # what the network buys: solver work and wall time against the same method without it,
+    # the second with the solver runs and training that build the network counted
+    out["surrogate_gain"] = {"solver_work": p["traced_items"] / g["traced_items"], "wall": p["wall_s"] / g["wall_s"],
+                             "wall_with_build": p["wall_s"] / g["wall_s_with_surrogate_build"]}
     return out
 
 

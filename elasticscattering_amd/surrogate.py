@@ -2,9 +2,9 @@
 
 * :class:`MLP` — ctypes handle of the ensemble MLP of include/es_surrogate.h
   (6 -> 128 -> 128 -> 2, tanh; BF16 MFMA forward/backward, Adam).
-* :class:`Surrogate` — the MLP over physical inputs: features (log B, T,
-  log density, radius, tau, alpha) scaled to [-1, 1] over fixed ranges,
-  standardised targets.  Targets are asinh(sigma / S0)
+* :class:`Surrogate` — the MLP over physical inputs: the dimensionless
+  groups of (B, T, density, radius, tau, alpha) (:func:`groups`) scaled to
+  [-1, 1] over fixed ranges, standardised targets.  Targets are asinh(sigma / S0)
   per component: log-like over the decades sigma spans, finite where a
   material conducts nothing (every start position inside an impurity gives
   sigma = 0 exactly, at every B).
@@ -131,20 +131,49 @@ class MLP:
 
 # ----------------------------------------------------------------- features
 
-def features(B, T, density, radius, tau, alpha):
-    """Inputs scaled to [-1, 1]: (log B, T, log density, radius, tau, alpha)."""
-    B, T, density, radius, tau, alpha = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64)
-                                                               for v in (B, T, density, radius, tau, alpha)))
+# The solver's constants (es_abi.hip): electron charge, effective mass 5 m0,
+# hbar / k_B, and the particle speed of solver_runs.
+E_CHARGE = 1.60217646e-19
+M_EFF = 5.0 * 9.10938188e-31
+HBAR_OVER_KB = 1.05457148e-34 / 1.380649e-23
+SPEED = 7e5
+T_FLOOR = 1.0  # temperatures below 1 K are taken as 1 K (draw_materials floors T there)
 
-    def lin(v, lo, hi):
-        return 2.0 * (v - lo) / (hi - lo) - 1.0
 
-    def log(v, lo, hi):
-        return lin(np.log(v), math.log(lo), math.log(hi))
+def groups(B, T, density, radius, tau, alpha, speed=SPEED):
+    """The six inputs as the dimensionless groups the physics runs on:
+    log(omega tau) and log(omega tau_T) (cyclotron frequency omega = e B / m
+    against the coherent lifetime tau and the thermal one tau_T = hbar / (k_B
+    T), Simulation.h:90), log(l / R_c) (mean free path between disks l = 1 /
+    (2 r n) against the orbit radius R_c = v / omega), log(n r^2) (the share of
+    the plane the disks cover, which sets the free start positions), log r and
+    alpha.  Invertible for B, T, n, r, tau > 0."""
+    B, T, n, r, tau, al = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64)
+                                                 for v in (B, T, density, radius, tau, alpha)))
+    w = E_CHARGE * B / M_EFF
+    tT = HBAR_OVER_KB / np.maximum(T, T_FLOOR)
+    ell_over_rc = w / (2.0 * r * n * speed)
+    return np.stack([np.log(w * tau), np.log(w * tT), np.log(ell_over_rc), np.log(n * r * r), np.log(r), al], axis=-1)
+
+
+def _group_ranges():
+    """Each group's range over the RANGES box (every group is monotone in each input)."""
+    import itertools
 
     R = RANGES
-    return np.stack([log(B, *R["B"]), lin(T, *R["T"]), log(density, *R["density"]), lin(radius, *R["radius"]),
-                     lin(tau, *R["tau"]), lin(alpha, *R["alpha"])], axis=-1)
+    corners = np.array(list(itertools.product(R["B"], (T_FLOOR, R["T"][1]), R["density"], R["radius"], R["tau"],
+                                              R["alpha"])))
+    g = groups(*corners.T)
+    return g.min(axis=0), g.max(axis=0)
+
+
+_GROUP_LO, _GROUP_HI = _group_ranges()
+
+
+def features(B, T, density, radius, tau, alpha):
+    """The network's inputs: groups() scaled to [-1, 1] over the RANGES box."""
+    g = groups(B, T, density, radius, tau, alpha)
+    return 2.0 * (g - _GROUP_LO) / (_GROUP_HI - _GROUP_LO) - 1.0
 
 
 S0 = 1e-7  # target scale: asinh is logarithmic well above it, linear below
@@ -375,8 +404,18 @@ class _Curve:
         self.checks = []        # (index, error) of the acceptance checks
         self.accepted = False
         self.skipped = 0        # gaps the surrogate resolved without a solver point
+        self.trust = 0          # gaps the surrogate called resolved that the solver then confirmed
+        self.vouched = set()    # gaps of this round the surrogate called resolved (while not yet trusted)
+        self.bisect_passed = 0  # midpoint checks that found the interpolant already within safety * tol
         self.rough = 0          # points found rough at grid scale
         self.escaped = {}       # index -> escaped fraction of the solver run
+        self.free = 0           # start positions outside every impurity (solver work per point)
+        self.few_free = False   # solved at every field (fewer than full_below free positions)
+        self.reasons = {}       # why points were solved: init, bisect, fill, dense, few_free, check
+
+    def count(self, reason, k):
+        if k:
+            self.reasons[reason] = self.reasons.get(reason, 0) + k
 
     def scale(self):
         """The interpolation scale of this curve (LINEAR_FRACTION)."""
@@ -398,8 +437,8 @@ class _Curve:
 
 
 def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.15, kappa=0.1, n_checks=3,
-                   dense=2, grow=8, rough=0.15, top_probe=True, full_below=2000, escape_max=0.01, max_rounds=60,
-                   seed=11, **solver_kw):
+                   dense=2, grow=8, rough=0.15, top_probe=True, full_below=2000, escape_max=0.01, trust=2,
+                   max_rounds=60, seed=11, **solver_kw):
     """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
 
     Each curve starts from n_initial solver points spread evenly over the
@@ -415,7 +454,12 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     uncertainty) keep the solver run.  It does so only where the solver runs
     at both ends of the gap had at most escape_max of their particles end
     without a hit: such orbits make sigma step between neighbouring fields,
-    which a smooth surrogate cannot foresee.
+    which a smooth surrogate cannot foresee.  And only on a curve where the
+    solver has already confirmed `trust` of its verdicts: until then the gaps
+    it calls resolved are bisected all the same, each midpoint that then lies
+    within safety * tol counts for it, and one that does not ends its use on
+    that curve (a network whose curves are smooth but of the wrong shape is
+    caught so; a feature narrower than the gaps of the first rounds is not).
 
     The solver's sigma is not smooth everywhere: with few free start
     positions, or where many orbits close a full revolution without a hit
@@ -458,7 +502,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     if top_probe and n >= 4:  # two more points beside the last: a second difference at the high-field end
         init = sorted(set(init) | {n - 3, n - 2})
     opts = dict(tol=tol, safety=safety, kappa=kappa, n_checks=n_checks, dense=dense, grow=grow, rough=rough,
-                full_below=full_below, escape_max=escape_max)
+                full_below=full_below, escape_max=escape_max, trust=trust)
     try:
         return _sweep_rounds(curves, B, u, n, init, rng, max_rounds, opts, dict(solver_kw, reuse=reuse))
     finally:
@@ -540,21 +584,30 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
                 continue
             if c.phase == "init":
                 c.pending = [(i, None) for i in init]
+                c.count("init", len(init))
             elif c.phase == "refine":
                 gaps = sorted(c.active)
                 # the surrogate may close a gap only where few orbits end without a hit
                 calm = [g for g in gaps if max(c.escaped.get(g[0], 1.0), c.escaped.get(g[1], 1.0)) <= o["escape_max"]]
                 if c.prior is not None and calm:
                     idx = np.array(sorted(c.solved))
-                    keep = _prior_errors(u, idx, c.prior, calm, c.scale()) > kappa * tol
-                    c.skipped += int(np.count_nonzero(~keep))
-                    closed = {g for g, k in zip(calm, keep) if not k}
-                    gaps = [g for g in gaps if g not in closed]
+                    fine = _prior_errors(u, idx, c.prior, calm, c.scale()) <= kappa * tol
+                    if c.trust >= o["trust"]:
+                        c.skipped += int(np.count_nonzero(fine))
+                        closed = {g for g, f in zip(calm, fine) if f}
+                        gaps = [g for g in gaps if g not in closed]
+                    else:  # not yet trusted on this curve: its verdicts are checked by the solver first
+                        c.vouched = {g for g, f in zip(calm, fine) if f}
                 want = {i: None for i in c.dense if i not in c.solved}
+                c.count("few_free" if c.few_free else "dense", len(want))
                 for a, b in gaps:
                     if b - a <= 2 * o["dense"]:
-                        want.update({k: None for k in range(a + 1, b)})
+                        fill = [k for k in range(a + 1, b) if k not in want]
+                        want.update({k: None for k in fill})
+                        c.count("fill", len(fill))
                     else:
+                        if (a + b) // 2 not in want:
+                            c.count("bisect", 1)
                         want[(a + b) // 2] = (a, b)
                 c.pending = sorted(want.items())
                 c.active, c.dense = set(), set()
@@ -567,6 +620,7 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
                     continue
                 pts = rng.choice(free, size=min(o["n_checks"], free.size), replace=False)
                 c.pending = [(int(i), None) for i in sorted(pts)]
+                c.count("check", len(pts))
             if c.pending:
                 ask.append(ci)
         if not ask:
@@ -587,9 +641,11 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
                 if nf == 0:
                     c.phase, c.accepted = "done", True
                     continue
+                c.free = int(nf)
                 if nf < o["full_below"]:
                     # few free start positions: large steps, cheap runs; every field is solved
                     c.dense = {k for k in range(n) if k not in c.solved}
+                    c.few_free = True
                     c.prior = None
                 s_idx = sorted(c.solved)
                 c.active = {(a, b) for a, b in zip(s_idx[:-1], s_idx[1:]) if b - a > 1}
@@ -612,9 +668,18 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
                 c.prior = None  # the surrogate's curves are smooth: it cannot vouch for a rough one
             if c.phase == "refine":
                 for (i, gap), e in zip(c.pending, errs):
+                    if gap is not None and e <= tol * safety:
+                        c.bisect_passed += 1
+                    if gap in c.vouched and c.prior is not None:
+                        # the surrogate said this gap needed no solver point: right or wrong?
+                        if e <= tol * safety:
+                            c.trust += 1
+                        else:
+                            c.prior = None
                     if gap is not None and e > tol * safety:
                         a, b = gap
                         c.active |= {g for g in ((a, i), (i, b)) if g[1] - g[0] > 1}
+                c.vouched = set()
                 if not c.active and not c.dense:
                     c.phase = "check"
             else:  # check
@@ -638,8 +703,18 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
         solver_points=int(sum(len(c.solved) for c in curves)),
         points_per_curve=float(np.mean([len(c.solved) for c in curves])),
         gaps_skipped=int(sum(c.skipped for c in curves)),
+        bisect_passed=int(sum(c.bisect_passed for c in curves)),
+        surrogate_trusted=int(sum(c.trust >= o["trust"] and c.prior is not None for c in curves)),
         rough_points=int(sum(c.rough for c in curves)),
         max_check_error=float(max((e for c in curves for _, e in c.checks), default=0.0)),
         solver_seconds=solver_s, traced_items=int(items), wall_seconds=time.time() - t0,
     )
+    # Solver points and work by why they were asked for (work per point is
+    # proportional to the material's free start positions).
+    names = ("init", "bisect", "fill", "dense", "few_free", "check")
+    pts = {r: int(sum(c.reasons.get(r, 0) for c in curves)) for r in names}
+    wk = {r: float(sum(c.reasons.get(r, 0) * c.free for c in curves)) for r in names}
+    tot = sum(wk.values()) or 1.0
+    report["points_by_reason"] = pts
+    report["work_share_by_reason"] = {r: wk[r] / tot for r in names}
     return est, report

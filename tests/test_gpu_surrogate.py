@@ -1,16 +1,22 @@
-"""The learned part end to end on the GPU, at 128 x 128 positions (needs an MI355X).
+"""The learned part end to end on the GPU, at the sweep's full size (needs an MI355X).
 
-Training data come from the batched FP64 solver; the surrogate's held-out
-error is judged against those solver results; the adaptive sweep is run with
-and without the surrogate and every curve is compared, at all 400 fields,
-with the full FP64 solver sweep.  Bars: every curve accepted; every curve
-within 2e-3 of the full sweep at every field (1e-3 is the acceptance
-tolerance of the sampled checks); at least 1.5x less solver work (traced
-work items) than the full sweep on these 24 materials (rough and few-free
-curves are solved densely; over 200 materials the ratio is 5-6x, see the
-sweep figures of bench.py); and the surrogate-guided sweep doing at most 10 %
-more solver work than the plain one (the measured gain is printed; over 200
-materials it is a gain, see the sweep figures of bench.py).
+The surrogate is built as the bench builds it: 20,000 FP64 solver runs (500
+materials x 20 fields x 2 temperatures, 128 x 128 positions, both passes),
+4 ensemble members trained on 18,000 of them; the other 2,000 (50 materials
+it never saw) give its held-out error.
+
+The adaptive sweep then runs on 200 materials drawn with seed 79, a set none
+of the sweep's parameters was chosen on, over the 400 log-spaced fields of
+0.1-60 T, with and without the network, and every curve is compared at all
+400 fields with the full FP64 solver sweep.  Bars: every curve accepted;
+every curve within 1e-3 (max(|d xx|, |d xy|) / |sigma|) of the full sweep at
+every field, with the network and without it; at least 4x less solver work
+(traced work items) than the full sweep; and no more solver work with the
+network than without it.  The network's gain is printed, not asserted: on
+this held-out set its verdicts earn trust on a few curves and close no gap
+(1.000x), against 1.002x-1.004x on the sets the rule was chosen on; with the
+cost of building it the guided sweep is slower than the method without it
+(bench.py's sweep figures).
 """
 import math
 
@@ -21,21 +27,20 @@ from conftest import announce
 
 pytestmark = pytest.mark.gpu
 
-PPR = 128
+N_TRAIN = 18000
 
 
 @pytest.fixture(scope="module")
 def trained():
     from elasticscattering_amd import surrogate as S
 
-    announce("surrogate: 6000 solver runs at 128x128 positions")
-    X, Y, secs, items = S.training_set(300, 10, 2, seed=4, positions_per_row=PPR)
-    assert X.shape == (6000, 6) and np.isfinite(Y).all()
-    ntr = 5000
-    announce("surrogate: training 4 members")
-    sur = S.Surrogate(members=4, seed=2)
-    hist = sur.fit(X[:ntr], Y[:ntr], epochs=400, batch=256)
-    yield S, sur, X[ntr:], Y[ntr:], hist
+    announce("surrogate: 20000 solver runs at 128x128 positions")
+    X, Y, secs, items = S.training_set(500, 20, 2, seed=1)
+    assert X.shape == (20000, 6) and np.isfinite(Y).all()
+    announce("surrogate: training 4 members on 18000 runs")
+    sur = S.Surrogate(members=4, seed=3)
+    hist = sur.fit(X[:N_TRAIN], Y[:N_TRAIN], epochs=600, batch=256, lr=2e-3, lr_final=2e-5)
+    yield S, sur, X[N_TRAIN:], Y[N_TRAIN:], hist
     sur.close()
 
 
@@ -45,31 +50,33 @@ def test_surrogate_trains_and_generalises(trained):
     mean, spread = sur.predict(Xh)
     ok = Yh[:, 0] > 0
     rel = np.abs(mean[ok, 0] / Yh[ok, 0] - 1)
+    te = S.tensor_error(mean[ok], Yh[ok])
     print(f"held-out sigma_xx relative error vs FP64 solver: median {np.median(rel):.3g}, p90 "
-          f"{np.percentile(rel, 90):.3g}")
-    assert np.median(rel) < 0.2
+          f"{np.percentile(rel, 90):.3g}; tensor error median {np.median(te):.3g}, p90 {np.percentile(te, 90):.3g}")
+    assert np.median(rel) < 0.1
     assert spread.shape == mean.shape and np.all(spread >= 0)
 
 
 def test_adaptive_sweep_against_full_solver(trained):
     S, sur, _, _, _ = trained
     B = np.logspace(-1, math.log10(60), 400)
-    mats = S.draw_materials(24, seed=77)
-    announce("full FP64 sweep of 24 materials x 400 fields (truth)")
-    truth, _, full_items = S.solver_runs(mats, [B] * len(mats), positions_per_row=PPR)
+    mats = S.draw_materials(200, seed=79)
+    announce("full FP64 sweep of 200 materials x 400 fields (truth)")
+    truth, _, full_items = S.solver_runs(mats, [B] * len(mats))
     truth = np.stack([t[:, 0, :] for t in truth])
     reports = {}
     for name, sg in (("plain", None), ("surrogate", sur)):
         announce(f"adaptive sweep ({name})")
-        est, rep = S.adaptive_sweep(mats, B, surrogate=sg, positions_per_row=PPR)
-        err = S.tensor_error(est, truth)
+        est, rep = S.adaptive_sweep(mats, B, surrogate=sg)
+        err = S.tensor_error(est, truth).max(axis=1)
         rep["true_max_error"] = float(err.max())
         reports[name] = rep
-        print(name, rep)
+        print(name, {k: rep[k] for k in ("rounds", "solver_points", "gaps_skipped", "surrogate_trusted",
+                                          "bisect_passed", "traced_items", "true_max_error", "points_by_reason")})
         assert rep["accepted"] == len(mats)
-        assert err.max() <= 2e-3
+        assert err.max() <= 1e-3, [(int(i), float(err[i])) for i in np.flatnonzero(err > 1e-3)]
         print(f"{name}: solver work {full_items / rep['traced_items']:.2f}x less than the full sweep")
-        assert 1.5 * rep["traced_items"] <= full_items
+        assert 4 * rep["traced_items"] <= full_items
     gain = reports["plain"]["traced_items"] / reports["surrogate"]["traced_items"]
-    print(f"solver work saved by the surrogate: {gain:.3f}x")
-    assert reports["surrogate"]["traced_items"] <= 1.1 * reports["plain"]["traced_items"]
+    print(f"solver work saved by the surrogate: {gain:.4f}x ({reports['surrogate']['gaps_skipped']} gaps closed)")
+    assert reports["surrogate"]["traced_items"] <= reports["plain"]["traced_items"]

@@ -139,3 +139,75 @@ def test_pchip_matches_scipy(kind):
         got = S.pchip(x, y, xe)
         assert np.max(np.abs(got - ref) / (1 + np.abs(ref))) <= 1e-12
         assert np.allclose(S.pchip(x, y, x), y, rtol=0, atol=1e-14)
+
+
+def test_surrogate_earns_trust_before_closing_gaps(fake_solver, monkeypatch):
+    """The network may close a gap only on a curve where the solver has already
+    confirmed `trust` of its verdicts.  An exact network earns it and saves
+    points.  A smooth network that misses a 2 % bump of the solver's curve
+    (Gaussian in log B, width 0.4) closes the bump's gap when trusted at once
+    (trust=0: error above tol); with trust=2 (the default) its first verdicts
+    are checked, the bump shows at a midpoint and the curve is finished
+    without it."""
+    mats = [dict(m, tau=1e-13 * (i + 1)) for i, m in enumerate(_mats())]
+    _, plain = S.adaptive_sweep(mats, B)
+    for trust in (1, 2):
+        _, exact = S.adaptive_sweep(mats, B, surrogate=_ExactSurrogate(mats), trust=trust)
+        assert exact["surrogate_trusted"] > 0 and exact["gaps_skipped"] > 0
+        assert exact["solver_points"] < plain["solver_points"]
+
+    rng = np.random.default_rng(5)
+    mats = [dict(m, tau=1e-13 * (i + 1), c=rng.uniform(-1.5, 3.5)) for i, m in enumerate(_mats(12, seed=2))]
+    base = S.solver_runs
+
+    def bump(m, f):
+        return 1 + 0.02 * np.exp(-((np.log(np.asarray(f)) - m["c"]) / 0.4) ** 2)
+
+    def bumpy(materials, fields, temperatures=None, with_free=False, **kw):
+        r = base(materials, fields, temperatures, with_free, **kw)
+        for m, f, s in zip(materials, fields, r[0]):
+            s[:, 0, 0] *= bump(m, f)
+        return r
+
+    monkeypatch.setattr(S, "solver_runs", bumpy)
+
+    def worst(est):
+        out = 0.0
+        for m, e in zip(mats, est):
+            t = np.zeros((len(B), 2)) if m["zero"] else _drude(m, B) * np.stack([bump(m, B), np.ones(len(B))], -1)
+            out = max(out, S.tensor_error(e, t).max())
+        return out
+
+    est, eager = S.adaptive_sweep(mats, B, surrogate=_ExactSurrogate(mats), trust=0)
+    assert worst(est) > 1e-3
+    est, guarded = S.adaptive_sweep(mats, B, surrogate=_ExactSurrogate(mats))
+    assert worst(est) <= 1e-3 and guarded["accepted"] == len(mats)
+    assert guarded["surrogate_trusted"] < eager["surrogate_trusted"]
+
+
+def test_sweep_reports_points_by_reason(fake_solver):
+    mats = _mats()
+    _, rep = S.adaptive_sweep(mats, B)
+    pts = rep["points_by_reason"]
+    assert sum(pts.values()) == rep["solver_points"]
+    assert pts["init"] == 15 * len(mats) and pts["few_free"] == 0
+    assert abs(sum(rep["work_share_by_reason"].values()) - 1.0) < 1e-12
+    assert 0 <= rep["bisect_passed"] <= pts["bisect"]
+
+
+def test_groups_are_the_dimensionless_inputs():
+    B, T, n, r, tau, al = 2.0, 10.0, 1e15, 1e-8, 5e-13, 0.4
+    g = S.groups(B, T, n, r, tau, al)
+    w = S.E_CHARGE * B / S.M_EFF
+    assert g[0] == pytest.approx(math.log(w * tau))
+    assert g[1] == pytest.approx(math.log(w * S.HBAR_OVER_KB / T))
+    assert g[2] == pytest.approx(math.log((1 / (2 * r * n)) / (S.SPEED / w)))
+    assert g[3] == pytest.approx(math.log(n * r * r)) and g[4] == pytest.approx(math.log(r)) and g[5] == al
+    # below the floor temperature the thermal group saturates
+    assert S.groups(B, 0.1, n, r, tau, al)[1] == pytest.approx(S.groups(B, S.T_FLOOR, n, r, tau, al)[1])
+    # the box maps into [-1, 1], its corners onto the ends
+    rng = np.random.default_rng(3)
+    R = S.RANGES
+    X = np.stack([rng.uniform(*R[k], 1000) for k in ("B", "T", "density", "radius", "tau", "alpha")], axis=-1)
+    f = S.features(*X.T)
+    assert f.min() >= -1.0 - 1e-12 and f.max() <= 1.0 + 1e-12
