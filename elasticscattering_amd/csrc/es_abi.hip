@@ -642,6 +642,7 @@ struct Shape {
 
 // Screened solves pay where many disks cross the orbit in a cell: dense fields.
 constexpr double kScreenPopulation = 10.0;
+constexpr int kMaxBatchTemperatures = 4096;  // per batch sample; run in chunks of kMaxT
 
 
 // Device buffers and the optional captured graph of the stage kernels.
@@ -1682,6 +1683,41 @@ bool same_grid_desc(const es_grid_desc& a, const es_grid_desc& b)
            a.target_cell_population == b.target_cell_population && a.cells_per_row == b.cells_per_row;
 }
 
+// Temperatures run in chunks of kMaxT: chunk ch of every sample (its
+// temperatures ch*kMaxT ... ch*kMaxT + kMaxT - 1, none past its last) goes
+// into the plan's sample table and integrand factors.  The number of chunks
+// is that of the sample with the most temperatures.
+int batch_chunks(const es_batch* bt)
+{
+    size_t nT = 0;
+    for (const es_batch::Entry& en : bt->entries) nT = std::max(nT, en.T.size());
+    return nT == 0 ? 1 : (int)((nT + es::kMaxT - 1) / es::kMaxT);
+}
+
+void batch_set_chunk(es_batch* bt, int ch)
+{
+    Plan& pl = bt->plan;
+    const size_t t0 = (size_t)ch * es::kMaxT;
+    for (size_t e = 0; e < bt->entries.size() && e < pl.samples.size(); e++) {
+        const es_batch::Entry& en = bt->entries[e];
+        es::SampleDev& sd = pl.samples[e];
+        sd.nT = en.T.size() > t0 ? (int)std::min<size_t>(es::kMaxT, en.T.size() - t0) : 0;
+        for (int t = 0; t < es::kMaxT; t++) {
+            sd.tau[t] = t < sd.nT ? en.taus[t0 + t] : 1.0;
+            sd.max_lt[t] = 15.0 * sd.tau[t];
+            sd.inv_tau[t] = 1.0 / sd.tau[t];
+        }
+    }
+    pl.factor.assign((size_t)pl.nout * es::kMaxT, 0.0);
+    for (const es::PairDev& p : pl.pairs) {
+        const es_batch::Entry& en = bt->entries[p.sample];
+        const int nt = pl.samples[p.sample].nT;
+        for (int t = 0; t < nt; t++)
+            pl.factor[(size_t)p.out * es::kMaxT + t] =
+                integrand_factor(en.ppr, en.ps.phi_integrand_factor, en.st.particle_speed, p.w, en.taus[t0 + t]);
+    }
+}
+
 int batch_plan(es_batch* bt)
 {
     Ctx& c = bt->c;
@@ -1722,12 +1758,6 @@ int batch_plan(es_batch* bt)
                         "a start position lies outside the impurity cell grid (region_extends smaller than the "
                         "position offset); the reference would read out of bounds (lifetime.h:88-91)");
         es::SampleDev sd = sample_dev(f, en.ppr, en.ppq, en.dist, en.offx, en.offy, en.ps, en.st, en.coherent);
-        sd.nT = (int)en.T.size();
-        for (int t = 0; t < es::kMaxT; t++) {
-            sd.tau[t] = t < sd.nT ? en.taus[t] : 1.0;
-            sd.max_lt[t] = 15.0 * sd.tau[t];
-            sd.inv_tau[t] = 1.0 / sd.tau[t];
-        }
         pl.samples.push_back(sd);
         for (size_t b = 0; b < en.B.size(); b++) {
             es::PairDev p{};
@@ -1739,13 +1769,7 @@ int batch_plan(es_batch* bt)
         }
     }
     pl.nout = out;
-    pl.factor.assign((size_t)out * es::kMaxT, 0.0);
-    for (const es::PairDev& p : pl.pairs) {
-        const es_batch::Entry& en = bt->entries[p.sample];
-        for (size_t t = 0; t < en.T.size(); t++)
-            pl.factor[(size_t)p.out * es::kMaxT + t] =
-                integrand_factor(en.ppr, en.ps.phi_integrand_factor, en.st.particle_speed, p.w, en.taus[t]);
-    }
+    batch_set_chunk(bt, 0);
     plan_offsets(pl, c.seg_items);
     const int rc = plan_check(c, pl);
     if (rc) return rc;
@@ -1843,7 +1867,7 @@ void batch_outputs(es_batch* bt, const StageOut& so, es_sigma* out, es_metrics* 
     for (const es::PairDev& p : pl.pairs) {
         const es_batch::Entry& en = bt->entries[p.sample];
         const int nT = (int)en.T.size();
-        for (int t = 0; t < nT && out; t++) {
+        for (int t = 0; t < std::min(nT, es::kMaxT) && out; t++) {  // the first chunk; es_batch_run adds the others
             out[o + t].xx = so.sig[((size_t)p.out * es::kMaxT + t) * 2];
             out[o + t].xy = so.sig[((size_t)p.out * es::kMaxT + t) * 2 + 1];
         }
@@ -1915,8 +1939,9 @@ int es_batch_add_sample(es_batch* bt, const es_batch_sample* smp, int32_t* index
     if (smp->positions_per_row < 2 || smp->particles_per_quadrant < 2)
         return fail(c, ES_ERR_INVALID_ARG, "positions_per_row and particles_per_quadrant must be >= 2");
     if (smp->n_fields <= 0 || !smp->fields) return fail(c, ES_ERR_INVALID_ARG, "need at least one field");
-    if (smp->n_temperatures < 0 || smp->n_temperatures > es::kMaxT || (smp->n_temperatures > 0 && !smp->temperatures))
-        return fail(c, ES_ERR_INVALID_ARG, "a batch sample takes 0 to 8 temperatures");
+    if (smp->n_temperatures < 0 || smp->n_temperatures > kMaxBatchTemperatures ||
+        (smp->n_temperatures > 0 && !smp->temperatures))
+        return fail(c, ES_ERR_INVALID_ARG, "a batch sample takes 0 to 4096 temperatures");
     for (int b = 0; b < smp->n_fields; b++)
         if (!(smp->fields[b] > 0) || !std::isfinite(smp->fields[b]))
             return fail(c, ES_ERR_INVALID_ARG, "magnetic field must be > 0");
@@ -2072,8 +2097,42 @@ int es_batch_run(es_batch* bt, es_sigma* out, es_metrics* metrics, double* secon
     ES_HIP(c, hipSetDevice(c.device));
     StageOut so;
     double wall = 0;
-    const int rc = batch_run_once(bt, bt->shape, so, &wall);
+    int rc = batch_run_once(bt, bt->shape, so, &wall);
     if (rc) return rc;
+    const int nch = batch_chunks(bt);
+    if (nch > 1) {
+        // Later temperature chunks: the temperature stage again on the item
+        // lifetimes of the run (the trace again only where the plan has several
+        // segments, whose item buffers are reused), then the plan's first chunk back.
+        Plan& pl = bt->plan;
+        const bool trace = pl.segs.size() > 1;
+        std::vector<int> nblocks;
+        int nblocks64 = 0;
+        if (trace && (rc = trace_blocks(c, pl.clockwise, bt->shape, pl, nblocks, &nblocks64))) return rc;
+        for (int ch = 1; ch < nch && !rc; ch++) {
+            batch_set_chunk(bt, ch);
+            StageOut sc;
+            if (!(rc = stages_prepare(c, bt->s, pl)) &&
+                !(rc = stages_enqueue(c, bt->s, pl, bt->shape, trace, 0, nullptr, nblocks, true, nblocks64)) &&
+                !(rc = stages_collect(c, bt->s, pl, trace, false, sc))) {
+                so.t_temp += sc.t_temp;
+                wall += sc.t_trace + sc.t_temp;
+                size_t o = 0;
+                for (const es::PairDev& p : pl.pairs) {
+                    const int nT = (int)bt->entries[p.sample].T.size(), nt = pl.samples[p.sample].nT;
+                    for (int t = 0; t < nt && out; t++) {
+                        out[o + (size_t)ch * es::kMaxT + t].xx = sc.sig[((size_t)p.out * es::kMaxT + t) * 2];
+                        out[o + (size_t)ch * es::kMaxT + t].xy = sc.sig[((size_t)p.out * es::kMaxT + t) * 2 + 1];
+                    }
+                    o += nT;
+                }
+            }
+        }
+        batch_set_chunk(bt, 0);
+        const int rc0 = stages_prepare(c, bt->s, pl);
+        if (rc) return rc;
+        if (rc0) return rc0;
+    }
     if (seconds) *seconds = wall;
     batch_outputs(bt, so, out, metrics);
     return ES_OK;
@@ -2091,6 +2150,10 @@ int es_batch_launch(es_batch* bt)
 {
     if (!bt) return ES_ERR_INVALID_ARG;
     ES_HIP(bt->c, hipSetDevice(bt->c.device));
+    if (batch_chunks(bt) > 1)
+        return fail(bt->c, ES_ERR_INVALID_ARG,
+                    "es_batch_launch / es_batch_collect take up to 8 temperatures per sample; es_batch_run takes any "
+                    "number");
     return batch_launch(bt, bt->shape);
 }
 

@@ -213,7 +213,7 @@ typedef struct es_batch_sample {
     int32_t share_field_of;           /* >= 0: use the field of that earlier entry instead of drawing one */
     const double* fields;             /* n_fields values of B (copied) */
     int32_t n_fields;
-    const double* temperatures;       /* 0..8 temperatures (copied) */
+    const double* temperatures;       /* 0..4096 temperatures (copied; run 8 at a time) */
     int32_t n_temperatures;
 } es_batch_sample;
 
@@ -261,7 +261,10 @@ int es_batch_set_precision(es_batch* b, int32_t precision);
 /*
  * Runs every entry: out holds, entry by entry, n_fields * n_temperatures
  * sigmas (field-major); metrics (or NULL) one es_metrics per (entry, field);
- * seconds (or NULL) the device time of the whole stage sequence.
+ * seconds (or NULL) the device time of the whole stage sequence.  Entries
+ * with more than 8 temperatures take the temperature stage once per 8 of
+ * them, on the same traced lifetimes (the reference takes any number of
+ * temperatures, SimulationConfiguration.h:186-196).
  */
 int es_batch_run(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
 /*
@@ -273,7 +276,7 @@ int es_batch_run(es_batch* b, es_sigma* out, es_metrics* metrics, double* second
  * (No counterpart in the reference, whose ComputeSigmas blocks,
  * SimulationCL.cpp:310-341; for callers that pipeline samples.)
  */
-int es_batch_launch(es_batch* b);
+int es_batch_launch(es_batch* b);  /* up to 8 temperatures per entry (else ES_ERR_INVALID_ARG) */
 int es_batch_collect(es_batch* b, es_sigma* out, es_metrics* metrics, double* seconds);
 /*
  * Enqueues the drawing, binning and start-cell tests of the fields added since

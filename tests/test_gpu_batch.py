@@ -342,7 +342,12 @@ def test_batch_errors(es):
         assert "rotation" in str(ei.value)
     with es.Batch() as b:
         with pytest.raises(es.EsError):
-            _add(b, dict(_entries()[1], temperatures=list(range(1, 10))))  # more than 8 temperatures
+            _add(b, dict(_entries()[1], temperatures=list(range(1, 4098))))  # more than 4096 temperatures
+    with es.Batch() as b:
+        _add(b, dict(_entries()[1], temperatures=list(range(1, 10))))
+        with pytest.raises(es.EsError) as ei:
+            b.launch()  # launch / collect take up to 8 temperatures per entry
+        assert "es_batch_run" in str(ei.value)
         with pytest.raises(es.EsError):
             _add(b, dict(_entries()[1], share_field_of=3))  # no such entry
         with pytest.raises(es.EsError):
@@ -367,3 +372,55 @@ def test_batched_uneven_workload_subset(es):
             sim.init_sample(s["tau"], s["alpha"], s["is_clockwise"], s["particle_speed"], s["coherent"])
             ref, _ = sim.compute_sweep(s["fields"], s["temperatures"])
         assert _same_bits(sig[i], ref), i
+
+
+MANY_T = [0.5 + 7.3 * k for k in range(19)]  # 19 temperatures: chunks of 8, 8 and 3
+
+
+@pytest.mark.parametrize("segment_items", [None, "20000"])
+def test_more_than_eight_temperatures(es, segment_items):
+    """Entries with 19, 3 and 0 temperatures in one batch (the reference takes
+    any number, SimulationConfiguration.h:186-196): every sigma bit-equal to the
+    solver's (which chunks its temperatures the same way) and to the batch's
+    own 8-temperature runs; metrics unchanged; with one launch segment (the
+    later chunks reuse the traced lifetimes) and with several (traced again);
+    with graph capture on."""
+    announce(f"batch with 19 temperatures, segments {segment_items or 'one'}")
+    base = _entries()
+    entries = [dict(base[1], temperatures=MANY_T), dict(base[2], temperatures=MANY_T[:3], share_field_of=0),
+               dict(base[3], temperatures=[])]
+    old = os.environ.get("ES_SEGMENT_ITEMS")
+    if segment_items:
+        os.environ["ES_SEGMENT_ITEMS"] = segment_items
+    try:
+        b = es.Batch()
+    finally:
+        if old is None:
+            os.environ.pop("ES_SEGMENT_ITEMS", None)
+        else:
+            os.environ["ES_SEGMENT_ITEMS"] = old
+    with b:
+        for e in entries:
+            _add(b, e)
+        if segment_items:
+            assert b.launches()["segments"] > 1
+        sig, mets, secs = b.run()
+        assert secs > 0
+        assert sig[0].shape == (len(base[1]["fields"]), 19, 2) and sig[2].size == 0
+        sig_s, ms_s = _solo(es, entries[0], entries)
+        assert _same_bits(sig[0], sig_s)
+        sig_s, _ = _solo(es, entries[1], entries)
+        assert _same_bits(sig[1], sig_s)
+        for bi, (mb, msolo) in enumerate(zip(mets[0], ms_s)):
+            for k in METRIC_KEYS:
+                assert mb[k] == msolo[k], (bi, k)
+        b.set_capture(True)
+        for _ in range(2):
+            sig2, mets2, _ = b.run()
+            for x, y in zip(sig, sig2):
+                assert _same_bits(x, y)
+    with es.Batch() as b8:  # the first 8 and the next 8 as batches of their own
+        for t0 in (0, 8, 16):
+            _add(b8, dict(entries[0], temperatures=MANY_T[t0:t0 + 8]))
+        parts, _, _ = b8.run()
+    assert _same_bits(sig[0], np.concatenate(parts, axis=1))
