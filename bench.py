@@ -60,6 +60,11 @@ def _json(obj):
     return json.dumps(obj, default=lambda o: o.item() if hasattr(o, "item") else str(o))
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_flops(m, model):
     """Work of one metrics record, or of a record array (summed)."""
     return float(sum(model[k] * m[k].sum() if hasattr(m[k], "sum") else model[k] * m[k] for k in model))
@@ -250,9 +255,11 @@ def main():
         st["counters"] = iso["counters"]
         return st, dt_
 
+    progress("production FP64")
     stats, dt = timed("fp64")
     mixed = None
     if not args.no_extras:
+        progress("production mixed")
         mstats, mdt = timed("mixed")
         err = 0.0
         for a, b in zip(mstats["sigma"], stats["sigma"]):
@@ -320,9 +327,12 @@ def main():
     }
     if mixed is not None:
         line["mixed"] = mixed
+        progress("batched uneven (config 3)")
         line["batched_uneven"] = batched_uneven(es)
+        progress("sweep (config 5)")
         line["sweep"] = sweep_comparison(args.sweep_materials, args.train_runs)
     if not args.no_cpu_baseline:
+        progress("cpu baseline")
         cb = cpu_baseline(w)
         if cb is not None:
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
@@ -355,6 +365,7 @@ def batched_uneven(es):
             tb.set_precision(prec)
             for smp in samples:
                 tb.add(**dict(smp, fields=tune_fields))
+            progress(f"tuning {prec}")
             shape, tuned_s = tb.tune()
         tune_wall = time.perf_counter() - t0
         with es.Batch() as b:
@@ -400,6 +411,7 @@ def sweep_comparison(n_materials, train_runs):
            "tolerance": 1e-3, "error": "max(|d xx|, |d xy|) / |sigma| against the full FP64 sweep"}
     t0 = time.perf_counter()
     n_mat = max(1, train_runs // 40)
+    progress("surrogate training data")
     X, Y, gen_s, _ = S.training_set(n_mat, 20, 2, seed=1)
     sur = S.Surrogate(members=4, seed=3)
     sur.fit(X, Y, epochs=600, batch=256, lr=2e-3, lr_final=2e-5)
@@ -409,6 +421,7 @@ def sweep_comparison(n_materials, train_runs):
     B = np.logspace(-1, math.log10(60), 400)
     mats = S.draw_materials(n_materials, seed=77)
     t0 = time.perf_counter()
+    progress("full FP64 sweep")
     full, full_s, full_items = S.solver_runs(mats, [B] * n_materials)
     full_wall = time.perf_counter() - t0
     full = np.stack([f[:, 0, :] for f in full])
@@ -423,6 +436,7 @@ def sweep_comparison(n_materials, train_runs):
                            ("adaptive_fp64", None, "fp64")):
         # one untimed run first: the first use of a precision loads its kernels
         # and sizes the batch buffers, which would land on whichever variant ran first
+        progress(f"adaptive sweep {name}")
         S.adaptive_sweep(mats, B, surrogate=sg, precision=prec)
         t0 = time.perf_counter()
         est, rep = S.adaptive_sweep(mats, B, surrogate=sg, precision=prec)

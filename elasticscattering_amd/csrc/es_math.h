@@ -344,8 +344,14 @@ ESM_FN esm_dd esm_pio2_minus_2(esm_dd a)
  * step on sin, A + (u - sin A) / cos A, with sin A from the accurate sine
  * (2^-102); the step's second-order error is below 2^-136.
  */
-ESM_COLD double esm_asin_acc(int big, double uh, double ul, double Ah, double Al)
+ESM_COLD double esm_asin_acc(int big, double ax, double Ah, double Al)
 {
+    double uh = ax, ul = 0.0;
+    if (big) {  /* sqrt(z) as RN(sqrt z) + the exact residual over 2 RN(sqrt z) */
+        const double z = (1.0 - ax) * 0.5;
+        uh = sqrt(z);
+        ul = fma(-uh, uh, z) / (uh + uh);
+    }
     const esm_sc v = esm_sincos_acc(Ah, Al);
     const double d = ((uh - v.s.h) + (ul - v.s.l)) / v.c.h;  /* uh - sin A is exact (Sterbenz) */
     esm_dd A = esm_fast_two_sum(Ah, Al + d);
@@ -368,14 +374,27 @@ ESM_FN double esm_asin(double x)
     const int big = ax > 0.5;
     double uh = ax, ul = 0.0;
     if (big) {
-        const double z = (1.0 - ax) * 0.5;
+        const double z = (1.0 - ax) * 0.5;  /* exact, in [2^-54, 1/4] */
+#ifdef __HIP_DEVICE_COMPILE__
+        /* uh within an ulp of sqrt(z) by Newton steps from v_rsq_f64 (z is normal, no scaling),
+           h within 2^-50 of 1/(2 uh); ul = (z - uh^2) h is the residual to 2^-50 of itself */
+        const double y = __builtin_amdgcn_rsq(z);
+        double g = z * y, h = 0.5 * y;
+        const double e = fma(-h, g, 0.5);
+        g = fma(g, e, g);
+        h = fma(h, e, h);
+        g = fma(fma(-g, g, z), h, g);
+        uh = fma(fma(-g, g, z), h, g);
+        ul = fma(-uh, uh, z) * h;
+#else
         uh = sqrt(z);
         ul = fma(-uh, uh, z) / (uh + uh);
+#endif
     }
     const esm_dd A = esm_asin_core(uh, ul);
     const esm_dd R = big ? esm_pio2_minus_2(A) : A;
     double r = R.h;
-    if (!esm_rn_ok(R.h, R.l, ESM_EPS_FAST)) r = esm_asin_acc(big, uh, ul, A.h, A.l);
+    if (!esm_rn_ok(R.h, R.l, ESM_EPS_FAST)) r = esm_asin_acc(big, ax, A.h, A.l);
     return x < 0.0 ? -r : r;
 }
 
