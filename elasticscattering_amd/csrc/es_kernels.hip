@@ -1658,8 +1658,14 @@ __device__ __forceinline__ void sigma_unit(const SigmaArgs& a, const PairDev& pr
     const double wji = simpson_weight(j, sd.ppr) * simpson_weight(i, sd.ppr);
     // sin, cos of the end angle for the untruncated lifetime, shared by every
     // temperature whose cap does not cut it (the same argument, so the same bits).
+    // The sigma terms take the device library's sin, cos and exp (errors of about
+    // one ulp), not the correctly rounded ones the trace needs: no branch depends
+    // on them, each term only feeds a compensated sum, and the correctly rounded
+    // functions make this stage half again as slow for a change far below the
+    // sums' 1e-12 agreement with the compensated oracle.  (The per-position
+    // images keep the correctly rounded functions: they are compared bit for bit.)
     double sa0, ca0;
-    esm_sincos(phi + ws * lt_raw, &sa0, &ca0);
+    sincos(phi + ws * lt_raw, &sa0, &ca0);
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         if (t < nT) {
@@ -1667,9 +1673,9 @@ __device__ __forceinline__ void sigma_unit(const SigmaArgs& a, const PairDev& pr
             const double lt = (lt_raw < ml) ? lt_raw : ml;  // Simulation.h:78-86
             if (lt != 0.0) {
                 const double tau = sd.tau[t];
-                const double z = esm_exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
+                const double z = exp(esm_div_rcp(-lt, tau, sd.inv_tau[t]));
                 double sa = sa0, ca = ca0;
-                if (lt != lt_raw) esm_sincos(phi + ws * lt, &sa, &ca);
+                if (lt != lt_raw) sincos(phi + ws * lt, &sa, &ca);
                 double r = cphi - ca * z;
                 r += ws * tau * sa * z;
                 r -= ws * tau * sphi;
