@@ -4,10 +4,12 @@
   (6 -> 128 -> 128 -> 2, tanh; BF16 MFMA forward/backward, Adam).
 * :class:`Surrogate` — the MLP over physical inputs: the dimensionless
   groups of (B, T, density, radius, tau, alpha) (:func:`groups`) scaled to
-  [-1, 1] over fixed ranges, standardised targets.  Targets are asinh(sigma / S0)
-  per component: log-like over the decades sigma spans, finite where a
-  material conducts nothing (every start position inside an impurity gives
-  sigma = 0 exactly, at every B).
+  [-1, 1] over fixed ranges, standardised targets (:func:`net_targets`):
+  asinh(sigma_xx / S0), log-like over the decades sigma_xx spans and finite
+  where a material conducts nothing (every start position inside an impurity
+  gives sigma = 0 exactly, at every B), and the Hall ratio sigma_xy / sigma_xx,
+  which the error measure below sees directly and which crosses zero smoothly
+  where sigma_xy changes sign.
 * :func:`solver_runs` — sigma = coherent + incoherent pass
   (SimulationRunner.cpp:161-173) from the batched GPU solver, for material
   draws; the surrogate's training data and the sweep's solver calls.
@@ -188,6 +190,23 @@ def from_targets(y, scale=S0):
     return scale * np.sinh(np.asarray(y, dtype=np.float64))
 
 
+def net_targets(sig):
+    """The network's targets of sigma [..., 2]: (asinh(sigma_xx / S0), sigma_xy / sigma_xx),
+    the ratio 0 where sigma_xx = 0.  The tensor error of a prediction is
+    about max(|d sigma_xx| / sigma_xx, |d ratio|) where sigma_xy << sigma_xx."""
+    sig = np.asarray(sig, dtype=np.float64)
+    xx, xy = sig[..., 0], sig[..., 1]
+    ok = xx != 0.0
+    return np.stack([np.arcsinh(xx / S0), np.where(ok, xy / np.where(ok, xx, 1.0), 0.0)], axis=-1)
+
+
+def net_sigma(t):
+    """sigma [..., 2] of network targets (the inverse of net_targets where sigma_xx != 0)."""
+    t = np.asarray(t, dtype=np.float64)
+    xx = S0 * np.sinh(t[..., 0])
+    return np.stack([xx, xx * t[..., 1]], axis=-1)
+
+
 # A curve is interpolated in asinh(sigma / s) with s this fraction of the
 # smallest |sigma| solved on it (at least S0): a component smaller than that
 # (sigma_xy crossing zero) is interpolated linearly, as the error measure
@@ -294,21 +313,25 @@ class Surrogate:
 
     def fit(self, X, sig, epochs=300, batch=256, lr=2e-3, lr_final=1e-5, shuffle_seed=7):
         f = features(*X.T)
-        y = targets(sig)
+        y = net_targets(sig)
         self.mean = y.mean(axis=0)
         self.std = y.std(axis=0) + 1e-12
         self.mlp.set_data(f, (y - self.mean) / self.std)
         return self.mlp.train(epochs, batch, lr, lr_final, shuffle_seed=shuffle_seed)
 
     def predict_targets(self, X):
-        """Per-member targets [members][n][2] (asinh(sigma / S0))."""
+        """Per-member network targets [members][n][2] (net_targets)."""
         y = self.mlp.predict(features(*np.asarray(X).T)).astype(np.float64)
         return y * self.std + self.mean
+
+    def predict_members(self, X):
+        """Per-member sigma [members][n][2]."""
+        return net_sigma(self.predict_targets(X))
 
     def predict(self, X):
         """(mean sigma [n][2], spread of the targets [n][2]) over the ensemble."""
         y = self.predict_targets(X)
-        return from_targets(y.mean(axis=0)), y.std(axis=0)
+        return net_sigma(y.mean(axis=0)), y.std(axis=0)
 
     def close(self):
         self.mlp.close()
@@ -438,7 +461,7 @@ class _Curve:
 
 def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=0.15, kappa=0.1, n_checks=3,
                    dense=2, grow=8, rough=0.15, top_probe=True, full_below=2000, escape_max=0.01, trust=2,
-                   max_rounds=60, seed=11, **solver_kw):
+                   max_rounds=60, seed=11, vouch_width=20, **solver_kw):
     """Builds every material's sigma curve on the grid B; returns (curves [n_mat][n_B][2], report).
 
     Each curve starts from n_initial solver points spread evenly over the
@@ -454,7 +477,11 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     uncertainty) keep the solver run.  It does so only where the solver runs
     at both ends of the gap had at most escape_max of their particles end
     without a hit: such orbits make sigma step between neighbouring fields,
-    which a smooth surrogate cannot foresee.  And only on a curve where the
+    which a smooth surrogate cannot foresee.  And only on gaps of at most
+    `vouch_width` grid steps: the first gaps (about 30 steps) are always
+    bisected, since the solver's sigma can step by about tol between two
+    neighbouring fields even where no orbit escapes, and only a solver point
+    inside a gap sees such a step.  And only on a curve where the
     solver has already confirmed `trust` of its verdicts: until then the gaps
     it calls resolved are bisected all the same, each midpoint that then lies
     within safety * tol counts for it, and one that does not ends its use on
@@ -492,7 +519,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
         # every material's curve from every member, in one inference launch
         X = np.concatenate([np.stack([B, np.full(n, m["T"]), np.full(n, m["density"]), np.full(n, m["radius"]),
                                       np.full(n, m["tau"]), np.full(n, m["alpha"])], axis=-1) for m in materials])
-        P = from_targets(surrogate.predict_targets(X))  # [members][n_mat * n][2]
+        P = surrogate.predict_members(X)  # [members][n_mat * n][2]
         priors = [P[:, k * n:(k + 1) * n] for k in range(len(materials))]
     curves = [_Curve(m, u, pr) for m, pr in zip(materials, priors)]
     init = [int(i) for i in np.unique(np.round(np.linspace(0, n - 1, max(2, n_initial))).astype(int))]
@@ -502,7 +529,7 @@ def adaptive_sweep(materials, B, surrogate=None, tol=1e-3, n_initial=13, safety=
     if top_probe and n >= 4:  # two more points beside the last: a second difference at the high-field end
         init = sorted(set(init) | {n - 3, n - 2})
     opts = dict(tol=tol, safety=safety, kappa=kappa, n_checks=n_checks, dense=dense, grow=grow, rough=rough,
-                full_below=full_below, escape_max=escape_max, trust=trust)
+                full_below=full_below, escape_max=escape_max, trust=trust, vouch_width=vouch_width)
     try:
         return _sweep_rounds(curves, B, u, n, init, rng, max_rounds, opts, dict(solver_kw, reuse=reuse))
     finally:
@@ -588,7 +615,8 @@ def _sweep_rounds(curves, B, u, n, init, rng, max_rounds, o, solver_kw):
             elif c.phase == "refine":
                 gaps = sorted(c.active)
                 # the surrogate may close a gap only where few orbits end without a hit
-                calm = [g for g in gaps if max(c.escaped.get(g[0], 1.0), c.escaped.get(g[1], 1.0)) <= o["escape_max"]]
+                calm = [g for g in gaps if max(c.escaped.get(g[0], 1.0), c.escaped.get(g[1], 1.0)) <= o["escape_max"]
+                        and g[1] - g[0] <= o["vouch_width"]]
                 if c.prior is not None and calm:
                     idx = np.array(sorted(c.solved))
                     fine = _prior_errors(u, idx, c.prior, calm, c.scale()) <= kappa * tol

@@ -11,12 +11,10 @@ of the sweep's parameters was chosen on, over the 400 log-spaced fields of
 400 fields with the full FP64 solver sweep.  Bars: every curve accepted;
 every curve within 1e-3 (max(|d xx|, |d xy|) / |sigma|) of the full sweep at
 every field, with the network and without it; at least 4x less solver work
-(traced work items) than the full sweep; and no more solver work with the
-network than without it.  The network's gain is printed, not asserted: on
-this held-out set its verdicts earn trust on a few curves and close no gap
-(1.000x), against 1.002x-1.004x on the sets the rule was chosen on; with the
-cost of building it the guided sweep is slower than the method without it
-(bench.py's sweep figures).
+(traced work items) than the full sweep; and strictly less solver work with
+the network than without it.  The gain is small (about 1 %, printed): the
+network may close only gaps of at most 20 grid steps whose ends the solver
+has already resolved, on curves where its earlier verdicts were confirmed.
 """
 import math
 
@@ -53,7 +51,7 @@ def test_surrogate_trains_and_generalises(trained):
     te = S.tensor_error(mean[ok], Yh[ok])
     print(f"held-out sigma_xx relative error vs FP64 solver: median {np.median(rel):.3g}, p90 "
           f"{np.percentile(rel, 90):.3g}; tensor error median {np.median(te):.3g}, p90 {np.percentile(te, 90):.3g}")
-    assert np.median(rel) < 0.1
+    assert np.median(rel) < 0.06
     assert spread.shape == mean.shape and np.all(spread >= 0)
 
 
@@ -79,4 +77,5 @@ def test_adaptive_sweep_against_full_solver(trained):
         assert 4 * rep["traced_items"] <= full_items
     gain = reports["plain"]["traced_items"] / reports["surrogate"]["traced_items"]
     print(f"solver work saved by the surrogate: {gain:.4f}x ({reports['surrogate']['gaps_skipped']} gaps closed)")
-    assert reports["surrogate"]["traced_items"] <= reports["plain"]["traced_items"]
+    assert reports["surrogate"]["gaps_skipped"] > 0
+    assert reports["surrogate"]["traced_items"] < reports["plain"]["traced_items"]

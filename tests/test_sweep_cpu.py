@@ -61,17 +61,17 @@ def test_zero_material_stops_after_first_round(fake_solver):
 
 
 class _ExactSurrogate:
-    """A surrogate that knows the curves exactly (asinh targets per member)."""
+    """A surrogate that knows the curves exactly (one member)."""
 
     def __init__(self, mats):
         self.mats = mats
 
-    def predict_targets(self, X):
+    def predict_members(self, X):
         """Rows of any materials (matched by tau): [1 member][rows][2]."""
         out = np.empty((len(X), 2))
         for m in self.mats:
             rows = X[:, 4] == m["tau"]
-            out[rows] = S.targets(_drude(m, X[rows, 0]))
+            out[rows] = _drude(m, X[rows, 0])
         return out[None]
 
 
@@ -113,6 +113,15 @@ def test_tensor_error_conventions():
 def test_targets_round_trip():
     sig = np.array([[0.5, 1e-3], [0.0, 0.0], [1e-9, -1e-12]])
     np.testing.assert_allclose(S.from_targets(S.targets(sig)), sig, rtol=1e-12, atol=1e-30)
+
+
+def test_network_targets_round_trip():
+    """The network's targets: asinh(sigma_xx / S0) and the Hall ratio, 0 where sigma_xx = 0."""
+    sig = np.array([[0.5, 1e-3], [0.0, 0.0], [1e-9, -1e-12], [0.2, -0.05]])
+    t = S.net_targets(sig)
+    assert t[1, 1] == 0.0 and t[3, 1] == pytest.approx(-0.25)
+    assert t[0, 0] == pytest.approx(math.asinh(0.5 / S.S0))
+    np.testing.assert_allclose(S.net_sigma(t), sig, rtol=1e-12, atol=1e-30)
 
 
 @pytest.mark.parametrize("kind", ["random", "monotone", "flat_steps", "smooth"])
