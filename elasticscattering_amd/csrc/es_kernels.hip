@@ -119,13 +119,14 @@ __device__ __forceinline__ void dd_add_exact_product(DD& a, double x, double y)
 
 struct BinGeom {
     double lo, size, cell_size, r;
+    double inv_size;  // RN(1 / size): esm_div_rcp gives the IEEE quotient's bits
     int cpr;
 };
 
 __device__ __forceinline__ void grid_cell(const BinGeom& g, double x, double y, int& cx, int& cy)
 {
-    cx = (int)((x - g.lo) / g.size * (double)(g.cpr));  // Grid.cpp:182-188
-    cy = (int)((y - g.lo) / g.size * (double)(g.cpr));
+    cx = (int)(esm_div_rcp(x - g.lo, g.size, g.inv_size) * (double)(g.cpr));  // Grid.cpp:182-188
+    cy = (int)(esm_div_rcp(y - g.lo, g.size, g.inv_size) * (double)(g.cpr));
 }
 
 __device__ __forceinline__ bool in_grid(int cx, int cy, int cpr)
@@ -133,8 +134,15 @@ __device__ __forceinline__ bool in_grid(int cx, int cy, int cpr)
     return (cx >= 0 && cx < cpr) && (cy >= 0 && cy < cpr);
 }
 
-// Grid.cpp:112-180 for impurity (x, y): up to 9 distinct target cells.
-__device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
+// Grid.cpp:112-180 for impurity (x, y): the up to 9 distinct target cells,
+// kept in registers as 9 fixed candidates with a bit each in the returned
+// mask (bit k: cell[k] is a target).  Candidates 0-4 are the probe cells
+// {c, c+(r,0), c-(r,0), c+(0,r), c-(0,r)}: one is a target when it lies in
+// the grid and differs from every earlier probe (the reference's std::set;
+// an earlier equal probe outside the grid makes this one outside too).
+// Candidates 5-8 are the diagonal cells tl, tr, ll, lr of the centre cell,
+// taken when that corner lies within r (strictly) and the cell is in the grid.
+__device__ __forceinline__ unsigned int bin_candidates(const BinGeom& g, double x, double y, int32_t (&cell)[9])
 {
     int px[5], py[5];
     grid_cell(g, x, y, px[0], py[0]);
@@ -142,17 +150,14 @@ __device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
     grid_cell(g, x - g.r, y, px[2], py[2]);
     grid_cell(g, x, y + g.r, px[3], py[3]);
     grid_cell(g, x, y - g.r, px[4], py[4]);
-    int n = 0;
-    int ax[5], ay[5], na = 0;
+    unsigned int mask = 0u;
+#pragma unroll
     for (int k = 0; k < 5; k++) {
         bool seen = false;
-        for (int m = 0; m < na; m++) seen = seen || (ax[m] == px[k] && ay[m] == py[k]);
-        if (!seen && in_grid(px[k], py[k], g.cpr)) {
-            ax[na] = px[k];
-            ay[na] = py[k];
-            na++;
-            out[n++] = py[k] * g.cpr + px[k];
-        }
+#pragma unroll
+        for (int m = 0; m < k; m++) seen = seen || (px[m] == px[k] && py[m] == py[k]);
+        mask |= (!seen && in_grid(px[k], py[k], g.cpr)) ? (1u << k) : 0u;
+        cell[k] = py[k] * g.cpr + px[k];
     }
     const double llx = g.lo + px[0] * g.cell_size;  // Grid.h:64-67
     const double lly = g.lo + py[0] * g.cell_size;
@@ -161,16 +166,16 @@ __device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
     const double trx = llx + g.cell_size, try_ = lly + g.cell_size;
     const double ir2 = g.r * g.r;
     const int cx = px[0], cy = py[0];
-    double dx, dy;
-    dx = tlx - x; dy = tly - y;
-    if (dx * dx + dy * dy < ir2 && in_grid(cx - 1, cy + 1, g.cpr)) out[n++] = (cy + 1) * g.cpr + (cx - 1);
-    dx = trx - x; dy = try_ - y;
-    if (dx * dx + dy * dy < ir2 && in_grid(cx + 1, cy + 1, g.cpr)) out[n++] = (cy + 1) * g.cpr + (cx + 1);
-    dx = llx - x; dy = lly - y;
-    if (dx * dx + dy * dy < ir2 && in_grid(cx - 1, cy - 1, g.cpr)) out[n++] = (cy - 1) * g.cpr + (cx - 1);
-    dx = lrx - x; dy = lry - y;
-    if (dx * dx + dy * dy < ir2 && in_grid(cx + 1, cy - 1, g.cpr)) out[n++] = (cy - 1) * g.cpr + (cx + 1);
-    return n;
+    const double qx[4] = {tlx, trx, llx, lrx}, qy[4] = {tly, try_, lly, lry};
+    const int ox[4] = {-1, 1, -1, 1}, oy[4] = {1, 1, -1, -1};
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const double dx = qx[d] - x, dy = qy[d] - y;
+        const bool hit = (dx * dx + dy * dy < ir2) && in_grid(cx + ox[d], cy + oy[d], g.cpr);
+        mask |= hit ? (1u << (5 + d)) : 0u;
+        cell[5 + d] = (cy + oy[d]) * g.cpr + (cx + ox[d]);
+    }
+    return mask;
 }
 
 // Every kernel of the binning and of the start-cell test works on a table of
@@ -179,6 +184,8 @@ __device__ int bin_targets(const BinGeom& g, double x, double y, int32_t* out)
 
 // Targets and per-cell counts of every impurity; with `generate` the centres
 // are drawn first (es_rng.h) and stored in x, y.  counts are zero on entry.
+// All of an impurity's counting atomics are issued before their results are
+// used; the targets are stored compacted in candidate order.
 __global__ __launch_bounds__(256) void k_bin_count(const BinJob* jobs)
 {
     const BinJob& J = jobs[blockIdx.y];
@@ -193,15 +200,25 @@ __global__ __launch_bounds__(256) void k_bin_count(const BinJob* jobs)
         px = J.x[i];
         py = J.y[i];
     }
-    const BinGeom g{J.lo, J.size, J.cell_size, J.radius, J.cpr};
-    int32_t t[9];
-    const int k = bin_targets(g, px, py, t);
-    J.s.ntarg[i] = (uint8_t)k;
+    const BinGeom g{J.lo, J.size, J.cell_size, J.radius, 1.0 / J.size, J.cpr};
+    int32_t cell[9];
+    const unsigned int mask = bin_candidates(g, px, py, cell);
     // The count's old value is the entry's slot in its cell (any order: the
     // cell is sorted by impurity id afterwards).
-    for (int m = 0; m < k; m++) {
-        J.s.targets[9 * i + m] = t[m];
-        J.s.slot[9 * i + m] = atomicAdd(&J.s.counts[t[m]], 1);
+    int32_t slot[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) slot[k] = ((mask >> k) & 1u) ? atomicAdd(&J.s.counts[cell[k]], 1) : 0;
+    J.s.ntarg[i] = (uint8_t)__popc(mask);
+    int32_t* const tg = J.s.targets + 9 * i;
+    int32_t* const sl = J.s.slot + 9 * i;
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        if ((mask >> k) & 1u) {
+            tg[n] = cell[k];
+            sl[n] = slot[k];
+            n++;
+        }
     }
 }
 
@@ -765,6 +782,7 @@ struct CoopWave {
     unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
     const double2* ixy[64];          // per lane: its sample's binned field
     float thr[64];                   // per lane: travel-angle threshold of step 2
+    float pe[64], pf[64];            // per lane: travel pseudo-angles of the cell's entry and exit (d_first_cross_time_sel)
     unsigned int nexact[64];         // per lane: cooperative exact solves this cell
     int32_t qidx[kCoopCap];          // queued disk: index in its owner's field
     float qlb[kCoopCap];             // queued disk: lower bound of its travel angles
@@ -888,6 +906,39 @@ __device__ __forceinline__ float d_time_threshold(double w, double t)
     return (t < ES_INF) ? (float)(w * t) + 1e-5f : 3.0e38f;
 }
 
+// The cell's range rx -> ry (entry -> exit, or -> the start angle when there
+// is no next cell) as travel pseudo-angles from the start, for
+// d_first_cross_time_sel: the entry of the first cell is the start itself
+// (0); the end of a last cell is a whole turn (4); (-1, 5) when AngleInRange
+// accepts every angle (details.h:121).  The walk's ranges follow each other
+// from the start and end at or before a whole turn (GetNextCell stops when
+// the next range would contain the start angle), so pe < pf; NaN (nothing
+// decided) when an entry or exit angle lies where asin's slope exceeds the
+// bound, when an entry or exit is within the margin of the start (its
+// pseudo-angle could have wrapped round), or when rx or ry lies outside
+// [0, 2 pi]: the start angle phi can (phi_start = -alpha), and AngleInRange
+// compares raw values, so such a range is not the travelled arc.
+template <bool CW>
+__device__ __forceinline__ void cell_range_pseudo(const Orb& o, const Isect& entry, const Isect& exit, bool avail,
+                                                  bool first, double phi0, float& pe, float& pf)
+{
+    const double rx = entry.ang, ry = avail ? exit.ang : phi0;
+    if (fabs(rx - ry) < 1e-10) {
+        pe = -1.0f;
+        pf = 5.0f;
+        return;
+    }
+    const double lim = kAngleSlopeLimit * o.R;
+    const double evy = entry.y - o.cy, fvy = exit.y - o.cy;
+    const double e = first ? 0.0 : d_travel_pseudo<CW>(o.u0x, o.u0y, entry.x - o.cx, evy);
+    const double f = avail ? d_travel_pseudo<CW>(o.u0x, o.u0y, exit.x - o.cx, fvy) : 4.0;
+    const bool e_ok = first || (fabs(evy) < lim && e > kRangeMargin && e < 4.0 - kRangeMargin);
+    const bool f_ok = !avail || (fabs(fvy) < lim && f > kRangeMargin && f < 4.0 - kRangeMargin);
+    const bool ok = e_ok && f_ok && e < f && rx >= 0.0 && rx <= ES_PI2 && ry >= 0.0 && ry <= ES_PI2;
+    pe = ok ? (float)e : __builtin_nanf("");
+    pf = (float)f;
+}
+
 // GetFirstCrossTime of disk gi of owner `own`'s field, folded into the
 // owner's minimum as `lifetime = (t < lifetime) ? t : lifetime` would
 // (lifetime.h:117-123): NaN and times >= DBL_MAX never lower it, and no solve
@@ -909,7 +960,8 @@ __device__ __forceinline__ void d_coop_solve(CoopWave& L, int own, int gi)
     ot.radd2 = ot.rmin2 = ot.u0x = ot.u0y = 0.0;
     ot.r = 0.0f;
     const double2 p = L.ixy[own][gi];
-    const double t = d_first_cross_time<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own]);
+    const double t = d_first_cross_time_sel<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own], L.orb[10][own] - ot.cx,
+                                                L.orb[11][own] - ot.cy, L.pe[own], L.pf[own]);
     if (t < ES_INF) atomicMin(&L.tmin[own], d_order_key(t));
     atomicAdd(&L.nexact[own], 1u);
 }
@@ -1066,6 +1118,7 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
             avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
             L.orb[8][lane] = entry.ang;
             L.orb[9][lane] = avail ? st.nx.ang : phi0;
+            cell_range_pseudo<CW>(o, entry, st.nx, avail, st.cells == 0, phi0, L.pe[lane], L.pf[lane]);
             if (SCREEN) d_range_window<CW>(phi0, entry.ang, avail ? st.nx.ang : phi0, ds);
             const int ci = entry.cy * g.cpr + entry.cx;
             s = (ci > 0) ? g.cell_end[ci - 1] : 0;
@@ -1126,19 +1179,19 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
                         const double2 p = load_disk(gxy, qi);
                         h = d_disk_screen<CW>(ds, o.cx - p.x, o.cy - p.y, lb);
                     }
+                    int qq = qi;
                     float ql = lb;
                     if (h && lb < first_lb) {  // a new lowest bound: the previous one is queued
-                        const int t = qi;
-                        qi = first_i;
+                        qq = first_i;
                         ql = first_lb;
                         h = first_i >= 0;
                         first_lb = lb;
-                        first_i = t;
+                        first_i = qi;
                     }
                     const unsigned long long m = __ballot(h);
                     if (h) {
                         const int pos = top + __popcll(m & below);
-                        L.qidx[pos] = qi;
+                        L.qidx[pos] = qq;
                         L.qlb[pos] = ql;
                         L.qown[pos] = (uint8_t)lane;
                     }

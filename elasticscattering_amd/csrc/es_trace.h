@@ -114,6 +114,52 @@ __device__ __forceinline__ bool d_in_range(double phi, double rx, double ry)
     return (low < phi && phi < high) || (low < (phi + ES_PI2) && (phi + ES_PI2) < high);
 }
 
+// ------------------------------------------------------ travel pseudo-angles
+//
+// Not reference operations: a cheap order on points of the orbit, used to
+// decide which points need their angle (asin) at all.  The travel angle from
+// a point A to a point X of the orbit (both as vectors from the centre) is
+// the angle swept in the direction of motion, in [0, 2 pi).  Its
+// pseudo-angle, from the dot and cross products of the two vectors, is in
+// [0, 4), increases with it, and changes by at least the change of the
+// travel angle times 1/2 and at most the change itself (slope in [1/2, 1]).
+// So two pseudo-angles more than m apart belong to travel angles more than m
+// apart.
+//
+// The reference's angle of a point (GetAngle, details.h:93-107) is asin of
+// (y - cy) / R, which for a point within an ulp or so of the circle and with
+// |y - cy| < 0.9999 R (asin's slope below 71) is the point's true angle to
+// about 1e-13 rad, and to 1e-10 rad in the |y - cy| <= 1e-10 R branch.  Every
+// comparison below keeps a margin of at least 1e-9 (1e-6 against the FP32
+// cell-range values), so an order decided here is the order the reference's
+// computed angles have; where a margin or the |y - cy| bound fails, nothing
+// is decided and the reference operations run.
+
+constexpr double kTravelMargin = 1e-9;      // between two FP64 pseudo-angles
+constexpr double kRangeMargin = 1e-6;       // against a cell range bound stored as FP32
+constexpr double kAngleSlopeLimit = 0.9999; // |y - cy| / R below this: asin's slope < 71
+
+__device__ __forceinline__ double d_pseudo_angle(double x, double y)
+{
+    const double ax = fabs(x), ay = fabs(y);
+    const double sum = ax + ay;
+    const double r = (sum > 0.0) ? ay / sum : 0.0;  // IEEE: the margins assume an error near 1e-16
+    const bool up = y >= 0.0;
+    const bool plus = up ? (x >= 0.0) : (x < 0.0);
+    const double base = up ? (plus ? 0.0 : 2.0) : (plus ? 2.0 : 4.0);
+    return plus ? base + r : base - r;
+}
+
+// Pseudo-angle of the travel angle from A = (ax, ay) to V = (vx, vy) (vectors
+// from the orbit centre).
+template <bool CW>
+__device__ __forceinline__ double d_travel_pseudo(double ax, double ay, double vx, double vy)
+{
+    const double dot = ax * vx + ay * vy;
+    const double crs = ax * vy - ay * vx;
+    return d_pseudo_angle(dot, CW ? -crs : crs);
+}
+
 // GetFirstCrossTime with GetCrossPoints inlined (details.h:153-206).
 template <bool CW>
 __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, double py, double rx, double ry)
@@ -130,6 +176,68 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
 #pragma unroll 1
     for (int k = 0; k < 2; k++) {
         // point 1: (bx + uy*ys, by + (-ux)*ys); point 2: (bx + (-uy)*ys, by + ux*ys)
+        const double ex = k ? -uy : uy;
+        const double ey = k ? ux : -ux;
+        const double phi_k = d_angle<CW>(bx + ex * ys, by + ey * ys, o);
+        if (d_in_range<CW>(phi_k, rx, ry)) {
+            const double t = esm_div_rcp(d_cross_angle<CW>(o.pa, phi_k), o.w, o.inv_w);
+            tt = (tt < t) ? tt : t;
+        }
+    }
+    return tt;
+}
+
+// d_first_cross_time with the angles taken only where they can matter (same
+// result).  (u0x, u0y): the start point minus the centre; pe, pf: the travel
+// pseudo-angles from the start of the cell's entry and exit points (the
+// range rx -> ry as travelled), pe = -1, pf = 5 when AngleInRange accepts
+// every angle (|rx - ry| < 1e-10), pe NaN when they are not known well
+// enough.  A crossing point is in the range when its travel pseudo-angle p
+// lies in (pe, pf) by the margin, out of it when outside by the margin,
+// undecided otherwise or when p is within the margin of a whole turn (its
+// angle could round to either side of the start).  Of two points in range
+// only the earlier can give the minimum time (the time is the travel angle
+// over w).  Decided points out of range and the later point are not
+// evaluated; an undecided point and the chosen one run the reference
+// operations.
+template <bool CW>
+__device__ __forceinline__ double d_first_cross_time_sel(const Orb& o, double px, double py, double rx, double ry,
+                                                         double u0x, double u0y, float pe, float pf)
+{
+    const double qx = o.cx - px, qy = o.cy - py;
+    const double dist_squared = qx * qx + qy * qy;
+    const double dist = sqrt(dist_squared);
+    const double inv_d = 1.0 / dist;  // RN(1 / (2 dist)) = RN(1 / dist) / 2
+    const double xs = esm_div_rcp(dist_squared + o.R2 - o.r2, 2.0 * dist, 0.5 * inv_d);
+    const double ys = sqrt(o.R2 - xs * xs);
+    const double ux = esm_div_rcp(px - o.cx, dist, inv_d), uy = esm_div_rcp(py - o.cy, dist, inv_d);
+    const double bx = o.cx + ux * xs, by = o.cy + uy * xs;
+    unsigned int need = 3u;  // bit k: point k runs the reference operations
+    {
+        const double lim = kAngleSlopeLimit * o.R;
+        bool ok = (ys == ys) && (pe == pe);
+        double p[2];
+        unsigned int in = 0u, out = 0u;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const double ex = k ? -uy : uy;
+            const double ey = k ? ux : -ux;
+            const double vx = (bx + ex * ys) - o.cx, vy = (by + ey * ys) - o.cy;
+            ok = ok && fabs(vy) < lim;
+            p[k] = d_travel_pseudo<CW>(u0x, u0y, vx, vy);
+            const bool turn_ok = p[k] > kTravelMargin && p[k] < 4.0 - kTravelMargin;
+            in |= (turn_ok && p[k] > (double)pe + kRangeMargin && p[k] < (double)pf - kRangeMargin) ? (1u << k) : 0u;
+            out |= (turn_ok && (p[k] < (double)pe - kRangeMargin || p[k] > (double)pf + kRangeMargin)) ? (1u << k) : 0u;
+        }
+        if (ok && (in | out) == 3u) {
+            need = in;
+            if (in == 3u) need = (p[0] + kTravelMargin < p[1]) ? 1u : (p[1] + kTravelMargin < p[0]) ? 2u : 3u;
+        }
+    }
+    double tt = ES_INF;
+    while (need) {  // one copy of the angle code
+        const int k = __builtin_ctz(need);
+        need &= need - 1u;
         const double ex = k ? -uy : uy;
         const double ey = k ? ux : -ux;
         const double phi_k = d_angle<CW>(bx + ex * ys, by + ey * ys, o);

@@ -139,6 +139,41 @@ static inline v2 mk2(double x, double y) { v2 r; r.x = x; r.y = y; return r; }
 static inline v2i mk2i(int x, int y) { v2i r; r.x = x; r.y = y; return r; }
 static inline double sq(double v) { return v * v; }  /* pow(v, 2) */
 
+#ifdef ORACLE_CHECK_SELECTION
+/*
+ * Check build (liboracle_esm_sel.so), not part of the restatement: the GPU
+ * trace decides from travel pseudo-angles which crossing points of a disk
+ * need their angle (es_trace.h d_first_cross_time_sel, es_kernels.hip
+ * cell_range_pseudo), on margins meant to make that choice never change a
+ * result.  This build runs the same decisions beside the reference
+ * operations and counts: solves, points the GPU would evaluate, solves whose
+ * time would differ, and points decided in (or out of) range that the
+ * reference's AngleInRange puts out (in).  The decisions are restated here
+ * with exact division in the pseudo-angle (the kernel divides in IEEE too).
+ */
+static int64_t sel_cnt[8];
+#define SEL_ADD(k, v) __atomic_fetch_add(&sel_cnt[k], (v), __ATOMIC_RELAXED)
+static const double kSelTravelMargin = 1e-9, kSelRangeMargin = 1e-6, kSelSlopeLimit = 0.9999;
+static double sel_pseudo(double x, double y)
+{
+    const double ax = fabs(x), ay = fabs(y);
+    const double sum = ax + ay;
+    const double r = (sum > 0.0) ? ay / sum : 0.0;
+    const int up = y >= 0.0;
+    const int plus = up ? (x >= 0.0) : (x < 0.0);
+    const double base = up ? (plus ? 0.0 : 2.0) : (plus ? 2.0 : 4.0);
+    return plus ? base + r : base - r;
+}
+static double sel_travel(int cw, double ax, double ay, double vx, double vy)
+{
+    const double dot = ax * vx + ay * vy;
+    const double crs = ax * vy - ay * vx;
+    return sel_pseudo(dot, cw ? -crs : crs);
+}
+static _Thread_local double sel_u0x, sel_u0y;
+static _Thread_local float sel_pe, sel_pf;
+#endif
+
 /* ---------------------------------------------------------------- details.h */
 
 /* details.h:79-83 */
@@ -229,6 +264,42 @@ static double get_first_cross_time(const orc_orbit* orbit, v2 ip, double ir, dou
         const double t = get_cross_angle(orbit->particle_angle, phi2, orbit->clockwise) / w;
         traversal_time = (traversal_time < t) ? traversal_time : t;
     }
+#ifdef ORACLE_CHECK_SELECTION
+    {
+        const double phik[2] = {phi1, phi2};
+        double tk[2] = {C_INF, C_INF};
+        for (int k = 0; k < 2; k++)
+            if (angle_in_range(phik[k], valid_range, orbit->clockwise))
+                tk[k] = get_cross_angle(orbit->particle_angle, phik[k], orbit->clockwise) / w;
+        const double lim = kSelSlopeLimit * orbit->radius;
+        int ok = cp[0] == cp[0] && cp[1] == cp[1] && cp[2] == cp[2] && cp[3] == cp[3] && sel_pe == sel_pe;
+        double pp[2];
+        unsigned in = 0u, out = 0u;
+        for (int k = 0; k < 2; k++) {
+            const double vx = cp[2 * k] - orbit->center.x, vy = cp[2 * k + 1] - orbit->center.y;
+            ok = ok && fabs(vy) < lim;
+            pp[k] = sel_travel(orbit->clockwise, sel_u0x, sel_u0y, vx, vy);
+            const int turn_ok = pp[k] > kSelTravelMargin && pp[k] < 4.0 - kSelTravelMargin;
+            in |= (turn_ok && pp[k] > (double)sel_pe + kSelRangeMargin && pp[k] < (double)sel_pf - kSelRangeMargin) ? (1u << k) : 0u;
+            out |= (turn_ok && (pp[k] < (double)sel_pe - kSelRangeMargin || pp[k] > (double)sel_pf + kSelRangeMargin)) ? (1u << k) : 0u;
+        }
+        unsigned need = 3u;
+        if (ok && (in | out) == 3u) {
+            need = in;
+            if (in == 3u) need = (pp[0] + kSelTravelMargin < pp[1]) ? 1u : (pp[1] + kSelTravelMargin < pp[0]) ? 2u : 3u;
+        }
+        double sel = C_INF;
+        for (int k = 0; k < 2; k++) if ((need >> k) & 1u) sel = (sel < tk[k]) ? sel : tk[k];
+        SEL_ADD(0, 1);
+        SEL_ADD(1, __builtin_popcount(need));
+        if (memcmp(&sel, &traversal_time, sizeof(double)) != 0) SEL_ADD(2, 1);
+        for (int k = 0; k < 2; k++) {
+            const int exact_in = angle_in_range(phik[k], valid_range, orbit->clockwise);
+            if (((in >> k) & 1u) && !exact_in) SEL_ADD(3, 1);
+            if (((out >> k) & 1u) && exact_in) SEL_ADD(4, 1);
+        }
+    }
+#endif
     return traversal_time;
 }
 
@@ -449,6 +520,31 @@ static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is
         orc_intersection entry_point = next_intersection;
         const int next_cell_available = get_next_cell(&p->orbit, p->phi, is, &entry_point, &next_intersection);
         const v2 valid_phi_range = mk2(entry_point.incident_angle, next_cell_available ? next_intersection.incident_angle : p->phi);
+#ifdef ORACLE_CHECK_SELECTION
+        {   /* the cell's range as the GPU trace forms it (cell_range_pseudo) */
+            const orc_orbit* o = &p->orbit;
+            sel_u0x = p->position.x - o->center.x;
+            sel_u0y = p->position.y - o->center.y;
+            const double rx = valid_phi_range.x, ry = valid_phi_range.y;
+            if (fabs(rx - ry) < 1e-10) {
+                sel_pe = -1.0f;
+                sel_pf = 5.0f;
+            } else {
+                const int first = cells == 0;
+                const double lim = kSelSlopeLimit * o->radius;
+                const double evy = entry_point.position.y - o->center.y, fvy = next_intersection.position.y - o->center.y;
+                const double e = first ? 0.0 : sel_travel(o->clockwise, sel_u0x, sel_u0y, entry_point.position.x - o->center.x, evy);
+                const double f = next_cell_available
+                                     ? sel_travel(o->clockwise, sel_u0x, sel_u0y, next_intersection.position.x - o->center.x, fvy)
+                                     : 4.0;
+                const int e_ok = first || (fabs(evy) < lim && e > kSelRangeMargin && e < 4.0 - kSelRangeMargin);
+                const int f_ok = !next_cell_available || (fabs(fvy) < lim && f > kSelRangeMargin && f < 4.0 - kSelRangeMargin);
+                const int ok = e_ok && f_ok && e < f && rx >= 0.0 && rx <= C_PI2 && ry >= 0.0 && ry <= C_PI2;
+                sel_pe = ok ? (float)e : __builtin_nanf("");
+                sel_pf = (float)f;
+            }
+        }
+#endif
 
         const int cell_idx = get_index(entry_point.entering_cell, is->cells_per_row);
         const int impurity_start = (cell_idx > 0) ? b->imp_index[cell_idx - 1] : 0;
@@ -1254,3 +1350,14 @@ EXPORT double orc_m_sin(double x) { return O_SIN(x); }
 EXPORT double orc_m_cos(double x) { return O_COS(x); }
 EXPORT double orc_m_asin(double x) { return O_ASIN(x); }
 EXPORT double orc_m_exp(double x) { return O_EXP(x); }
+
+#ifdef ORACLE_CHECK_SELECTION
+/* Counters of the selection check since the last reset: solves, points the
+   GPU would evaluate, solves whose time would differ, points wrongly decided
+   in range, points wrongly decided out of range. */
+EXPORT void orc_selection_stats(int64_t* out5)
+{
+    for (int k = 0; k < 5; k++) out5[k] = __atomic_load_n(&sel_cnt[k], __ATOMIC_RELAXED);
+}
+EXPORT void orc_selection_reset(void) { memset(sel_cnt, 0, sizeof(sel_cnt)); }
+#endif

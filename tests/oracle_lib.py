@@ -1,13 +1,15 @@
 """ctypes access to the CPU oracle (oracle/es_oracle.c) for the tests.
 
 Test infrastructure: this module and the oracle it loads are the checker, never
-the thing measured or shipped.  Four libraries:
+the thing measured or shipped.  Five libraries:
 
 * ``esm``  — oracle built on the project's correctly rounded es_math.h (the
   GPU's math);
 * ``libm`` — the same restatement on glibc's sin/cos/asin/exp;
 * ``esm_libm_asin`` — es_math.h except glibc's asin (attributes the
   differences between the two builds above to one function);
+* ``esm_sel`` — the esm build that also checks the GPU trace's choice of
+  which crossing points need their angle (orc_selection_stats);
 * ``ref``  — the reference's own kernel headers compiled from the reference
   tree (oracle/_ref/libesref.so, only where that tree was present at build).
 """
@@ -120,6 +122,7 @@ def lib_path(kind):
         "esm": os.path.join(ORACLE_DIR, "liboracle_esm.so"),
         "libm": os.path.join(ORACLE_DIR, "liboracle_libm.so"),
         "esm_libm_asin": os.path.join(ORACLE_DIR, "liboracle_esm_libm_asin.so"),
+        "esm_sel": os.path.join(ORACLE_DIR, "liboracle_esm_sel.so"),
         "ref": os.path.join(ORACLE_DIR, "_ref", "libesref.so"),
     }[kind]
 
