@@ -159,19 +159,28 @@ __global__ __launch_bounds__(kThreads) void k_mlp(MlpArgs a)
     }
     __syncthreads();
 
-    // Layer 1: z1 = x W1^T + b1 (K = 16, one step per 16-unit tile).
+    // Layer 1: z1 = x W1^T + b1 (K = 16, one step per 16-unit tile).  The
+    // weight fragments and biases of all 8 tiles are loaded first: one round
+    // trip to the cache, not one per tile.
     float h1f[8][4];
     {
+        v4s bw[8];
+        float bb[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            bw[t] = ld4(wb + BW1 + (16 * t + lr) * kInP + 4 * lg);
+            bb[t] = prm[OFF_B1 + 16 * t + lr];
+        }
         const v4s av = ld4(&s.xb[(rw + lr) * S16 + 4 * lg]);
         const v4s al = ld4(&s.xl[(rw + lr) * S16 + 4 * lg]);
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const v4s bv = ld4(wb + BW1 + (16 * t + lr) * kInP + 4 * lg);
+            const v4s bv = bw[t];
             v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
             acc = mfma16(av, bv, acc);
             acc = mfma16(al, bv, acc);
             const int col = 16 * t + lr;
-            const float bias = prm[OFF_B1 + col];
+            const float bias = bb[t];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = rw + 4 * lg + r;
@@ -187,27 +196,39 @@ __global__ __launch_bounds__(kThreads) void k_mlp(MlpArgs a)
     }
     __syncthreads();
 
-    // Layer 2: z2 = h1 W2^T + b2 (K = 128).
+    // Layer 2: z2 = h1 W2^T + b2 (K = 128); the next k-step's 8 weight
+    // fragments are loaded while this one's MFMAs run.
     float h2f[8][4];
     {
         v4f acc[8];
+        float bb[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < 8; t++) {
+            acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            bb[t] = prm[OFF_B2 + 16 * t + lr];
+        }
+        v4s bw[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) bw[t] = ld4(wb + BW2 + (16 * t + lr) * kH + 4 * lg);
 #pragma unroll
         for (int ks = 0; ks < 8; ks++) {
+            v4s nx[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) nx[t] = ld4(wb + BW2 + (16 * t + lr) * kH + 16 * ((ks + 1) & 7) + 4 * lg);
             const v4s av = ld4(&s.h1b[(rw + lr) * S128 + 16 * ks + 4 * lg]);
             const v4s al = ld4(&s.h1l[(rw + lr) * S128 + 16 * ks + 4 * lg]);
 #pragma unroll
             for (int t = 0; t < 8; t++) {
-                const v4s bv = ld4(wb + BW2 + (16 * t + lr) * kH + 16 * ks + 4 * lg);
-                acc[t] = mfma16(av, bv, acc[t]);
-                acc[t] = mfma16(al, bv, acc[t]);
+                acc[t] = mfma16(av, bw[t], acc[t]);
+                acc[t] = mfma16(al, bw[t], acc[t]);
             }
+#pragma unroll
+            for (int t = 0; t < 8; t++) bw[t] = nx[t];
         }
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const int col = 16 * t + lr;
-            const float bias = prm[OFF_B2 + col];
+            const float bias = bb[t];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = rw + 4 * lg + r;
@@ -225,13 +246,17 @@ __global__ __launch_bounds__(kThreads) void k_mlp(MlpArgs a)
 
     // Layer 3: y = h2 W3^T + b3 (K = 128, one 16-wide output tile).
     v4f y = {0.0f, 0.0f, 0.0f, 0.0f};
+    {
+        v4s bw[8];
 #pragma unroll
-    for (int ks = 0; ks < 8; ks++) {
-        const v4s av = ld4(&s.h2b[(rw + lr) * S128 + 16 * ks + 4 * lg]);
-        const v4s al = ld4(&s.h2l[(rw + lr) * S128 + 16 * ks + 4 * lg]);
-        const v4s bv = ld4(wb + BW3 + lr * kH + 16 * ks + 4 * lg);
-        y = mfma16(av, bv, y);
-        y = mfma16(al, bv, y);
+        for (int ks = 0; ks < 8; ks++) bw[ks] = ld4(wb + BW3 + lr * kH + 16 * ks + 4 * lg);
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++) {
+            const v4s av = ld4(&s.h2b[(rw + lr) * S128 + 16 * ks + 4 * lg]);
+            const v4s al = ld4(&s.h2l[(rw + lr) * S128 + 16 * ks + 4 * lg]);
+            y = mfma16(av, bw[ks], y);
+            y = mfma16(al, bw[ks], y);
+        }
     }
     const int o = lr;
     const float b3 = prm[OFF_B3 + o];
@@ -289,12 +314,14 @@ __global__ __launch_bounds__(kThreads) void k_mlp(MlpArgs a)
 
     // dz2 = (dy W3) * (1 - h2^2): [64 x 16] x [16 x 128].
     {
+        v4s bw[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) bw[t] = ld4(wb + BW3T + (16 * t + lr) * kOutP + 4 * lg);
         const v4s av = ld4(&s.dyb[(rw + lr) * S16 + 4 * lg]);
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const v4s bv = ld4(wb + BW3T + (16 * t + lr) * kOutP + 4 * lg);
             v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma16(av, bv, acc);
+            acc = mfma16(av, bw[t], acc);
             const int col = 16 * t + lr;
             float sum = 0.0f;
 #pragma unroll
@@ -335,19 +362,25 @@ __global__ __launch_bounds__(kThreads) void k_mlp(MlpArgs a)
             for (int r = 0; r < 4; r++) g[OFF_W2 + (16 * mt + 4 * lg + r) * kH + 16 * t + lr] = acc[t][r];
     }
 
-    // dz1 = (dz2 W2) * (1 - h1^2): [64 x 128] x [128 x 128], W2^T from global memory.
+    // dz1 = (dz2 W2) * (1 - h1^2): [64 x 128] x [128 x 128], W2^T from global
+    // memory, the next k-step's fragments loaded while this one's MFMAs run.
     {
         v4f acc[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        v4s bw[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) bw[t] = ld4(wb + BW2T + (16 * t + lr) * kH + 4 * lg);
 #pragma unroll
         for (int ks = 0; ks < 8; ks++) {
+            v4s nx[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) nx[t] = ld4(wb + BW2T + (16 * t + lr) * kH + 16 * ((ks + 1) & 7) + 4 * lg);
             const v4s av = ld4(&s.h2b[(rw + lr) * S128 + 16 * ks + 4 * lg]);
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const v4s bv = ld4(wb + BW2T + (16 * t + lr) * kH + 16 * ks + 4 * lg);
-                acc[t] = mfma16(av, bv, acc[t]);
-            }
+            for (int t = 0; t < 8; t++) acc[t] = mfma16(av, bw[t], acc[t]);
+#pragma unroll
+            for (int t = 0; t < 8; t++) bw[t] = nx[t];
         }
 #pragma unroll
         for (int t = 0; t < 8; t++) {
