@@ -2,11 +2,12 @@
 the FP64 path, which is bit-exact to the CPU oracle (needs an MI355X).
 
 Stated errors against FP64, checked here on every case: each lifetime t within
-1e-5 t + 3e-8 / omega — the FP32 error of the travel angle plus the FP64
-path's own floor (its angles come from asin(y / R), which near y = +-R is
-off by up to 1e-8 rad, test_oracle_kats.py); a missed or extra hit would be
-an O(1) error.  sigma_xx and sigma_xy within 1e-6 of |sigma|; the work
-counters within 0.1 %.
+1e-5 t + 5e-7 / omega — the FP32 error of the travel angle (relative, but a
+root of a crossing near tangency keeps fewer digits: up to 1.9e-7 rad measured
+over 128 random materials, test_gpu_random_fields.py) plus the FP64 path's own
+floor (its angles come from asin(y / R), which near y = +-R is off by up to
+3e-8 rad, test_oracle_kats.py); a missed or extra hit would be an O(1) error.
+sigma_xx and sigma_xy within 1e-6 of |sigma|; the work counters within 0.1 %.
 """
 import math
 import os
@@ -21,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LT_REL = 1e-5
-ANGLE_FLOOR = 3e-8  # rad
+ANGLE_FLOOR = 5e-7  # rad
 SIGMA_REL = 1e-6
 E_CHARGE, M_EFF = 1.60217646e-19, 5.0 * 9.10938188e-31  # constants.h:23-26
 

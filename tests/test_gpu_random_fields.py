@@ -8,9 +8,12 @@ rotation senses, both passes), each at 48x48 start positions x 4 x 15 phi
 points and three fields.  Every lifetime must equal the oracle's (es_math.h
 build) bit for bit and every metric must be equal; the kernel chooses the
 trace variant (cooperative, screened or not) from the field's density, so
-the draws run through all of them (the last test checks that both the
-screened and the plain cooperative trace ran).  The draws are fixed by the
-seed: a failure names its draw, and the same draw fails the same way again.
+the draws run through all of them (a test checks that both the screened and
+the plain cooperative trace ran).  The same draws run the mixed-precision path
+against FP64 (bit-exact to the oracle by the first test) at its stated bounds:
+each lifetime t within 1e-5 t + 5e-7 / omega, sigma within 1e-6 of |sigma|, the
+work counters within 0.1 %.  The draws are fixed by the seed: a failure names
+its draw, and the same draw fails the same way again.
 """
 import math
 
@@ -25,6 +28,7 @@ pytestmark = pytest.mark.gpu
 N_DRAWS = 128
 METRIC_KEYS = ("cells_passed", "impurities_tested", "particles_inside_impurity", "particles_escaped",
                "particles_at_bound", "max_impurities_tested", "max_cells_passed", "edge_hits", "cross_calls")
+E_CHARGE, M_EFF = 1.60217646e-19, 5.0 * 9.10938188e-31  # constants.h:23-26
 SCREEN_POPULATION = 10.0  # indexed impurities per cell from which the solver screens disks
 _variants = {}  # draw -> True when its field's trace was the screened one
 
@@ -89,6 +93,32 @@ def test_random_material_bit_exact(es, oracle_esm, k):
     if not screened:
         assert skipped == 0, (k, skipped)  # the plain trace solves every candidate disk
     _variants[k] = screened
+
+
+@pytest.mark.parametrize("k", range(N_DRAWS))
+def test_random_material_mixed_within_bounds(es, k):
+    d = _draw(k)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with es.Simulation(48, 15, precision=prec) as sim:
+            sim.generate_impurities(d["seed"], d["rs"], d["ext"], d["radius"], density=d["density"],
+                                    target_cell_population=d["target"], cells_per_row=d["cpr"])
+            for coherent in (0, 1):
+                sim.init_sample(d["tau"], d["alpha"], d["cw"], d["speed"], coherent)
+                announce(f"random draw {k}, {prec}: coherent={coherent} {d}")
+                res[(prec, coherent)] = [sim.compute_sigmas(B, [3.0, 60.0], want_lifetimes=True) for B in d["fields"]]
+    for coherent in (0, 1):
+        for B, (s64, m64, l64), (s32, m32, l32) in zip(d["fields"], res[("fp64", coherent)], res[("mixed", coherent)]):
+            omega = E_CHARGE * B / M_EFF
+            bad = np.flatnonzero(np.abs(l32 - l64) > 1e-5 * np.abs(l64) + 5e-7 / omega)
+            assert bad.size == 0, (k, coherent, B, bad[:5], l64[bad[:5]], l32[bad[:5]])
+            for (x64, y64), (x32, y32) in zip(s64, s32):
+                norm = math.hypot(x64, y64)
+                err = max(abs(x32 - x64), abs(y32 - y64))
+                assert (err <= 1e-6 * norm) if norm > 0 else (err == 0.0), (k, coherent, B, x64, x32, y64, y32)
+            for key in ("cells_passed", "impurities_tested", "cross_calls", "particles_escaped"):
+                assert abs(m32[key] - m64[key]) <= 1e-3 * max(m64[key], 1000), (k, coherent, B, key)
+            assert m32["particles_inside_impurity"] == m64["particles_inside_impurity"]
 
 
 def test_draws_ran_both_trace_variants():
