@@ -49,8 +49,9 @@
 // crossing within
 // error of its cell's range ends, an exit or a hit within error of a full
 // revolution or of the angles m, 2 pi - m above, a hit within the FP64
-// path's angle floor of the start, and a lifetime within 1e-5 of the bound
-// time (lifetime.h:142-146).
+// path's angle floor of the start, a hit or an exit near the top or bottom
+// of the orbit (where the reference's angle of it can be NaN), and a lifetime
+// within 1e-5 of the bound time (lifetime.h:142-146).
 #pragma once
 
 #include "es_trace.h"
@@ -159,6 +160,21 @@ __device__ __forceinline__ void t32_begin(Trace32& st, int c0x, int c0y)
 
 __device__ __forceinline__ bool t32_near(float a, float b, float band) { return fabsf(a - b) <= band; }
 
+// A point of the orbit at the top or bottom (y - cy = +-R) can come out of the
+// reference's FP64 operations a rounding step outside the circle: GetAngle
+// then takes asin of a value above 1 (details.h:93-107) and the angle is NaN,
+// so AngleInRange rejects that crossing (details.h:119-136) and GetNextCell
+// never picks that exit (cell_grid.h:80-89).  The NaN needs the point within
+// ~3e-8 rad of the pole; the FP32 coordinate below is good to ~7e-7 (in units
+// of R), so a hit or an exit within T32_POLE of a pole goes to FP64.
+#define T32_POLE 2e-6f
+
+// x of (X - centre) / R for the orbit point at root r: (X - S) / R - n.
+__device__ __forceinline__ bool t32_near_pole(const Orb32& o, Root32 r)
+{
+    return fabsf(t32_coord(r, o.nx, o.tx) - o.nx) < T32_POLE;
+}
+
 // Exit of the current cell and the reference's accepted travel angles in it.
 struct Cell32 {
     float k_in, band_in;    // entry key and its error band
@@ -186,6 +202,9 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
     // ---- exit of the cell: smallest key after the entry over the 4 edges
     float k_out = 2.0f;
     int e_out = -1;
+    Root32 r_out;           // the exit's root
+    r_out.z = 0.0f;
+    r_out.inv = false;
     float k_second = 2.0f;   // best key of another edge, for the corner test
     float k_flag = 3.0f;     // smallest key of an event FP32 cannot decide
     bool bad = false;
@@ -203,13 +222,29 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
         const float dscale = t_c * t_c + fabsf(2.0f * n_c * cc) + cc * cc;
         if (disc <= T32_REL * dscale) {
             if (disc > -T32_REL * dscale) {
-                // near tangency: undecidable if the touching point lies on the edge
+                // Near tangency: undecidable if a crossing could lie on the
+                // edge.  disc = 1 - d^2 (d: the line's distance from the
+                // centre, in units of R) is the squared half-chord, so any
+                // crossing lies between the touching point and the roots of
+                // the largest discriminant within the band — a half-chord of
+                // up to sqrt(T32_REL dscale) ~ 5e-3 R, far wider than cband.
                 Root32 tp;
                 tp.inv = fabsf(t_c) > fabsf(A);
                 tp.z = tp.inv ? (A / -t_c) : (-t_c / A);
-                const float a = t32_coord(tp, n_o, t_o);
-                const float kt = t32_key(tp);
-                if (a > lo - cband && a < hi + cband && kt > k_in - band_in) k_flag = fminf(k_flag, kt);
+                float amin = t32_coord(tp, n_o, t_o), amax = amin;
+                float kmin = t32_key(tp), kmax = kmin;
+                Root32 rw[2];
+                if (t32_roots(A, t_c, -cc, T32_REL * dscale, rw[0], rw[1])) {
+#pragma unroll
+                    for (int k = 0; k < 2; k++) {
+                        const float ak = t32_coord(rw[k], n_o, t_o), kk = t32_key(rw[k]);
+                        amin = fminf(amin, ak);
+                        amax = fmaxf(amax, ak);
+                        kmin = fminf(kmin, kk);
+                        kmax = fmaxf(kmax, kk);
+                    }
+                }
+                if (amax > lo - cband && amin < hi + cband && kmax > k_in - band_in) k_flag = fminf(k_flag, kmin);
             }
             continue;
         }
@@ -249,6 +284,7 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
                 if (e != e_out) k_second = k_out;
                 k_out = kk;
                 e_out = e;
+                r_out = rt[k];
             } else if (e != e_out && kk < k_second) {
                 k_second = kk;
             }
@@ -258,6 +294,7 @@ __device__ __forceinline__ bool t32_exit(const Orb32& o, const DevGrid& g, Trace
     if (e_out >= 0 && k_second - k_out <= band_out) bad = true;  // two edges at one point
     if (k_flag <= k_out + band_out) bad = true;                 // an undecidable event before the exit
     if (e_out >= 0 && k_out >= 2.0f - 1e-5f) bad = true;        // exit near a full revolution
+    if (e_out >= 0 && t32_near_pole(o, r_out)) bad = true;      // exit where the reference's angle may be NaN
     const int ncx = st.cx + ((e_out == 2) ? 1 : (e_out == 3) ? -1 : 0);
     const int ncy = st.cy + ((e_out == 0) ? 1 : (e_out == 1) ? -1 : 0);
     bool avail = e_out >= 0 && ncx >= 0 && ncx < g.cpr && ncy >= 0 && ncy < g.cpr;
@@ -374,12 +411,26 @@ __device__ __forceinline__ bool t32_disk(const Disk32Ctx& c, double dx, double d
     const float A = (float)((4.0 - 4.0 * bn) + qd);  // 4 (1 - b) + q (small near theta = pi)
     const float gband = T32_REL * (fabsf(gg) + 2.0f * c.rr) + 1e-14f * (float)(ppn + 2.0 * fabs(bn));
     if (fabsf(gg - c.glo) <= gband || fabsf(gg - c.ghi) <= gband) {
-        // within error of tangency: undecidable if the touching point is in range
+        // Within error of tangency: undecidable if a crossing could be in
+        // range.  The crossings lie between the touching point and the roots
+        // of the largest discriminant the band allows, (g - glo)(ghi - g) at
+        // |g - glo| or |g - ghi| = gband; keys spanning the wrap at 0 / 2 are
+        // undecidable too.
         Root32 tp;
         tp.inv = fabsf(2.0f * a) > fabsf(A);
         tp.z = tp.inv ? (A / (2.0f * a)) : (2.0f * a / A);
-        const float kt = t32_key(tp);
-        return !(kt > c.k_in - c.band_in && kt < c.b_hi);
+        float kmin = t32_key(tp), kmax = kmin;
+        Root32 rw[2];
+        if (t32_roots(A, -2.0f * a, q, gband * (c.ghi - c.glo + gband), rw[0], rw[1])) {
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const float kk = t32_key(rw[k]);
+                kmin = fminf(kmin, kk);
+                kmax = fmaxf(kmax, kk);
+            }
+        }
+        if (kmax - kmin > 1.0f) return false;
+        return !(kmax > c.k_in - c.band_in && kmin < c.b_hi);
     }
     if (!(gg > c.glo && gg < c.ghi)) return true;
     cross = true;
@@ -419,6 +470,7 @@ __device__ __forceinline__ bool t32_finish(const Orb32& o, int cap, const Cell32
         (t32_near(best_k, o.km, T32_REL * o.km + T32_END) || t32_near(best_k, o.km2, T32_REL * o.km2 + T32_END)))
         bad = true;  // a hit at the quirk angles
     if (!bad && best_k < T32_END) bad = true;  // a hit within the FP64 floor of the start
+    if (!bad && best_k < 2.0f && t32_near_pole(o, best)) bad = true;  // a hit whose reference angle may be NaN
     if (bad) {
         flags = ES_F_RECOMPUTE;
         theta = 0.0;

@@ -117,3 +117,40 @@ def test_mixed_first_exit_at_the_start(es, material, fields):
     _compare(es, lambda s: s.generate_impurities(m["seed"], 1e-6, 5e-7, m["radius"], density=m["density"],
                                                  target_cell_population=4),
              128, 65, m["tau"], m["alpha"], 1, 7e5, 0, fields, [m["T"]])
+
+
+@pytest.mark.parametrize("sample, field, item", [(42, None, 363126), (2, 34, 592471), (47, 39, 550036)])
+def test_mixed_pole_hit_and_grazing_edges(es, sample, field, item):
+    """Regressions from the 64-sample uneven batch, each one particle that the
+    FP32 trace once ended differently from FP64 (an O(1) lifetime error):
+    * sample 42 at exactly 40 T: a disk's earlier crossing lies at the top of the
+      orbit and comes out of the FP64 operations at (y - cy) / R =
+      1 + 2^-52, so asin gives NaN and the reference drops that crossing
+      (details.h:93-136); hits and exits near the orbit's top or bottom now go
+      to FP64;
+    * sample 2 at 24.9 T and sample 47 at 40 T: the orbit crosses a grid line
+      by 8e-6 R and 1.1e-5 R (a brief excursion into one or two cells); the
+      near-tangent edge's touching point lies off the edge but its crossings,
+      a half-chord sqrt(disc) ~ 4e-3 R away, lie on it; near-tangent edges
+      and disks are now undecidable over that whole span.
+    The whole sample runs at that field; every lifetime within the stated
+    bound, the named one among them."""
+    from elasticscattering_amd.workloads import BATCHED_UNEVEN, batched_samples
+
+    s = batched_samples()[sample]
+    B = 40.0 if field is None else BATCHED_UNEVEN["fields"][field]  # field None: 40 T exactly
+    announce(f"mixed vs fp64: uneven sample {sample} at B={B:.4g}, item {item}")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with es.Simulation(s["positions_per_row"], s["particles_per_quadrant"], precision=prec) as sim:
+            sim.generate_impurities(s["seed"], s["region_size"], s["region_extends"], s["impurity_radius"],
+                                    density=s["density"], target_cell_population=s["target_cell_population"])
+            sim.init_sample(s["tau"], s["alpha"], s["is_clockwise"], s["particle_speed"], False)
+            res[prec] = sim.compute_sigmas(B, s["temperatures"], want_lifetimes=True)
+    (s64, _, l64), (s32, _, l32) = res["fp64"], res["mixed"]
+    omega = E_CHARGE * B / M_EFF
+    assert l64[item] > 0.0
+    bad = np.flatnonzero(np.abs(l32 - l64) > LT_REL * np.abs(l64) + ANGLE_FLOOR / omega)
+    assert bad.size == 0, (sample, B, bad[:5], l64[bad[:5]], l32[bad[:5]])
+    for (x64, y64), (x32, y32) in zip(s64, s32):
+        assert max(abs(x32 - x64), abs(y32 - y64)) <= SIGMA_REL * math.hypot(x64, y64)
