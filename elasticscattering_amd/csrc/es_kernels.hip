@@ -775,10 +775,12 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a)
 // is what shrinks.
 
 constexpr int kCoopStack = 320;  // FP32 trace: < 64 left over + up to 4 new disks per lane per group
-constexpr int kCoopCap = 576;    // FP64 trace: queued disks; solved past kCoopCap - 64 (screened) or - 256
+constexpr int kCoopCap = 448;    // FP64 trace: queued disks; solved past kCoopCap - 64 (screened) or - 256.
+                                 // CoopWave x 4 waves must stay under 53.8 KB for 3 blocks per CU
 
 struct CoopWave {
-    double orb[12][64];              // per lane: cx cy R r2 pa w inv_R inv_w rx ry P0x P0y
+    double orb[14][64];              // per lane: cx cy R r2 pa w inv_R inv_w entry(x y) P0x P0y exit(x y); NaN
+                                     // entry x: the first cell, NaN exit x: no next cell
     unsigned long long tmin[64];     // per lane: order key of the cell's minimum time
     const double2* ixy[64];          // per lane: its sample's binned field
     float thr[64];                   // per lane: travel-angle threshold of step 2
@@ -852,31 +854,41 @@ struct DiskScreen {
 };
 
 // The window of centre pseudo-angles whose crossings can fall in the cell's
-// range (entry rx, exit ry, as AngleInRange takes them, details.h:119-136):
-// its accepted angles lie on the arc travelled from rx to ry.  No window when
-// every angle is accepted (|rx - ry| < 1e-10), the arc is a whole turn, or it
-// ends within dmax + 1e-4 of a full turn.
+// range: its accepted angles lie on the arc travelled from the entry to the
+// exit (AngleInRange of entry and exit angles, details.h:119-136; in a first
+// cell from a raw start angle phi0 < 0 the accepted set is part of that arc or
+// empty).  The travel angles of the two points are taken in FP32 from the
+// start (error < 1e-6 rad, inside the window's 1e-5 margin).  No window in a
+// cell without a next one (a range end at the raw start angle), when the arc
+// wraps or is empty, or when it ends within dmax + 1e-4 of a full turn.
 template <bool CW>
-__device__ __forceinline__ void d_range_window(double pa, double rx, double ry, DiskScreen& ds)
+__device__ __forceinline__ float d_travel_f32(double u0x, double u0y, double vx, double vy)
+{
+    const double dot = u0x * vx + u0y * vy;
+    const double crs = u0x * vy - u0y * vx;
+    float a = atan2f((float)(CW ? -crs : crs), (float)dot);  // (-pi, pi]
+    return (a < 0.0f) ? a + 6.2831855f : a;
+}
+
+template <bool CW>
+__device__ __forceinline__ void d_range_window(const Orb& o, bool first, bool avail, double phi0, double ex, double ey,
+                                               double fx, double fy, DiskScreen& ds)
 {
     ds.lo_pa = -2.0f;
     ds.hi_pa = 5.0f;
-    if (fabs(rx - ry) < 1e-10 || !(ds.dmax < 1.0f)) return;
-    const double low = CW ? ry : rx;
-    double high = CW ? rx : ry;
-    high = (high < low) ? (high + ES_PI2) : high;
-    const double span = high - low;
-    if (!(span < ES_PI2)) return;
-    const double tin = d_cross_angle<CW>(pa, rx);  // travel angle of the entry
-    const double th = tin + span + (double)ds.dmax + 1e-5;
-    if (!(th < ES_PI2 - 1e-4)) return;
-    float s, c;
-    sincosf((float)th, &s, &c);
-    ds.hi_pa = pseudo_angle(c, s) - 1e-6f;
-    const double tl = tin - (double)ds.dmax - 1e-5;
-    if (tl > 0.0) {
-        sincosf((float)tl, &s, &c);
-        ds.lo_pa = pseudo_angle(c, s) + 1e-6f;
+    if (!avail || !(ds.dmax < 1.0f)) return;
+    const float tin = first ? 0.0f : d_travel_f32<CW>(o.u0x, o.u0y, ex - o.cx, ey - o.cy);
+    const float tout = d_travel_f32<CW>(o.u0x, o.u0y, fx - o.cx, fy - o.cy);
+    if (!(tout > tin)) return;
+    const float th = tout + ds.dmax + 1e-5f;
+    if (!(th < 6.2831855f - 1e-4f)) return;
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    ds.hi_pa = pseudo_angle(cs, sn) - 1e-6f;
+    const float tl = tin - ds.dmax - 1e-5f;
+    if (tl > 0.0f) {
+        sincosf(tl, &sn, &cs);
+        ds.lo_pa = pseudo_angle(cs, sn) + 1e-6f;
     }
 }
 
@@ -906,37 +918,70 @@ __device__ __forceinline__ float d_time_threshold(double w, double t)
     return (t < ES_INF) ? (float)(w * t) + 1e-5f : 3.0e38f;
 }
 
-// The cell's range rx -> ry (entry -> exit, or -> the start angle when there
-// is no next cell) as travel pseudo-angles from the start, for
-// d_first_cross_time_sel: the entry of the first cell is the start itself
-// (0); the end of a last cell is a whole turn (4); (-1, 5) when AngleInRange
-// accepts every angle (details.h:121).  The walk's ranges follow each other
-// from the start and end at or before a whole turn (GetNextCell stops when
-// the next range would contain the start angle), so pe < pf; NaN (nothing
-// decided) when an entry or exit angle lies where asin's slope exceeds the
-// bound, when an entry or exit is within the margin of the start (its
-// pseudo-angle could have wrapped round), or when rx or ry lies outside
-// [0, 2 pi]: the start angle phi can (phi_start = -alpha), and AngleInRange
-// compares raw values, so such a range is not the travelled arc.
+// The cell's accepted range (AngleInRange of rx = the entry angle, or the start
+// angle phi0 in the first cell, and ry = the exit angle, or phi0 without a
+// next cell, details.h:119-136) as travel pseudo-angles from the start, for
+// d_first_cross_time_sel, from the entry and exit points alone: (e, f), with
+// e = 0 in the first cell and f = 4 (a whole turn) without a next cell, or
+// (-1, 5) when it accepts every angle (rx = ry = phi0: the first cell has no
+// next one).  A raw start angle phi0 < 0 at a range end makes AngleInRange
+// accept another set than the arc travelled (m = -phi0): clockwise, the first
+// cell takes (0, theta_e) while the exit comes before travel 2 pi - m (the
+// bottom of the orbit, velocity angle 0) and nothing after it, a last cell
+// (theta_i, 2 pi - m); counter-clockwise, the first cell [m, theta_e) (m: the
+// top of the orbit), a last cell (theta_i, 2 pi) past m and nothing before
+// it.  NaN (nothing decided) when a point lies beyond the slope bound or
+// within the margin of the start, of a whole turn, of the other end or of one
+// of those turning points, and in the clockwise last cell entered after
+// 2 pi - m (two ranges).
 template <bool CW>
 __device__ __forceinline__ void cell_range_pseudo(const Orb& o, const Isect& entry, const Isect& exit, bool avail,
                                                   bool first, double phi0, float& pe, float& pf)
 {
-    const double rx = entry.ang, ry = avail ? exit.ang : phi0;
-    if (fabs(rx - ry) < 1e-10) {
+    if (first && !avail) {
         pe = -1.0f;
         pf = 5.0f;
         return;
     }
-    const double lim = kAngleSlopeLimit * o.R;
+    const double lim = kAngleSlopeLimit * o.R, RM = kRangeMargin;
     const double evy = entry.y - o.cy, fvy = exit.y - o.cy;
     const double e = first ? 0.0 : d_travel_pseudo<CW>(o.u0x, o.u0y, entry.x - o.cx, evy);
     const double f = avail ? d_travel_pseudo<CW>(o.u0x, o.u0y, exit.x - o.cx, fvy) : 4.0;
-    const bool e_ok = first || (fabs(evy) < lim && e > kRangeMargin && e < 4.0 - kRangeMargin);
-    const bool f_ok = !avail || (fabs(fvy) < lim && f > kRangeMargin && f < 4.0 - kRangeMargin);
-    const bool ok = e_ok && f_ok && e < f && rx >= 0.0 && rx <= ES_PI2 && ry >= 0.0 && ry <= ES_PI2;
-    pe = ok ? (float)e : __builtin_nanf("");
+    const bool e_ok = first || (fabs(evy) < lim && e > RM && e < 4.0 - RM);
+    const bool f_ok = !avail || (fabs(fvy) < lim && f > RM && f < 4.0 - RM);
+    const bool ok = e_ok && f_ok && f - e > RM;
+    pe = __builtin_nanf("");
     pf = (float)f;
+    if (phi0 < 0.0 && (first || !avail)) {
+        // The turning point's pseudo-angle is measured from the point itself
+        // when it lies at least kTurnMin from the start; nearer, the point is
+        // within rounding of the start, and only the slope bounds of its
+        // travel angle are used (pseudo(2 pi - m) in [4 - m, 4 - m / 2]).
+        const double m = -phi0;
+        const bool far = m >= kTurnMin;
+        if (CW) {
+            const double pb = far ? d_travel_pseudo<CW>(o.u0x, o.u0y, 0.0, -o.R) : 4.0;  // travel 2 pi - m
+            const double pb_lo = far ? pb : 4.0 - m, pb_hi = far ? pb : 4.0 - 0.5 * m;
+            if (first) {
+                if (ok && f < pb_lo - RM) pe = 0.0f;
+                else if (f_ok && f > pb_hi + RM) pe = pf = 5.0f;
+            } else if (far && e_ok && e < pb - RM) {
+                pe = (float)e;
+                pf = (float)pb;
+            }
+        } else if (far) {
+            const double pt = d_travel_pseudo<CW>(o.u0x, o.u0y, 0.0, o.R);  // travel m
+            if (first) {
+                if (ok && f > pt + RM) pe = (float)pt;
+            } else if (e_ok && e > pt + RM) {
+                pe = (float)e;
+            } else if (e_ok && e < pt - RM) {
+                pe = pf = 5.0f;
+            }
+        }
+    } else if (ok) {
+        pe = (float)e;
+    }
 }
 
 // GetFirstCrossTime of disk gi of owner `own`'s field, folded into the
@@ -960,8 +1005,9 @@ __device__ __forceinline__ void d_coop_solve(CoopWave& L, int own, int gi)
     ot.radd2 = ot.rmin2 = ot.u0x = ot.u0y = 0.0;
     ot.r = 0.0f;
     const double2 p = L.ixy[own][gi];
-    const double t = d_first_cross_time_sel<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own], L.orb[10][own] - ot.cx,
-                                                L.orb[11][own] - ot.cy, L.pe[own], L.pf[own]);
+    const double t = d_first_cross_time_sel<CW>(ot, p.x, p.y, L.orb[8][own], L.orb[9][own], L.orb[12][own],
+                                                L.orb[13][own], L.orb[10][own] - ot.cx, L.orb[11][own] - ot.cy,
+                                                L.pe[own], L.pf[own]);
     if (t < ES_INF) atomicMin(&L.tmin[own], d_order_key(t));
     atomicAdd(&L.nexact[own], 1u);
 }
@@ -1115,11 +1161,15 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         bool avail = false;
         if (active) {
             const Isect entry = st.nx;
-            avail = d_next_cell<CW>(o, phi0, g, entry, st.nx, st.edge_hits);
-            L.orb[8][lane] = entry.ang;
-            L.orb[9][lane] = avail ? st.nx.ang : phi0;
-            cell_range_pseudo<CW>(o, entry, st.nx, avail, st.cells == 0, phi0, L.pe[lane], L.pf[lane]);
-            if (SCREEN) d_range_window<CW>(phi0, entry.ang, avail ? st.nx.ang : phi0, ds);
+            const bool first = st.cells == 0;
+            avail = d_next_cell_lazy<CW>(o, phi0, first, g, entry, st.nx, st.edge_hits);
+            const double nan = __builtin_nan("");
+            L.orb[8][lane] = first ? nan : entry.x;
+            L.orb[9][lane] = entry.y;
+            L.orb[12][lane] = avail ? st.nx.x : nan;
+            L.orb[13][lane] = st.nx.y;
+            cell_range_pseudo<CW>(o, entry, st.nx, avail, first, phi0, L.pe[lane], L.pf[lane]);
+            if (SCREEN) d_range_window<CW>(o, first, avail, phi0, entry.x, entry.y, st.nx.x, st.nx.y, ds);
             const int ci = entry.cy * g.cpr + entry.cx;
             s = (ci > 0) ? g.cell_end[ci - 1] : 0;
             n = g.cell_end[ci] - s;

@@ -96,6 +96,21 @@ __device__ __forceinline__ double d_angle(double x, double y, const Orb& o)
     return d_smod_2pi(angle + (CW ? ES_HALF_PI : -ES_HALF_PI));
 }
 
+// d_angle out of line, for the paths that rarely run (one copy of the asin
+// code for all of them instead of one inlined per call site).
+template <bool CW>
+__device__ __attribute__((noinline)) double d_angle_rare(double x, double y, double cx, double cy, double R,
+                                                         double inv_R, double eps_r)
+{
+    Orb o;
+    o.cx = cx;
+    o.cy = cy;
+    o.R = R;
+    o.inv_R = inv_R;
+    o.eps_r = eps_r;
+    return d_angle<CW>(x, y, o);
+}
+
 // GetCrossAngle (details.h:111-116).
 template <bool CW>
 __device__ __forceinline__ double d_cross_angle(double p, double q)
@@ -127,17 +142,19 @@ __device__ __forceinline__ bool d_in_range(double phi, double rx, double ry)
 // apart.
 //
 // The reference's angle of a point (GetAngle, details.h:93-107) is asin of
-// (y - cy) / R, which for a point within an ulp or so of the circle and with
-// |y - cy| < 0.9999 R (asin's slope below 71) is the point's true angle to
-// about 1e-13 rad, and to 1e-10 rad in the |y - cy| <= 1e-10 R branch.  Every
+// (y - cy) / R.  For a point within a few ulps of the circle (every crossing
+// and edge point the reference forms is) and with |y - cy| < (1 - 1e-10) R,
+// asin's slope is below 7.1e4, so that angle is the point's true angle to
+// about 3e-11 rad, and to 1e-10 rad in the |y - cy| <= 1e-10 R branch.  Every
 // comparison below keeps a margin of at least 1e-9 (1e-6 against the FP32
 // cell-range values), so an order decided here is the order the reference's
 // computed angles have; where a margin or the |y - cy| bound fails, nothing
 // is decided and the reference operations run.
 
-constexpr double kTravelMargin = 1e-9;      // between two FP64 pseudo-angles
-constexpr double kRangeMargin = 1e-6;       // against a cell range bound stored as FP32
-constexpr double kAngleSlopeLimit = 0.9999; // |y - cy| / R below this: asin's slope < 71
+constexpr double kTravelMargin = 1e-9;             // between two FP64 pseudo-angles
+constexpr double kRangeMargin = 1e-6;              // against a cell range bound stored as FP32
+constexpr double kAngleSlopeLimit = 0.9999999999;  // |y - cy| / R below this: asin's slope < 7.1e4
+constexpr double kTurnMin = 1e-6;                  // a turning point nearer the start is taken by bounds only
 
 __device__ __forceinline__ double d_pseudo_angle(double x, double y)
 {
@@ -188,21 +205,24 @@ __device__ __forceinline__ double d_first_cross_time(const Orb& o, double px, do
 }
 
 // d_first_cross_time with the angles taken only where they can matter (same
-// result).  (u0x, u0y): the start point minus the centre; pe, pf: the travel
-// pseudo-angles from the start of the cell's entry and exit points (the
-// range rx -> ry as travelled), pe = -1, pf = 5 when AngleInRange accepts
-// every angle (|rx - ry| < 1e-10), pe NaN when they are not known well
-// enough.  A crossing point is in the range when its travel pseudo-angle p
-// lies in (pe, pf) by the margin, out of it when outside by the margin,
-// undecided otherwise or when p is within the margin of a whole turn (its
-// angle could round to either side of the start).  Of two points in range
-// only the earlier can give the minimum time (the time is the travel angle
-// over w).  Decided points out of range and the later point are not
-// evaluated; an undecided point and the chosen one run the reference
-// operations.
+// result).  (u0x, u0y): the start point minus the centre; pe, pf: the cell's
+// accepted range as travel pseudo-angles from the start (cell_range_pseudo:
+// pe = -1, pf = 5 when AngleInRange accepts every angle, pe NaN when not
+// known well enough).  A crossing point is in the range when its travel
+// pseudo-angle p lies in (pe, pf) by the margin, out of it when outside by
+// the margin, undecided otherwise or when p is within the margin of a whole
+// turn (its angle could round to either side of the start).  Of two points
+// in range only the earlier can give the minimum time (the time is the
+// travel angle over w).  Decided points out of range and the later point are
+// not evaluated, and a point decided in range takes its time without
+// AngleInRange; only when something is undecided do both points run the
+// reference operations, with the cell's range (rx, ry) as the reference forms
+// it: the angles of the entry (ex, ey) and exit (fx, fy) points, or the start
+// angle o.pa for a NaN ex (the first cell) or fx (no next cell).
 template <bool CW>
-__device__ __forceinline__ double d_first_cross_time_sel(const Orb& o, double px, double py, double rx, double ry,
-                                                         double u0x, double u0y, float pe, float pf)
+__device__ __forceinline__ double d_first_cross_time_sel(const Orb& o, double px, double py, double ex, double ey,
+                                                         double fx, double fy, double u0x, double u0y, float pe,
+                                                         float pf)
 {
     const double qx = o.cx - px, qy = o.cy - py;
     const double dist_squared = qx * qx + qy * qy;
@@ -213,6 +233,7 @@ __device__ __forceinline__ double d_first_cross_time_sel(const Orb& o, double px
     const double ux = esm_div_rcp(px - o.cx, dist, inv_d), uy = esm_div_rcp(py - o.cy, dist, inv_d);
     const double bx = o.cx + ux * xs, by = o.cy + uy * xs;
     unsigned int need = 3u;  // bit k: point k runs the reference operations
+    bool decided = false;    // every point of need decided in range
     {
         const double lim = kAngleSlopeLimit * o.R;
         bool ok = (ys == ys) && (pe == pe);
@@ -231,17 +252,26 @@ __device__ __forceinline__ double d_first_cross_time_sel(const Orb& o, double px
         }
         if (ok && (in | out) == 3u) {
             need = in;
-            if (in == 3u) need = (p[0] + kTravelMargin < p[1]) ? 1u : (p[1] + kTravelMargin < p[0]) ? 2u : 3u;
+            decided = true;
+            if (in == 3u) {
+                need = (p[0] + kTravelMargin < p[1]) ? 1u : (p[1] + kTravelMargin < p[0]) ? 2u : 3u;
+                decided = need != 3u;
+            }
         }
     }
     double tt = ES_INF;
+    double rx = 0.0, ry = 0.0;
+    if (!decided) {  // the cell's range as the reference forms it (lifetime.h:106-115)
+        rx = (ex == ex) ? d_angle_rare<CW>(ex, ey, o.cx, o.cy, o.R, o.inv_R, o.eps_r) : o.pa;
+        ry = (fx == fx) ? d_angle_rare<CW>(fx, fy, o.cx, o.cy, o.R, o.inv_R, o.eps_r) : o.pa;
+    }
     while (need) {  // one copy of the angle code
         const int k = __builtin_ctz(need);
         need &= need - 1u;
-        const double ex = k ? -uy : uy;
-        const double ey = k ? ux : -ux;
-        const double phi_k = d_angle<CW>(bx + ex * ys, by + ey * ys, o);
-        if (d_in_range<CW>(phi_k, rx, ry)) {
+        const double dx = k ? -uy : uy;
+        const double dy = k ? ux : -ux;
+        const double phi_k = d_angle<CW>(bx + dx * ys, by + dy * ys, o);
+        if (decided || d_in_range<CW>(phi_k, rx, ry)) {
             const double t = esm_div_rcp(d_cross_angle<CW>(o.pa, phi_k), o.w, o.inv_w);
             tt = (tt < t) ? tt : t;
         }
@@ -379,6 +409,144 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
     const bool same_cell = next.cx == last.cx && next.cy == last.cy;
     const bool inside = next.cx >= 0 && next.cx < g.cpr && next.cy >= 0 && next.cy < g.cpr;
     return !same_cell && inside && !d_in_range<CW>(phi0, last.ang, next.ang);
+}
+
+// d_next_cell without the angles where they cannot matter (same exit, same
+// next cell, same availability).  `last.ang` may be NaN: the entry's angle not
+// taken yet (it is only needed below when something is undecided); `first`:
+// the first cell, whose entry is the start point and whose entry angle is
+// phi0 itself.  The exit is the candidate with the smallest travel pseudo-angle
+// from the entry (GetNextCell keeps the smallest dphi, cell_grid.h:80-89),
+// decided when every candidate and the entry lie within the slope bound, the
+// winner is more than the margin from the entry and from a whole turn, and
+// the runner-up more than the margin behind it; its angle is then left NaN
+// in `next.ang`.  "The next arc holds the start angle" (cell_grid.h:177-180)
+// is the exit's travel from the start lying before the entry's (a later cell)
+// — false in a first cell except counter-clockwise from a raw start angle
+// phi0 < 0, where it holds while the exit comes before the top of the orbit
+// (travel -phi0, where the velocity angle is 0) — decided by margins the same
+// way.  Anything undecided takes the reference operations, with the angles
+// they need formed from the same points (the same bits).
+template <bool CW>
+__device__ __forceinline__ bool d_next_cell_lazy(const Orb& o, double phi0, bool first, const DevGrid& g,
+                                                 const Isect& last, Isect& next, int& edge_hits)
+{
+    next = last;
+    next.ang = __builtin_nan("");
+    const double L = g.L;
+    const double dp_eps = 1e-6 * L;
+    CellEdges c;
+    d_cell_edges(o, g, last.cx, last.cy, c);
+    const double llx = c.X0, lly = c.Y0;
+    const double lim = kAngleSlopeLimit * o.R;
+
+    unsigned int cand = 0;  // bit 2 e + k: a point in its segment, not the entry point
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        double base, t;
+        if (!d_edge_chord(o, c, e, base, t)) continue;
+        const double l0 = (e < 2) ? c.X0 : c.Y0, l1 = (e < 2) ? c.X1 : c.Y1;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const double a = k ? (base - t) : (base + t);
+            if (!(a > l0 && a < l1)) continue;
+            edge_hits += 1;
+            double ix, iy;
+            d_edge_xy(c, e, k, base, t, ix, iy);
+            if (fabs(ix - last.x) > dp_eps || fabs(iy - last.y) > dp_eps) cand |= 1u << (2 * e + k);
+        }
+    }
+    // The candidates by travel pseudo-angle from the entry: the smallest and
+    // the runner-up.
+    const unsigned int cand0 = cand;
+    bool ok = first || fabs(last.y - o.cy) < lim;
+    double pb = 5.0, ps = 5.0;
+    while (cand) {
+        const int bit = __builtin_ctz(cand);
+        cand &= cand - 1u;
+        const int e = bit >> 1, k = bit & 1;
+        double base = 0.0, t = 0.0, ix, iy;
+        (void)d_edge_chord(o, c, e, base, t);  // the same operations: the same point
+        d_edge_xy(c, e, k, base, t, ix, iy);
+        const double vy = iy - o.cy;
+        ok = ok && fabs(vy) < lim;
+        const double p = d_travel_pseudo<CW>(last.x - o.cx, last.y - o.cy, ix - o.cx, vy);
+        if (p < pb) {
+            ps = pb;
+            pb = p;
+            next.x = ix;
+            next.y = iy;
+            next.cx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
+            next.cy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
+        } else if (p < ps) {
+            ps = p;
+        }
+    }
+    const bool decided = ok && (cand0 == 0u || (pb > kTravelMargin && pb < 4.0 - kTravelMargin && ps - pb > kTravelMargin));
+    double last_ang = last.ang;
+    if (!decided) {  // the reference's choice: the smallest dphi from the entry's angle
+        next = last;
+        if (last_ang != last_ang) last_ang = d_angle_rare<CW>(last.x, last.y, o.cx, o.cy, o.R, o.inv_R, o.eps_r);
+        double best_dphi = ES_PI2;
+        cand = cand0;
+        while (cand) {
+            const int bit = __builtin_ctz(cand);
+            cand &= cand - 1u;
+            const int e = bit >> 1, k = bit & 1;
+            double base = 0.0, t = 0.0, ix, iy;
+            (void)d_edge_chord(o, c, e, base, t);
+            d_edge_xy(c, e, k, base, t, ix, iy);
+            const double ia = d_angle_rare<CW>(ix, iy, o.cx, o.cy, o.R, o.inv_R, o.eps_r);
+            const double dphi = d_cross_angle<CW>(last_ang, ia);
+            if (dphi < best_dphi) {
+                next.x = ix;
+                next.y = iy;
+                next.cx = last.cx + ((e == 2) ? 1 : (e == 3) ? -1 : 0);
+                next.cy = last.cy + ((e == 0) ? 1 : (e == 1) ? -1 : 0);
+                next.ang = ia;
+                best_dphi = dphi;
+            }
+        }
+    }
+
+    const double delta = 1e-7 * L;
+    const bool is_xlow = (next.x - llx) < delta;
+    const bool is_xtop = (-next.x + L + llx) < delta;
+    const bool is_ylow = (next.y - lly) < delta;
+    const bool is_ytop = (-next.y + L + lly) < delta;
+    int ox = next.cx - last.cx, oy = next.cy - last.cy;
+    if (is_xtop && is_ylow) { ox = 1; oy = -1; }
+    if (is_xlow && is_ylow) { ox = -1; oy = -1; }
+    if (is_xlow && is_ytop) { ox = -1; oy = 1; }
+    if (is_xtop && is_ytop) { ox = 1; oy = 1; }
+    next.cx = last.cx + ox;
+    next.cy = last.cy + oy;
+
+    const bool same_cell = next.cx == last.cx && next.cy == last.cy;
+    const bool inside = next.cx >= 0 && next.cx < g.cpr && next.cy >= 0 && next.cy < g.cpr;
+    if (same_cell || !inside) return false;
+    if (decided) {
+        const double fvy = next.y - o.cy;
+        const double f = d_travel_pseudo<CW>(o.u0x, o.u0y, next.x - o.cx, fvy);
+        bool dec = fabs(fvy) < lim && f > kRangeMargin && f < 4.0 - kRangeMargin;
+        bool held = false;
+        if (first) {
+            if (!CW && phi0 < 0.0) {  // the top of the orbit, travel m = -phi0: measured only from kTurnMin on
+                const double pt = d_travel_pseudo<CW>(o.u0x, o.u0y, 0.0, o.R);
+                dec = dec && -phi0 >= kTurnMin && fabs(f - pt) > kRangeMargin;
+                held = f < pt;
+            }
+        } else {
+            const double evy = last.y - o.cy;
+            const double e = d_travel_pseudo<CW>(o.u0x, o.u0y, last.x - o.cx, evy);
+            dec = dec && fabs(evy) < lim && e > kRangeMargin && e < 4.0 - kRangeMargin && fabs(f - e) > kRangeMargin;
+            held = f < e;
+        }
+        if (dec) return !held;
+        if (last_ang != last_ang) last_ang = d_angle_rare<CW>(last.x, last.y, o.cx, o.cy, o.R, o.inv_R, o.eps_r);
+        next.ang = d_angle_rare<CW>(next.x, next.y, o.cx, o.cy, o.R, o.inv_R, o.eps_r);
+    }
+    return !d_in_range<CW>(phi0, last_ang, next.ang);
 }
 
 // Screening bound (not a reference operation): a lower bound on the travel

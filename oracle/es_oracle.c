@@ -151,9 +151,9 @@ static inline double sq(double v) { return v * v; }  /* pow(v, 2) */
  * reference's AngleInRange puts out (in).  The decisions are restated here
  * with exact division in the pseudo-angle (the kernel divides in IEEE too).
  */
-static int64_t sel_cnt[8];
+static int64_t sel_cnt[16];
 #define SEL_ADD(k, v) __atomic_fetch_add(&sel_cnt[k], (v), __ATOMIC_RELAXED)
-static const double kSelTravelMargin = 1e-9, kSelRangeMargin = 1e-6, kSelSlopeLimit = 0.9999;
+static const double kSelTravelMargin = 1e-9, kSelRangeMargin = 1e-6, kSelSlopeLimit = 0.9999999999, kSelTurnMin = 1e-6;
 static double sel_pseudo(double x, double y)
 {
     const double ax = fabs(x), ay = fabs(y);
@@ -172,6 +172,11 @@ static double sel_travel(int cw, double ax, double ay, double vx, double vy)
 }
 static _Thread_local double sel_u0x, sel_u0y;
 static _Thread_local float sel_pe, sel_pf;
+/* next-cell check: the candidates GetNextCell considers (in their segment and
+   not the entry point), the start angle and whether this is the first cell */
+static _Thread_local double sel_cand[8][2];
+static _Thread_local int sel_ncand, sel_first;
+static _Thread_local double sel_phi0;
 #endif
 
 /* ---------------------------------------------------------------- details.h */
@@ -284,10 +289,16 @@ static double get_first_cross_time(const orc_orbit* orbit, v2 ip, double ir, dou
             out |= (turn_ok && (pp[k] < (double)sel_pe - kSelRangeMargin || pp[k] > (double)sel_pf + kSelRangeMargin)) ? (1u << k) : 0u;
         }
         unsigned need = 3u;
+        int decided = 0;
         if (ok && (in | out) == 3u) {
             need = in;
-            if (in == 3u) need = (pp[0] + kSelTravelMargin < pp[1]) ? 1u : (pp[1] + kSelTravelMargin < pp[0]) ? 2u : 3u;
+            decided = 1;
+            if (in == 3u) {
+                need = (pp[0] + kSelTravelMargin < pp[1]) ? 1u : (pp[1] + kSelTravelMargin < pp[0]) ? 2u : 3u;
+                decided = need != 3u;
+            }
         }
+        if (!decided) SEL_ADD(5, 1);
         double sel = C_INF;
         for (int k = 0; k < 2; k++) if ((need >> k) & 1u) sel = (sel < tk[k]) ? sel : tk[k];
         SEL_ADD(0, 1);
@@ -383,6 +394,13 @@ static void update_best_intersect(const orc_orbit* orbit, v2 pt, v2i next_cell, 
                                   const orc_intersection* last, orc_intersection* closest)
 {
     tl_edge_hits += 1;
+#ifdef ORACLE_CHECK_SELECTION
+    if (different_point(pt, last->position, L) && sel_ncand < 8) {
+        sel_cand[sel_ncand][0] = pt.x;
+        sel_cand[sel_ncand][1] = pt.y;
+        sel_ncand += 1;
+    }
+#endif
     const double incident_angle = get_angle(pt, orbit);
     const double dphi = get_cross_angle(last->incident_angle, incident_angle, orbit->clockwise);
     if (dphi < closest->dphi && different_point(pt, last->position, L)) {
@@ -418,12 +436,75 @@ static int update_first_boundary_intersect(const orc_orbit* orbit, v2i cell_offs
     return i1_valid || i2_valid;
 }
 
+#ifdef ORACLE_CHECK_SELECTION
+/*
+ * The exit and the "next arc holds the start angle" test decided from travel
+ * pseudo-angles, as a trace that takes no angle of the exit point would:
+ * counters 6 / 7 (exit undecided / decided wrongly) and 8 / 9 (the same for
+ * the next cell's availability).
+ */
+static void sel_check_next_cell(const orc_orbit* o, const orc_intersection* last, const orc_intersection* next,
+                                int same_cell, int in_bounds, int avail)
+{
+    const int cw = o->clockwise;
+    const double lim = kSelSlopeLimit * o->radius, m = kSelTravelMargin;
+    const double evx = last->position.x - o->center.x, evy = last->position.y - o->center.y;
+    int ok = sel_first || fabs(evy) < lim;  /* the first cell's entry angle is phi itself */
+    int best = -1;
+    double pb = 5.0, ps = 5.0;
+    for (int j = 0; j < sel_ncand; j++) {
+        const double vx = sel_cand[j][0] - o->center.x, vy = sel_cand[j][1] - o->center.y;
+        ok = ok && fabs(vy) < lim;
+        const double pj = sel_travel(cw, evx, evy, vx, vy);
+        if (pj < pb) {
+            ps = pb;
+            pb = pj;
+            best = j;
+        } else if (pj < ps) {
+            ps = pj;
+        }
+    }
+    const int decided = ok && (sel_ncand == 0 || (pb > m && pb < 4.0 - m && ps - pb > m));
+    if (!decided) {
+        SEL_ADD(6, 1);
+        return;
+    }
+    if (best >= 0 && (memcmp(&sel_cand[best][0], &next->position.x, 8) != 0 ||
+                      memcmp(&sel_cand[best][1], &next->position.y, 8) != 0))
+        SEL_ADD(7, 1);
+    if (best < 0) return;  /* no exit: the same cell, no next cell either way */
+    /* the arc from the entry to the exit holds the start angle */
+    const double fx = next->position.x - o->center.x, fy = next->position.y - o->center.y;
+    const double f = sel_travel(cw, sel_u0x, sel_u0y, fx, fy);
+    int in = 0, dec = fabs(fy) < lim && f > kSelRangeMargin && f < 4.0 - kSelRangeMargin;
+    if (sel_first) {
+        if (!cw && sel_phi0 < 0.0) {  /* counter-clockwise, raw start angle m = -phi < 0: held when theta_e < m */
+            const double pt = sel_travel(cw, sel_u0x, sel_u0y, 0.0, o->radius);  /* the top point: travel m */
+            dec = dec && -sel_phi0 >= kSelTurnMin && fabs(f - pt) > kSelRangeMargin;
+            in = f < pt;
+        }
+    } else {
+        const double e = sel_travel(cw, sel_u0x, sel_u0y, evx, evy);
+        dec = dec && fabs(evy) < lim && e > kSelRangeMargin && e < 4.0 - kSelRangeMargin && fabs(f - e) > kSelRangeMargin;
+        in = f < e;
+    }
+    if (!dec) {
+        SEL_ADD(8, 1);
+        return;
+    }
+    if ((!same_cell && in_bounds && !in) != avail) SEL_ADD(9, 1);
+}
+#endif
+
 /* cell_grid.h:132-181 */
 static int get_next_cell(const orc_orbit* orbit, double phi, const orc_impurity_settings* is,
                          orc_intersection* last, orc_intersection* next)
 {
     last->dphi = C_PI2;
     *next = *last;
+#ifdef ORACLE_CHECK_SELECTION
+    sel_ncand = 0;
+#endif
 
     const double L = is->cell_size;
     const v2 low_left = to_world(last->entering_cell, is->spawn_region_start, L);
@@ -452,8 +533,12 @@ static int get_next_cell(const orc_orbit* orbit, double phi, const orc_impurity_
     }
 
     const int same_cell = next->entering_cell.x == last->entering_cell.x && next->entering_cell.y == last->entering_cell.y;
-    return !same_cell && within_bounds(next->entering_cell, is->cells_per_row) &&
-           !angle_in_range(phi, mk2(last->incident_angle, next->incident_angle), orbit->clockwise);
+    const int avail = !same_cell && within_bounds(next->entering_cell, is->cells_per_row) &&
+                      !angle_in_range(phi, mk2(last->incident_angle, next->incident_angle), orbit->clockwise);
+#ifdef ORACLE_CHECK_SELECTION
+    sel_check_next_cell(orbit, last, next, same_cell, within_bounds(next->entering_cell, is->cells_per_row), avail);
+#endif
+    return avail;
 }
 
 /* --------------------------------------------------------------- lifetime.h */
@@ -518,31 +603,65 @@ static double trace_orbit(const orc_particle* p, const orc_impurity_settings* is
     tl_edge_hits = 0;
     while (1) {
         orc_intersection entry_point = next_intersection;
+#ifdef ORACLE_CHECK_SELECTION
+        sel_u0x = p->position.x - p->orbit.center.x;
+        sel_u0y = p->position.y - p->orbit.center.y;
+        sel_first = cells == 0;
+        sel_phi0 = p->phi;
+#endif
         const int next_cell_available = get_next_cell(&p->orbit, p->phi, is, &entry_point, &next_intersection);
         const v2 valid_phi_range = mk2(entry_point.incident_angle, next_cell_available ? next_intersection.incident_angle : p->phi);
 #ifdef ORACLE_CHECK_SELECTION
-        {   /* the cell's range as the GPU trace forms it (cell_range_pseudo) */
+        {   /* the cell's range as the GPU trace forms it (cell_range_pseudo): from the entry and
+               exit points only; the raw start angle phi < 0 at a range end (first cell, or a cell
+               without a next one) as the accepted sets it gives */
             const orc_orbit* o = &p->orbit;
-            sel_u0x = p->position.x - o->center.x;
-            sel_u0y = p->position.y - o->center.y;
-            const double rx = valid_phi_range.x, ry = valid_phi_range.y;
-            if (fabs(rx - ry) < 1e-10) {
-                sel_pe = -1.0f;
+            const int first = cells == 0, av = next_cell_available;
+            const double phi0 = p->phi, RM = kSelRangeMargin;
+            if (first && !av) {
+                sel_pe = -1.0f;  /* rx = ry = phi: every angle */
                 sel_pf = 5.0f;
             } else {
-                const int first = cells == 0;
                 const double lim = kSelSlopeLimit * o->radius;
-                const double evy = entry_point.position.y - o->center.y, fvy = next_intersection.position.y - o->center.y;
-                const double e = first ? 0.0 : sel_travel(o->clockwise, sel_u0x, sel_u0y, entry_point.position.x - o->center.x, evy);
-                const double f = next_cell_available
-                                     ? sel_travel(o->clockwise, sel_u0x, sel_u0y, next_intersection.position.x - o->center.x, fvy)
-                                     : 4.0;
-                const int e_ok = first || (fabs(evy) < lim && e > kSelRangeMargin && e < 4.0 - kSelRangeMargin);
-                const int f_ok = !next_cell_available || (fabs(fvy) < lim && f > kSelRangeMargin && f < 4.0 - kSelRangeMargin);
-                const int ok = e_ok && f_ok && e < f && rx >= 0.0 && rx <= C_PI2 && ry >= 0.0 && ry <= C_PI2;
-                sel_pe = ok ? (float)e : __builtin_nanf("");
-                sel_pf = (float)f;
+                const double evx = entry_point.position.x - o->center.x, evy = entry_point.position.y - o->center.y;
+                const double fvx = next_intersection.position.x - o->center.x, fvy = next_intersection.position.y - o->center.y;
+                const double e = first ? 0.0 : sel_travel(o->clockwise, sel_u0x, sel_u0y, evx, evy);
+                const double f = av ? sel_travel(o->clockwise, sel_u0x, sel_u0y, fvx, fvy) : 4.0;
+                const int e_ok = first || (fabs(evy) < lim && e > RM && e < 4.0 - RM);
+                const int f_ok = !av || (fabs(fvy) < lim && f > RM && f < 4.0 - RM);
+                const int ok = e_ok && f_ok && f - e > RM;
+                float pe = __builtin_nanf(""), pf = (float)f;
+                if (phi0 < 0.0 && (first || !av)) {
+                    const double mm = -phi0;
+                    const int far = mm >= kSelTurnMin;  /* nearer: the turning point within rounding of the start */
+                    if (o->clockwise) {
+                        const double pb = far ? sel_travel(1, sel_u0x, sel_u0y, 0.0, -o->radius) : 4.0;  /* travel 2 pi - m */
+                        const double pb_lo = far ? pb : 4.0 - mm, pb_hi = far ? pb : 4.0 - 0.5 * mm;
+                        if (first) {
+                            if (ok && f < pb_lo - RM) pe = 0.0f;                            /* (0, theta_e) */
+                            else if (f_ok && f > pb_hi + RM) pe = pf = 5.0f;                /* nothing */
+                        } else if (far && e_ok && e < pb - RM) {
+                            pe = (float)e;                                                  /* (theta_i, 2 pi - m) */
+                            pf = (float)pb;
+                        }
+                    } else if (far) {
+                        const double pt = sel_travel(0, sel_u0x, sel_u0y, 0.0, o->radius);  /* travel m */
+                        if (first) {
+                            if (ok && f > pt + RM) pe = (float)pt;                          /* [m, theta_e) */
+                        } else if (e_ok && e > pt + RM) {
+                            pe = (float)e;                                                  /* (theta_i, 2 pi) */
+                        } else if (e_ok && e < pt - RM) {
+                            pe = pf = 5.0f;                                                 /* nothing */
+                        }
+                    }
+                } else if (ok) {
+                    pe = (float)e;
+                }
+                sel_pe = pe;
+                sel_pf = pf;
             }
+            SEL_ADD(10, sel_pe == sel_pe ? 0 : 1);
+            SEL_ADD(11, 1);
         }
 #endif
 
@@ -1358,6 +1477,12 @@ EXPORT double orc_m_exp(double x) { return O_EXP(x); }
 EXPORT void orc_selection_stats(int64_t* out5)
 {
     for (int k = 0; k < 5; k++) out5[k] = __atomic_load_n(&sel_cnt[k], __ATOMIC_RELAXED);
+}
+/* All 16 counters: 5 solves left undecided, 6 / 7 exits undecided / decided wrongly,
+   8 / 9 next-cell availability undecided / wrong, 10 cell ranges undecided, 11 cell steps. */
+EXPORT void orc_selection_stats16(int64_t* out16)
+{
+    for (int k = 0; k < 16; k++) out16[k] = __atomic_load_n(&sel_cnt[k], __ATOMIC_RELAXED);
 }
 EXPORT void orc_selection_reset(void) { memset(sel_cnt, 0, sizeof(sel_cnt)); }
 #endif
