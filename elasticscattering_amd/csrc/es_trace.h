@@ -161,7 +161,13 @@ __device__ __forceinline__ double d_pseudo_angle(double x, double y)
 {
     const double ax = fabs(x), ay = fabs(y);
     const double sum = ax + ay;
-    const double r = (sum > 0.0) ? ay / sum : 0.0;  // IEEE: the margins assume an error near 1e-16
+    // ay / sum by v_rcp_f64 and one Newton step (relative error below 1e-14;
+    // an IEEE division is three times the instructions).  Every use compares
+    // pseudo-angles on margins of 1e-9 or more.  sum >= |A| |V| is a normal
+    // number for any orbit the solver accepts.
+    double q = __builtin_amdgcn_rcp(sum);
+    q = fma(fma(-sum, q, 1.0), q, q);
+    const double r = (sum > 0.0) ? ay * q : 0.0;
     const bool up = y >= 0.0;
     const bool plus = up ? (x >= 0.0) : (x < 0.0);
     const double base = up ? (plus ? 0.0 : 2.0) : (plus ? 2.0 : 4.0);
