@@ -19,6 +19,12 @@ range that AngleInRange (details.h:119-136) puts out of (into) it, no decided
 exit may differ from GetNextCell's (cell_grid.h:132-181) and no decided
 availability from the reference's.  The GPU side is the bit-exact lifetime
 tests (test_gpu_parity.py, test_gpu_configs.py, test_gpu_random_fields.py).
+
+The check computes a pseudo-angle's quotient by IEEE division; the GPU takes
+it from v_rcp_f64 and one Newton step (relative error below 1e-14).  Every
+decision compares pseudo-angles on margins of at least 1e-9, so the two can
+differ only for values within ~1e-14 of a margin, where both choices are
+still on the safe side of the reference's rounding errors (~1e-13).
 """
 import ctypes as C
 import math
