@@ -936,7 +936,8 @@ __device__ __forceinline__ float d_time_threshold(double w, double t)
 // 2 pi - m (two ranges).
 template <bool CW>
 __device__ __forceinline__ void cell_range_pseudo(const Orb& o, const Isect& entry, const Isect& exit, bool avail,
-                                                  bool first, double phi0, float& pe, float& pf)
+                                                  bool first, double phi0, double e_known, double f_known, float& pe,
+                                                  float& pf)
 {
     if (first && !avail) {
         pe = -1.0f;
@@ -945,8 +946,10 @@ __device__ __forceinline__ void cell_range_pseudo(const Orb& o, const Isect& ent
     }
     const double lim = kAngleSlopeLimit * o.R, RM = kRangeMargin;
     const double evy = entry.y - o.cy, fvy = exit.y - o.cy;
-    const double e = first ? 0.0 : d_travel_pseudo<CW>(o.u0x, o.u0y, entry.x - o.cx, evy);
-    const double f = avail ? d_travel_pseudo<CW>(o.u0x, o.u0y, exit.x - o.cx, fvy) : 4.0;
+    // e_known / f_known: the same pseudo-angles when the cell step took them
+    // (NaN otherwise)
+    const double e = first ? 0.0 : (e_known == e_known) ? e_known : d_travel_pseudo<CW>(o.u0x, o.u0y, entry.x - o.cx, evy);
+    const double f = !avail ? 4.0 : (f_known == f_known) ? f_known : d_travel_pseudo<CW>(o.u0x, o.u0y, exit.x - o.cx, fvy);
     const bool e_ok = first || (fabs(evy) < lim && e > RM && e < 4.0 - RM);
     const bool f_ok = !avail || (fabs(fvy) < lim && f > RM && f < 4.0 - RM);
     const bool ok = e_ok && f_ok && f - e > RM;
@@ -1162,13 +1165,14 @@ __device__ __forceinline__ void trace_coop_body(const TraceArgs& a)
         if (active) {
             const Isect entry = st.nx;
             const bool first = st.cells == 0;
-            avail = d_next_cell_lazy<CW>(o, phi0, first, g, entry, st.nx, st.edge_hits);
+            double e_known, f_known;
+            avail = d_next_cell_lazy<CW>(o, phi0, first, g, entry, st.nx, st.edge_hits, e_known, f_known);
             const double nan = __builtin_nan("");
             L.orb[8][lane] = first ? nan : entry.x;
             L.orb[9][lane] = entry.y;
             L.orb[12][lane] = avail ? st.nx.x : nan;
             L.orb[13][lane] = st.nx.y;
-            cell_range_pseudo<CW>(o, entry, st.nx, avail, first, phi0, L.pe[lane], L.pf[lane]);
+            cell_range_pseudo<CW>(o, entry, st.nx, avail, first, phi0, e_known, f_known, L.pe[lane], L.pf[lane]);
             if (SCREEN) d_range_window<CW>(o, first, avail, phi0, entry.x, entry.y, st.nx.x, st.nx.y, ds);
             const int ci = entry.cy * g.cpr + entry.cx;
             s = (ci > 0) ? g.cell_end[ci - 1] : 0;

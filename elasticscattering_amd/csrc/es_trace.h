@@ -436,8 +436,10 @@ __device__ __forceinline__ bool d_next_cell(const Orb& o, double phi0, const Dev
 // they need formed from the same points (the same bits).
 template <bool CW>
 __device__ __forceinline__ bool d_next_cell_lazy(const Orb& o, double phi0, bool first, const DevGrid& g,
-                                                 const Isect& last, Isect& next, int& edge_hits)
+                                                 const Isect& last, Isect& next, int& edge_hits, double& pe_start,
+                                                 double& pf_start)
 {
+    pe_start = pf_start = __builtin_nan("");
     next = last;
     next.ang = __builtin_nan("");
     const double L = g.L;
@@ -535,6 +537,7 @@ __device__ __forceinline__ bool d_next_cell_lazy(const Orb& o, double phi0, bool
     if (decided) {
         const double fvy = next.y - o.cy;
         const double f = d_travel_pseudo<CW>(o.u0x, o.u0y, next.x - o.cx, fvy);
+        pf_start = f;
         bool dec = fabs(fvy) < lim && f > kRangeMargin && f < 4.0 - kRangeMargin;
         bool held = false;
         if (first) {
@@ -546,6 +549,7 @@ __device__ __forceinline__ bool d_next_cell_lazy(const Orb& o, double phi0, bool
         } else {
             const double evy = last.y - o.cy;
             const double e = d_travel_pseudo<CW>(o.u0x, o.u0y, last.x - o.cx, evy);
+            pe_start = e;
             dec = dec && fabs(evy) < lim && e > kRangeMargin && e < 4.0 - kRangeMargin && fabs(f - e) > kRangeMargin;
             held = f < e;
         }
